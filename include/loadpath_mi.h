@@ -1,0 +1,165 @@
+/*
+ * loadpath_mi.h — C-ABI of the MI355X (gfx950) voxel-SIMP library liblpmi.so.
+ *
+ * This is the drop-in boundary for loadpath's voxel SIMP hot path. Every
+ * entry point replaces one reference interface, cited as file:line into the
+ * loadpath repository. The Python host package (loadpath_amd) binds these
+ * with ctypes; INTEGRATION.md shows the binding a loadpath maintainer adds.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only. Node-grid fields are component-major
+ *    (3, nx+1, ny+1, nz+1), C order, exactly mg.GridElasticity.to_grid's
+ *    layout (mg.py:495-499); several right-hand sides are stacked in front:
+ *    (nrhs, 3, nx+1, ny+1, nz+1). Cell fields are (nx, ny, nz), C order
+ *    (mg.py:508-511). nx, ny, nz are the even-padded cell counts
+ *    (mg.py:438-441).
+ *  - `device_ptr` != 0: pointers are device pointers (e.g. a torch tensor's
+ *    data_ptr()) used in place; 0: host pointers, the library copies.
+ *  - Every call returns an int status: LP_OK, LP_NOT_CONVERGED,
+ *    LP_SINGULAR, or a negative error; lp_last_error() gives the message
+ *    (thread-local). A handle is not re-entrant: one host thread per handle,
+ *    as mgops.py:29-30 requires of a GridElasticity.
+ */
+#ifndef LOADPATH_MI_H
+#define LOADPATH_MI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LP_OK 0
+#define LP_NOT_CONVERGED 1   /* mg.py:944-946 RuntimeError("linear solve did not converge") */
+#define LP_SINGULAR 2        /* mg.py:580-586 singular coarsest factor -> strict hierarchy */
+#define LP_ERR_HIP (-1)
+#define LP_ERR_ARG (-2)
+#define LP_ERR_STATE (-3)
+
+/* precision of the V-cycle (the outer CG is always FP64) */
+#define LP_PREC_FP64 0       /* FP64 everywhere ("plain FP64") */
+#define LP_PREC_FP32 1       /* FP64 CG + FP32 V-cycle: the reference split (mg.py:28-32, gpu.py:483-521) */
+#define LP_PREC_MIXED 2      /* FP64 refinement residual + FP32 inner PCG + FP16-MFMA fine Ke.U */
+
+#define LP_SMOOTH_CHEBYSHEV 0  /* mg.py:796-835 */
+#define LP_SMOOTH_JACOBI 1     /* damped Jacobi (configs' V(2,2), omega 0.6) */
+
+typedef struct lp_options {
+    double rtol;        /* ||r|| <= rtol ||b||, GridElasticity(rtol=) mg.py:428 */
+    int maxiter;        /* GridElasticity(maxiter=) */
+    int coarse_cells;   /* GridElasticity(coarse_cells=) mg.py:556 */
+    int degree;         /* Chebyshev degree / Jacobi sweeps, GridElasticity(degree=) */
+    int smoother;       /* LP_SMOOTH_* */
+    double omega;       /* damped-Jacobi weight */
+    int precision;      /* LP_PREC_* */
+    int max_rhs;        /* right-hand sides solved together (batched PCG) */
+    int lmax_iters;     /* power iterations for lambda_max (mg.py:771, 12) */
+    int max_levels;     /* 0 = no cap; else stop coarsening at this many levels */
+} lp_options;
+
+typedef struct lp_grid lp_grid;
+
+/* ---- element ----------------------------------------------------------- */
+/* fem.hex8_ke(nu, element) (fem.py:220-224): unit-cube 24x24 at E = 1.
+ * element: 0 = "im" (Wilson bubbles condensed), 1 = "plain" trilinear. */
+int lp_hex8_ke(double nu, int element, double* ke_out /* 576 */);
+
+/* ---- grid handle: mg.GridElasticity (mg.py:411-511) --------------------- */
+/* fixed_grid: (3, nx+1, ny+1, nz+1) uint8 (mg.py:447-451); ke: 576 doubles
+ * already scaled by the pitch (mg.py:446). */
+int lp_grid_create(lp_grid** out, int nx, int ny, int nz, const uint8_t* fixed_grid,
+                   const double* ke, const lp_options* opt);
+void lp_grid_destroy(lp_grid* g);
+
+/* GridElasticity.set_moduli(e) (mg.py:515-591): per-cell moduli on the
+ * padded cell grid; rebuilds every level (Galerkin coarsening, masks,
+ * lambda_max, coarsest Cholesky) on the device. Returns LP_SINGULAR if the
+ * coarsest factor fails even in the strict hierarchy.
+ * flags: bit 0 = e is a device pointer; bit 1 = defer lambda_max (the
+ * caller then calls lp_estimate_lmax per level). */
+int lp_set_moduli(lp_grid* g, const double* e_cells, int flags);
+
+/* GridElasticity._estimate_lmax (mg.py:771-794): 12 power iterations of
+ * D^-1 A at `level`, times 1.3. start: host (3, nodes) start vector, or NULL
+ * for the level's last vector (warm start) or a fixed hash start. */
+int lp_estimate_lmax(lp_grid* g, int level, const double* start);
+
+/* Hierarchy description: levels, their cell dims (3 ints each) and flags. */
+int lp_grid_info(lp_grid* g, int* nlevels, int* dims, int* strict, double* lmax);
+
+/* _Level.apply (mg.py:317-346) / mgops.apply_masked (mgops.py:247-261):
+ * out = m*K(m*u) + (1-m)*u at `level`. dtype_bytes 8: the FP64 operator
+ * (level 0 only), 4: the FP32 V-cycle copy. */
+int lp_apply(lp_grid* g, int level, int dtype_bytes, const void* u, void* out, int nrhs, int device_ptr);
+
+/* GridElasticity.precondition (mg.py:862-864): one V-cycle, FP64 in/out. */
+int lp_precondition(lp_grid* g, const double* r, double* z, int nrhs, int device_ptr);
+
+/* GridElasticity._pcg / solve_many on node grids (mg.py:872-965): b, x
+ * (nrhs, 3, nodes) FP64, already masked; warm != 0 starts from x (the
+ * energy-optimal warm start, mg.py:881-896). iters/converged per rhs. */
+int lp_solve(lp_grid* g, int nrhs, const double* b, double* x, int warm, int device_ptr,
+             int* iters, int* converged);
+
+/* GridElasticity.strain_energy (mg.py:975-988) / femops.element_energy
+ * (femops.py:236-242): se (nrhs, nx, ny, nz) = u_c^T Ke u_c. */
+int lp_strain_energy(lp_grid* g, int nrhs, const double* u, double* se, int device_ptr);
+
+/* Level data for parity checks: what = 0 mask bits (uint8, per node),
+ * 1 dinv (double, 3 x nodes), 2 cell dof indices (int64, (ncells, 24), the
+ * index arithmetic every apply kernel uses; fem.HexMesh.edof on the grid),
+ * 3 held bits (uint8 per node: alive and fixed; alive = mask | held). */
+int lp_level_data(lp_grid* g, int level, int what, void* out);
+
+/* mg.prolong / mg.restrict (mg.py:84-132) on device fields of 3 components:
+ * coarse (3, cx, cy, cz) <-> fine (3, 2cx-1, 2cy-1, 2cz-1), FP64. */
+int lp_prolong(const double* coarse, double* fine, int cx, int cy, int cz, int device_ptr);
+int lp_restrict(const double* fine, double* coarse, int cx, int cy, int cz, int device_ptr);
+
+/* ---- SIMP loop on the device (benchmarks/reference.py:214-257 and
+ *      simp.py:1143-1189, 1897-1904, 2247-2381) ---------------------------- */
+typedef struct lp_simp lp_simp;
+
+#define LP_FILTER_TOP3D 0     /* reach ceil(r)-1, row sums (reference.py:188-211) */
+#define LP_FILTER_PADDED 1    /* reach floor(r), full kernel sum (simp.py:1143-1189) */
+
+typedef struct lp_simp_stats {
+    double compliance;    /* sum_i sum_c E_c ce_c of the iteration's design */
+    double change;        /* max |xnew - x| */
+    double volume;        /* mean xphys after the update */
+    int cg_iters;         /* max PCG iterations over the load cases */
+    int oc_evals;         /* bisection evaluations */
+    double seconds;
+} lp_simp_stats;
+
+/* loads: (nrhs, 3, nodes) FP64 node-grid load vectors (masked inside).
+ * active: (nx, ny, nz) uint8 design cells or NULL (passive cells stay 0). */
+int lp_simp_create(lp_simp** out, lp_grid* g, int nrhs, const double* loads, const uint8_t* active,
+                   double volfrac, double rmin, int filter_kind, double penal, double emin, double move,
+                   int warm_start);
+void lp_simp_destroy(lp_simp* s);
+/* n OC iterations; stats_out holds n records. */
+int lp_simp_iterate(lp_simp* s, int n, lp_simp_stats* stats_out);
+/* what: 0 x, 1 xphys, 2 dc (filtered), 3 ce (summed over cases): (nx,ny,nz) doubles */
+int lp_simp_get(lp_simp* s, int what, double* out);
+int lp_simp_set_x(lp_simp* s, const double* x);
+/* displacements of the last solve, (nrhs, 3, nodes) doubles */
+int lp_simp_displacements(lp_simp* s, double* out);
+
+/* density filter alone: out = H v (unnormalised cone stencil on the cell
+ * grid, zero outside), simp.GridFilter.__matmul__ (simp.py:1164-1172). */
+int lp_filter_apply(int nx, int ny, int nz, double radius, int reach, const double* v, double* out,
+                    int device_ptr);
+
+/* ---- diagnostics -------------------------------------------------------- */
+const char* lp_last_error(void);
+int lp_version(void);
+/* average device time (ms) of the most recent timed kernel calls: see
+ * lp_time_apply; used by bench.py for the roofline figures. */
+int lp_time_apply(lp_grid* g, int level, int dtype_bytes, int nrhs, int reps, double* ms_per_apply);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LOADPATH_MI_H */

@@ -1,0 +1,225 @@
+// Internal declarations shared by the liblpmi.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/loadpath_mi.h"
+
+namespace lp {
+
+void set_error(const std::string& msg);
+
+#define LP_HIP(call)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            ::lp::set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " \
+                            + __FILE__ + ":" + std::to_string(__LINE__) + ": " #call); \
+            throw ::lp::HipFailure();                                                \
+        }                                                                            \
+    } while (0)
+
+struct HipFailure {};
+struct ArgFailure {};
+
+// Owning device allocation.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n) {
+        if (n <= bytes && p) return;
+        release();
+        if (n == 0) return;
+        LP_HIP(hipMalloc(&p, n));
+        bytes = n;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+enum LevelKind { LK_FINE = 0, LK_WEIGHTS = 1, LK_EXPLICIT = 2 };
+
+constexpr int kMaxLevels = 16;
+constexpr int kBlock = 256;
+
+// Geometry of one level: cells (nx, ny, nz), nodes (nx+1, ny+1, nz+1).
+struct Geom {
+    int nx = 0, ny = 0, nz = 0;
+    int Nx = 0, Ny = 0, Nz = 0;
+    size_t ncell = 0, nn = 0;
+    void set(int x, int y, int z) {
+        nx = x; ny = y; nz = z;
+        Nx = x + 1; Ny = y + 1; Nz = z + 1;
+        ncell = (size_t)x * y * z;
+        nn = (size_t)Nx * Ny * Nz;
+    }
+};
+
+struct Level {
+    int kind = LK_FINE;
+    Geom g;
+    Geom fine;                  // finer level's geometry (weights kind reads its E)
+    DBuf mask;                  // uint8 per node, bit c = component c free and live
+    DBuf dinv64, dinv32;        // 3*nn
+    DBuf e64, e32;              // fine: cell moduli
+    DBuf patch_idx;             // weights: int per cell, -1 or slot
+    DBuf patch64, patch32;      // weights: (npatch, 576)
+    int npatch = 0;
+    DBuf k64, k32;              // explicit: (576, ncell)
+    double lmax = 0.0;
+    long long n_active = 0;
+    // V-cycle work vectors (elem size 8 or 4) for max_rhs right-hand sides
+    DBuf vb, vx, vr, vd, vd2, vt;
+    DBuf lmax_vec;              // power-iteration vector (warm start)
+    DBuf held;                  // uint8 per node, bit c = component c held and alive (strict / patch)
+    bool lmax_warm = false;
+};
+
+struct Grid {
+    lp_options opt{};
+    Geom g0;
+    double ke[576];
+    DBuf ke64, ke32;            // device copies of Ke
+    DBuf mq64, mq32;            // 8 x 576 first-coarse-level matrices P_k^T Ke P_k
+    DBuf pk;                    // 8 x 576 child prolongations (double)
+    DBuf fixed;                 // uint8 per node, bit c = component c held
+    int nlevels = 0;
+    Level lv[kMaxLevels];
+    bool strict = false;
+    bool built = false;
+    hipStream_t stream = nullptr;
+    // coarsest dense inverse
+    int ncoarse = 0;            // dofs of the coarsest level (3 * nodes)
+    DBuf cinv;                  // ncoarse^2 doubles
+    DBuf cwork;                 // factor workspace
+    // CG vectors (FP64), max_rhs each
+    DBuf x, r, p, ap, z, b;
+    DBuf rv, zv;                // V-cycle precision copies of r and z at level 0
+    // reductions
+    DBuf partials;              // doubles
+    size_t partials_cap = 0;
+    DBuf scal;                  // per-rhs scalar block (CgScal)
+    DBuf status_h;              // unused placeholder
+    void* host_pinned = nullptr;
+    size_t host_pinned_bytes = 0;
+    // staging for host pointers
+    DBuf stage_a, stage_b;
+    int max_rhs = 1;
+    int vbytes() const { return opt.precision == LP_PREC_FP64 ? 8 : 4; }
+};
+
+// Per right-hand-side CG scalars, device resident.
+struct CgScal {
+    double rz, pap, alpha, beta, rr, bnorm, tol, tmp;
+    int active, converged, failed, iters;
+};
+
+// ---- element.cpp -------------------------------------------------------
+void hex8_ke_host(double nu, int element, double* ke);
+void child_prolongations_host(double* pk /* 8*576 */);
+
+// ---- apply.hip ---------------------------------------------------------
+// Epilogue selectors for the level operator y = A u at each node.
+enum EpiKind {
+    EPI_STORE = 0,        // out = y
+    EPI_STORE_DOT = 1,    // out = y; partial += u . y          (p . Ap)
+    EPI_CHEB = 2,         // x += d; r -= y; d2 = c1 d + c2 dinv r
+    EPI_RESID_SCALED = 3, // r = b - y; d = s dinv r
+    EPI_RESID_MASK = 4,   // out = (b - y) * m
+    EPI_JACOBI = 5,       // out = u + w dinv (b - y)
+    EPI_POWER = 6,        // out = dinv * y * m; partial += u . out
+    EPI_RESID = 7,        // out = b - y; partial += out . out (double)
+};
+
+struct EpiArgs {
+    int kind = EPI_STORE;
+    void* out = nullptr;
+    const void* b = nullptr;
+    void* x = nullptr;
+    void* r = nullptr;
+    const void* d = nullptr;
+    void* d2 = nullptr;
+    const void* dinv = nullptr;
+    double c1 = 0, c2 = 0;
+    double* partials = nullptr;   // per (rhs, block)
+    const CgScal* scal = nullptr; // skip inactive rhs when set
+    const double* scale_dev = nullptr;  // optional per-rhs scale read on device
+};
+
+// Launch y = A_level u with epilogue, for nrhs vectors of stride vstride.
+// Returns the number of blocks per rhs (for partial sums).
+int launch_apply(const Grid& G, int level, int elem_bytes, const void* u, size_t vstride, int nrhs,
+                 const EpiArgs& epi, hipStream_t s);
+int apply_blocks(const Grid& G, int level);
+void launch_apply_fine(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs,
+                       const EpiArgs& E, hipStream_t s);
+void launch_apply_weights(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs,
+                          const EpiArgs& E, hipStream_t s);
+void launch_apply_explicit(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs,
+                           const EpiArgs& E, hipStream_t s);
+
+// ---- vec.hip -------------------------------------------------------------
+void ensure_partials(Grid& G, size_t n);
+// sum partials[rhs*nb + i] -> dst[rhs] (double)
+void launch_reduce(const double* partials, int nb, int nrhs, double* dst, size_t dst_stride_bytes,
+                   hipStream_t s);
+int dot_blocks(size_t n);
+void launch_dot(const void* a, const void* b, int elem_bytes, size_t n, size_t stride, int nrhs,
+                double* partials, const CgScal* scal, hipStream_t s);
+void launch_cast(const void* src, int src_bytes, void* dst, int dst_bytes, size_t n, size_t stride,
+                 int nrhs, hipStream_t s);
+void launch_fill(void* dst, int elem_bytes, size_t n, double v, hipStream_t s);
+void launch_scale_by_dev(void* v, int elem_bytes, size_t n, size_t stride, int nrhs,
+                         const double* num, size_t num_stride, int mode, hipStream_t s);
+void launch_cheb_start(const void* b, const void* dinv, double s_, void* x, void* r, void* d,
+                       int elem_bytes, size_t n, size_t stride, int nrhs, hipStream_t st);
+void launch_axpy_inplace(void* x, const void* d, int elem_bytes, size_t n, size_t stride, int nrhs,
+                         hipStream_t st);
+void launch_jacobi_start(const void* b, const void* dinv, double w, void* x, int elem_bytes,
+                         size_t n, size_t stride, int nrhs, hipStream_t st);
+void launch_cg_step(Grid& G, int nrhs, hipStream_t s);   // x += a p; r -= a ap; partial rr
+void launch_xpby(Grid& G, int nrhs, hipStream_t s);      // p = z + beta p
+// finalize kernels on CgScal
+void launch_cg_finalize(Grid& G, int nrhs, int stage, int nb, hipStream_t s);
+// warm start: which 0 = partials of |b - s Ax|^2 (Ax in G.ap), 1 = apply the decision
+void launch_warm(Grid& G, int nrhs, int which, hipStream_t s);
+
+// ---- transfer.hip --------------------------------------------------------
+void launch_restrict_masked(const void* rf, const Geom& fine, const uint8_t* cmask, void* rc,
+                            const Geom& coarse, int elem_bytes, int nrhs, size_t fstride,
+                            size_t cstride, hipStream_t s);
+void launch_prolong_add(const void* xc, const uint8_t* cmask, const Geom& coarse, const uint8_t* fmask,
+                        void* xf, const Geom& fine, int elem_bytes, int nrhs, size_t cstride,
+                        size_t fstride, hipStream_t s);
+void launch_prolong_plain(const double* c, double* f, int cx, int cy, int cz, hipStream_t s);
+void launch_restrict_plain(const double* f, double* c, int cx, int cy, int cz, hipStream_t s);
+
+// ---- hierarchy.hip -------------------------------------------------------
+int build_hierarchy(Grid& G);             // returns LP_OK or LP_SINGULAR
+void level_cell_dofs(const Grid& G, int level, int64_t* out_dev);
+
+// ---- dense.hip -----------------------------------------------------------
+// Factor the SPD n x n matrix a (lower Cholesky, in place) and overwrite it
+// with its inverse. Returns false on a non-positive pivot.
+bool dense_spd_inverse(double* a, int n, double* work, hipStream_t s);
+void launch_dense_matvec(const double* ainv, int n, const void* b, void* x, int elem_bytes, int nrhs,
+                         size_t stride, hipStream_t s);
+
+// ---- solver.hip -----------------------------------------------------------
+void vcycle(Grid& G, int level, int nrhs);           // on lv[level].vb -> lv[level].vx
+void pcg(Grid& G, int nrhs, bool warm, int* iters, int* converged);
+
+}  // namespace lp

@@ -1,0 +1,723 @@
+// The grid handle, the V-cycle and the batched PCG, and the C-ABI around them.
+//
+// Reference: mg.GridElasticity (mg.py:411-988): _smooth (Chebyshev,
+// mg.py:796-835), _vcycle (mg.py:837-860), precondition (mg.py:862-864),
+// _pcg (mg.py:872-926), solve_many / _solve with the STALL -> strict fallback
+// (mg.py:928-965), _estimate_lmax (mg.py:771-794), strain_energy
+// (mg.py:975-988). The reference's GPU path (gpu.TorchMG, gpu.py:280-521)
+// syncs to the host for every dot product and solves the coarsest level on
+// the host; here alpha, beta and |r| live on the device and the host reads
+// one small status block per CG iteration.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "lp_internal.h"
+
+namespace lp {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+namespace {
+
+// fixed hash start vector in [-0.5, 0.5) for lambda_max when none is given
+template <typename T>
+__global__ void k_hash_start(T* v, const uint8_t* __restrict__ mask, size_t nn, unsigned seed) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * nn) return;
+    unsigned long long h = (unsigned long long)i * 0x9E3779B97F4A7C15ull + seed * 0xD1B54A32D192ED03ull;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 29;
+    const double r = (double)(h >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    const int c = (int)(i / nn);
+    v[i] = ((mask[i % nn] >> c) & 1) ? (T)r : T(0);
+}
+
+template <typename T>
+__global__ void k_mask_vec(T* v, const uint8_t* __restrict__ mask, size_t nn) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * nn) return;
+    const int c = (int)(i / nn);
+    if (!((mask[i % nn] >> c) & 1)) v[i] = T(0);
+}
+
+template <typename T>
+__global__ void k_from_double_masked(const double* __restrict__ src, T* dst, const uint8_t* __restrict__ mask, size_t nn) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * nn) return;
+    const int c = (int)(i / nn);
+    const T v = (T)src[i];
+    dst[i] = ((mask[i % nn] >> c) & 1) ? v * T(1) : T(0);
+}
+
+// se[rhs][cell] = u_c^T Ke u_c (FP64), one thread per cell
+__global__ void k_strain_energy(const double* __restrict__ u, const double* __restrict__ ke, Geom g, double* se,
+                                size_t vstride) {
+    const size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int rhs = blockIdx.y;
+    if (cell >= g.ncell) return;
+    const int Z = (int)(cell % g.nz), Y = (int)((cell / g.nz) % g.ny), X = (int)(cell / ((size_t)g.nz * g.ny));
+    const double* uu = u + rhs * vstride;
+    double ue[24];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const size_t node = ((size_t)(X + (b & 1)) * g.Ny + (Y + ((b >> 1) & 1))) * g.Nz + (Z + ((b >> 2) & 1));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ue[3 * b + c] = uu[c * g.nn + node];
+    }
+    double acc = 0;
+#pragma unroll
+    for (int i = 0; i < 24; ++i) {
+        double t = 0;
+#pragma unroll
+        for (int j = 0; j < 24; ++j) t += ke[i * 24 + j] * ue[j];
+        acc += ue[i] * t;
+    }
+    se[rhs * g.ncell + cell] = acc;
+}
+
+inline unsigned nblk(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+void swap_buf(DBuf& a, DBuf& b) {
+    std::swap(a.p, b.p);
+    std::swap(a.bytes, b.bytes);
+}
+
+int api_guard_code(int code) { return code; }
+
+template <typename F> int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const HipFailure&) {
+        return LP_ERR_HIP;
+    } catch (const ArgFailure&) {
+        return LP_ERR_ARG;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return LP_ERR_STATE;
+    }
+}
+
+}  // namespace
+
+// ---- lambda_max ------------------------------------------------------------------
+static double estimate_lmax_level(Grid& G, int i, const double* start_host) {
+    Level& L = G.lv[i];
+    const int eb = G.vbytes();
+    const size_t n = 3 * L.g.nn;
+    hipStream_t s = G.stream;
+    L.lmax_vec.alloc(n * eb);
+    if (start_host) {
+        DBuf tmp;
+        tmp.alloc(n * 8);
+        LP_HIP(hipMemcpyAsync(tmp.p, start_host, n * 8, hipMemcpyHostToDevice, s));
+        if (eb == 8)
+            hipLaunchKernelGGL(k_from_double_masked<double>, dim3(nblk(n)), dim3(kBlock), 0, s, tmp.as<double>(),
+                               L.lmax_vec.as<double>(), L.mask.as<uint8_t>(), L.g.nn);
+        else
+            hipLaunchKernelGGL(k_from_double_masked<float>, dim3(nblk(n)), dim3(kBlock), 0, s, tmp.as<double>(),
+                               L.lmax_vec.as<float>(), L.mask.as<uint8_t>(), L.g.nn);
+        LP_HIP(hipGetLastError());
+        LP_HIP(hipStreamSynchronize(s));
+    } else if (L.lmax_warm) {
+        if (eb == 8)
+            hipLaunchKernelGGL(k_mask_vec<double>, dim3(nblk(n)), dim3(kBlock), 0, s, L.lmax_vec.as<double>(),
+                               L.mask.as<uint8_t>(), L.g.nn);
+        else
+            hipLaunchKernelGGL(k_mask_vec<float>, dim3(nblk(n)), dim3(kBlock), 0, s, L.lmax_vec.as<float>(),
+                               L.mask.as<uint8_t>(), L.g.nn);
+        LP_HIP(hipGetLastError());
+    } else {
+        if (eb == 8)
+            hipLaunchKernelGGL(k_hash_start<double>, dim3(nblk(n)), dim3(kBlock), 0, s, L.lmax_vec.as<double>(),
+                               L.mask.as<uint8_t>(), L.g.nn, (unsigned)i);
+        else
+            hipLaunchKernelGGL(k_hash_start<float>, dim3(nblk(n)), dim3(kBlock), 0, s, L.lmax_vec.as<float>(),
+                               L.mask.as<uint8_t>(), L.g.nn, (unsigned)i);
+        LP_HIP(hipGetLastError());
+    }
+    const int its = G.opt.lmax_iters > 0 ? G.opt.lmax_iters : 12;
+    DBuf sc, av;
+    sc.alloc(2 * sizeof(double) * its);
+    av.alloc(n * eb);
+    const void* dinv = eb == 8 ? L.dinv64.p : L.dinv32.p;
+    const int nbd = dot_blocks(n);
+    const int nba = apply_blocks(G, i);
+    ensure_partials(G, std::max(nbd, nba));
+    double* S = sc.as<double>();
+    for (int it = 0; it < its; ++it) {
+        launch_dot(L.lmax_vec.p, L.lmax_vec.p, eb, n, n, 1, G.partials.as<double>(), nullptr, s);
+        launch_reduce(G.partials.as<double>(), nbd, 1, S + 2 * it, 0, s);
+        launch_scale_by_dev(L.lmax_vec.p, eb, n, n, 1, S + 2 * it, 0, 1, s);
+        EpiArgs E;
+        E.kind = EPI_POWER;
+        E.out = av.p;
+        E.dinv = dinv;
+        E.partials = G.partials.as<double>();
+        launch_apply(G, i, eb, L.lmax_vec.p, n, 1, E, s);
+        launch_reduce(G.partials.as<double>(), nba, 1, S + 2 * it + 1, 0, s);
+        swap_buf(L.lmax_vec, av);
+    }
+    std::vector<double> h(2 * its);
+    LP_HIP(hipMemcpyAsync(h.data(), S, h.size() * 8, hipMemcpyDeviceToHost, s));
+    LP_HIP(hipStreamSynchronize(s));
+    L.lmax_warm = true;
+    double lam = 1.0;
+    for (int it = 0; it < its; ++it) {
+        if (h[2 * it] == 0.0) return 2.0;   // the reference returns 2.0 when the vector vanishes
+        lam = h[2 * it + 1];
+    }
+    return 1.3 * std::max(lam, 1e-12);
+}
+
+// ---- smoothers and V-cycle ----------------------------------------------------------
+static void smooth(Grid& G, int i, int nrhs, bool from_zero) {
+    Level& L = G.lv[i];
+    const int eb = G.vbytes();
+    const size_t n = 3 * L.g.nn;
+    hipStream_t s = G.stream;
+    const void* dinv = eb == 8 ? L.dinv64.p : L.dinv32.p;
+    if (G.opt.smoother == LP_SMOOTH_JACOBI) {
+        int sweeps = G.opt.degree;
+        if (from_zero) {
+            launch_jacobi_start(L.vb.p, dinv, G.opt.omega, L.vx.p, eb, n, n, nrhs, s);
+            --sweeps;
+        }
+        for (int k = 0; k < sweeps; ++k) {
+            EpiArgs E;
+            E.kind = EPI_JACOBI;
+            E.out = L.vt.p;
+            E.b = L.vb.p;
+            E.dinv = dinv;
+            E.c1 = G.opt.omega;
+            launch_apply(G, i, eb, L.vx.p, n, nrhs, E, s);
+            swap_buf(L.vx, L.vt);
+        }
+        return;
+    }
+    const double hi = L.lmax, lo = hi / 30.0;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+    const double sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    if (from_zero) {
+        launch_cheb_start(L.vb.p, dinv, 1.0 / theta, L.vx.p, L.vr.p, L.vd.p, eb, n, n, nrhs, s);
+    } else {
+        EpiArgs E;
+        E.kind = EPI_RESID_SCALED;
+        E.b = L.vb.p;
+        E.r = L.vr.p;
+        E.d2 = L.vd.p;
+        E.dinv = dinv;
+        E.c1 = 1.0 / theta;
+        launch_apply(G, i, eb, L.vx.p, n, nrhs, E, s);
+    }
+    for (int it = 0; it < G.opt.degree; ++it) {
+        const bool last = it + 1 == G.opt.degree;
+        const double rho_new = 1.0 / (2.0 * sigma - rho);
+        if (!last) {
+            EpiArgs E;
+            E.kind = EPI_CHEB;
+            E.x = L.vx.p;
+            E.r = L.vr.p;
+            E.d = L.vd.p;
+            E.d2 = L.vd2.p;
+            E.dinv = dinv;
+            E.c1 = rho_new * rho;
+            E.c2 = 2.0 * rho_new / delta;
+            launch_apply(G, i, eb, L.vd.p, n, nrhs, E, s);
+            swap_buf(L.vd, L.vd2);
+        } else {
+            launch_axpy_inplace(L.vx.p, L.vd.p, eb, n, n, nrhs, s);
+        }
+        rho = rho_new;
+    }
+}
+
+void vcycle(Grid& G, int i, int nrhs) {
+    Level& L = G.lv[i];
+    const int eb = G.vbytes();
+    const size_t n = 3 * L.g.nn;
+    hipStream_t s = G.stream;
+    if (i == G.nlevels - 1) {
+        if (G.ncoarse > 0) {
+            launch_dense_matvec(G.cinv.as<double>(), G.ncoarse, L.vb.p, L.vx.p, eb, nrhs, n, s);
+        } else {
+            smooth(G, i, nrhs, true);
+            smooth(G, i, nrhs, false);
+        }
+        return;
+    }
+    smooth(G, i, nrhs, true);
+    Level& C = G.lv[i + 1];
+    const size_t nc = 3 * C.g.nn;
+    {
+        EpiArgs E;
+        E.kind = EPI_RESID_MASK;
+        E.out = L.vt.p;
+        E.b = L.vb.p;
+        launch_apply(G, i, eb, L.vx.p, n, nrhs, E, s);
+    }
+    launch_restrict_masked(L.vt.p, L.g, C.mask.as<uint8_t>(), C.vb.p, C.g, eb, nrhs, n, nc, s);
+    vcycle(G, i + 1, nrhs);
+    launch_prolong_add(C.vx.p, C.mask.as<uint8_t>(), C.g, L.mask.as<uint8_t>(), L.vx.p, L.g, eb, nrhs, nc, n, s);
+    smooth(G, i, nrhs, false);
+}
+
+// z = M r for the FP64 CG vectors r -> z
+static void precondition(Grid& G, const double* r, double* z, int nrhs) {
+    const int eb = G.vbytes();
+    Level& L = G.lv[0];
+    const size_t n = 3 * L.g.nn;
+    launch_cast(r, 8, L.vb.p, eb, n, n, nrhs, G.stream);
+    vcycle(G, 0, nrhs);
+    launch_cast(L.vx.p, eb, z, 8, n, n, nrhs, G.stream);
+}
+
+static void read_scal(Grid& G, int nrhs, std::vector<CgScal>& out) {
+    out.resize(nrhs);
+    LP_HIP(hipMemcpyAsync(G.host_pinned, G.scal.p, nrhs * sizeof(CgScal), hipMemcpyDeviceToHost, G.stream));
+    LP_HIP(hipStreamSynchronize(G.stream));
+    std::memcpy(out.data(), G.host_pinned, nrhs * sizeof(CgScal));
+}
+
+// Batched PCG on G.b -> G.x (FP64). Returns per-rhs iterations and flags.
+void pcg(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
+    hipStream_t s = G.stream;
+    const size_t n = 3 * G.g0.nn;
+    const int nbd = dot_blocks(n);
+    const int nba = apply_blocks(G, 0);
+    ensure_partials(G, (size_t)nrhs * std::max(nbd, nba));
+    double* P = G.partials.as<double>();
+    CgScal* S = G.scal.as<CgScal>();
+    launch_dot(G.b.p, G.b.p, 8, n, n, nrhs, P, nullptr, s);
+    launch_cg_finalize(G, nrhs, 3, nbd, s);
+    if (warm) {
+        EpiArgs E;
+        E.kind = EPI_STORE_DOT;
+        E.out = G.ap.p;
+        E.partials = P;
+        launch_apply(G, 0, 8, G.x.p, n, nrhs, E, s);
+        launch_cg_finalize(G, nrhs, 5, nba, s);
+        launch_dot(G.b.p, G.x.p, 8, n, n, nrhs, P, nullptr, s);
+        launch_cg_finalize(G, nrhs, 6, nbd, s);
+        launch_warm(G, nrhs, 0, s);
+        launch_cg_finalize(G, nrhs, 7, nbd, s);
+        launch_warm(G, nrhs, 1, s);
+    } else {
+        launch_fill(G.x.p, 8, n * nrhs, 0.0, s);
+        launch_cast(G.b.p, 8, G.r.p, 8, n, n, nrhs, s);
+    }
+    precondition(G, G.r.as<double>(), G.z.as<double>(), nrhs);
+    launch_cast(G.z.p, 8, G.p.p, 8, n, n, nrhs, s);
+    launch_dot(G.r.p, G.z.p, 8, n, n, nrhs, P, S, s);
+    launch_cg_finalize(G, nrhs, 4, nbd, s);
+    std::vector<CgScal> h;
+    for (int it = 1; it <= G.opt.maxiter; ++it) {
+        EpiArgs E;
+        E.kind = EPI_STORE_DOT;
+        E.out = G.ap.p;
+        E.partials = P;
+        E.scal = S;
+        launch_apply(G, 0, 8, G.p.p, n, nrhs, E, s);
+        launch_cg_finalize(G, nrhs, 0, nba, s);
+        launch_cg_step(G, nrhs, s);
+        launch_cg_finalize(G, nrhs, 1, nbd, s);
+        read_scal(G, nrhs, h);
+        bool any = false;
+        for (int k = 0; k < nrhs; ++k) any |= h[k].active != 0;
+        if (!any) break;
+        precondition(G, G.r.as<double>(), G.z.as<double>(), nrhs);
+        launch_dot(G.r.p, G.z.p, 8, n, n, nrhs, P, S, s);
+        launch_cg_finalize(G, nrhs, 2, nbd, s);
+        launch_xpby(G, nrhs, s);
+    }
+    read_scal(G, nrhs, h);
+    for (int k = 0; k < nrhs; ++k) {
+        iters[k] = h[k].iters + (h[k].failed ? 1 : 0);
+        converged[k] = h[k].converged;
+    }
+}
+
+static void estimate_all_lmax(Grid& G) {
+    for (int i = 0; i < G.nlevels; ++i) G.lv[i].lmax = estimate_lmax_level(G, i, nullptr);
+}
+
+static int set_moduli_impl(Grid& G, bool defer) {
+    int rc = build_hierarchy(G);
+    if (rc == LP_SINGULAR) {
+        if (G.strict) return LP_SINGULAR;
+        G.strict = true;
+        rc = build_hierarchy(G);
+        if (rc != LP_OK) return rc;
+    }
+    G.built = true;
+    if (!defer && G.opt.smoother == LP_SMOOTH_CHEBYSHEV) estimate_all_lmax(G);
+    return LP_OK;
+}
+
+}  // namespace lp
+
+using namespace lp;
+
+struct lp_grid : public Grid {};
+
+namespace {
+
+void* to_device(Grid& G, DBuf& stage, const void* p, size_t bytes, bool device_ptr) {
+    if (device_ptr) return const_cast<void*>(p);
+    stage.alloc(bytes);
+    LP_HIP(hipMemcpyAsync(stage.p, p, bytes, hipMemcpyHostToDevice, G.stream));
+    return stage.p;
+}
+
+void from_device(Grid& G, void* host, const void* dev, size_t bytes) {
+    LP_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, G.stream));
+    LP_HIP(hipStreamSynchronize(G.stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lp_last_error(void) { return g_err.c_str(); }
+int lp_version(void) { return 1; }
+
+int lp_grid_create(lp_grid** out, int nx, int ny, int nz, const uint8_t* fixed_grid, const double* ke,
+                   const lp_options* opt) {
+    return guarded([&]() -> int {
+        if (!out || nx < 1 || ny < 1 || nz < 1 || !fixed_grid || !ke || !opt) {
+            set_error("lp_grid_create: bad argument");
+            return LP_ERR_ARG;
+        }
+        if ((nx > 1 && nx % 2) || (ny > 1 && ny % 2) || (nz > 1 && nz % 2)) {
+            set_error("lp_grid_create: cell counts must be even (pad as mg.py:438-441)");
+            return LP_ERR_ARG;
+        }
+        lp_grid* G = new lp_grid();
+        try {
+            G->opt = *opt;
+            if (G->opt.max_rhs < 1) G->opt.max_rhs = 1;
+            if (G->opt.degree < 1) G->opt.degree = 2;
+            if (G->opt.maxiter < 1) G->opt.maxiter = 500;
+            if (G->opt.precision == LP_PREC_MIXED) G->opt.precision = LP_PREC_FP32;
+            G->max_rhs = G->opt.max_rhs;
+            G->g0.set(nx, ny, nz);
+            LP_HIP(hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking));
+            std::memcpy(G->ke, ke, sizeof(G->ke));
+            const size_t nn = G->g0.nn;
+            // fixed bits per node
+            std::vector<uint8_t> fb(nn, 0);
+            for (int c = 0; c < 3; ++c)
+                for (size_t i = 0; i < nn; ++i)
+                    if (fixed_grid[c * nn + i]) fb[i] |= (uint8_t)(1 << c);
+            G->fixed.alloc(nn);
+            LP_HIP(hipMemcpy(G->fixed.p, fb.data(), nn, hipMemcpyHostToDevice));
+            std::vector<float> kf(576);
+            for (int i = 0; i < 576; ++i) kf[i] = (float)ke[i];
+            G->ke64.alloc(576 * 8);
+            G->ke32.alloc(576 * 4);
+            LP_HIP(hipMemcpy(G->ke64.p, ke, 576 * 8, hipMemcpyHostToDevice));
+            LP_HIP(hipMemcpy(G->ke32.p, kf.data(), 576 * 4, hipMemcpyHostToDevice));
+            // M_q = P_q^T Ke P_q (mg.py:626)
+            std::vector<double> pk(8 * 576), mq(8 * 576, 0.0);
+            child_prolongations_host(pk.data());
+            for (int q = 0; q < 8; ++q) {
+                const double* P = pk.data() + q * 576;
+                double t[576];
+                for (int a = 0; a < 24; ++a)
+                    for (int j = 0; j < 24; ++j) {
+                        double v = 0;
+                        for (int b = 0; b < 24; ++b) v += ke[a * 24 + b] * P[b * 24 + j];
+                        t[a * 24 + j] = v;
+                    }
+                for (int i = 0; i < 24; ++i)
+                    for (int j = 0; j < 24; ++j) {
+                        double v = 0;
+                        for (int a = 0; a < 24; ++a) v += P[a * 24 + i] * t[a * 24 + j];
+                        mq[q * 576 + i * 24 + j] = v;
+                    }
+            }
+            std::vector<float> mqf(8 * 576);
+            for (int i = 0; i < 8 * 576; ++i) mqf[i] = (float)mq[i];
+            G->mq64.alloc(8 * 576 * 8);
+            G->mq32.alloc(8 * 576 * 4);
+            LP_HIP(hipMemcpy(G->mq64.p, mq.data(), 8 * 576 * 8, hipMemcpyHostToDevice));
+            LP_HIP(hipMemcpy(G->mq32.p, mqf.data(), 8 * 576 * 4, hipMemcpyHostToDevice));
+            const size_t vb = 3 * nn * G->max_rhs * 8;
+            G->x.alloc(vb);
+            G->r.alloc(vb);
+            G->p.alloc(vb);
+            G->ap.alloc(vb);
+            G->z.alloc(vb);
+            G->b.alloc(vb);
+            G->scal.alloc(G->max_rhs * sizeof(CgScal));
+            LP_HIP(hipMemset(G->scal.p, 0, G->max_rhs * sizeof(CgScal)));
+            G->host_pinned_bytes = G->max_rhs * sizeof(CgScal) + 64;
+            LP_HIP(hipHostMalloc(&G->host_pinned, G->host_pinned_bytes, hipHostMallocDefault));
+            G->lv[0].e64.alloc(G->g0.ncell * 8);
+        } catch (...) {
+            delete G;
+            throw;
+        }
+        *out = G;
+        return LP_OK;
+    });
+}
+
+void lp_grid_destroy(lp_grid* g) {
+    if (!g) return;
+    hipStreamSynchronize(g->stream);
+    if (g->host_pinned) hipHostFree(g->host_pinned);
+    hipStream_t s = g->stream;
+    delete g;
+    if (s) hipStreamDestroy(s);
+}
+
+int lp_set_moduli(lp_grid* g, const double* e, int flags) {
+    return guarded([&]() -> int {
+        if (!g || !e) return LP_ERR_ARG;
+        const size_t bytes = g->g0.ncell * 8;
+        if (flags & 1)
+            LP_HIP(hipMemcpyAsync(g->lv[0].e64.p, e, bytes, hipMemcpyDeviceToDevice, g->stream));
+        else
+            LP_HIP(hipMemcpyAsync(g->lv[0].e64.p, e, bytes, hipMemcpyHostToDevice, g->stream));
+        return set_moduli_impl(*g, (flags & 2) != 0);
+    });
+}
+
+int lp_estimate_lmax(lp_grid* g, int level, const double* start) {
+    return guarded([&]() -> int {
+        if (!g || !g->built || level < 0 || level >= g->nlevels) return LP_ERR_ARG;
+        g->lv[level].lmax = estimate_lmax_level(*g, level, start);
+        return LP_OK;
+    });
+}
+
+int lp_grid_info(lp_grid* g, int* nlevels, int* dims, int* strict, double* lmax) {
+    return guarded([&]() -> int {
+        if (!g) return LP_ERR_ARG;
+        if (nlevels) *nlevels = g->nlevels;
+        for (int i = 0; i < g->nlevels; ++i) {
+            if (dims) {
+                dims[3 * i] = g->lv[i].g.nx;
+                dims[3 * i + 1] = g->lv[i].g.ny;
+                dims[3 * i + 2] = g->lv[i].g.nz;
+            }
+            if (lmax) lmax[i] = g->lv[i].lmax;
+        }
+        if (strict) *strict = g->strict ? 1 : 0;
+        return LP_OK;
+    });
+}
+
+int lp_apply(lp_grid* g, int level, int dtype_bytes, const void* u, void* out, int nrhs, int device_ptr) {
+    return guarded([&]() -> int {
+        if (!g || !g->built || level < 0 || level >= g->nlevels || !u || !out || nrhs < 1) return LP_ERR_ARG;
+        if (dtype_bytes != 4 && dtype_bytes != 8) return LP_ERR_ARG;
+        if (dtype_bytes == 8 && level > 0 && g->opt.precision != LP_PREC_FP64) {
+            set_error("lp_apply: FP64 coarse operators exist only in the FP64 hierarchy");
+            return LP_ERR_ARG;
+        }
+        const Level& L = g->lv[level];
+        const size_t n = 3 * L.g.nn;
+        const size_t bytes = n * nrhs * dtype_bytes;
+        const void* du = to_device(*g, g->stage_a, u, bytes, device_ptr);
+        void* dout = device_ptr ? out : (g->stage_b.alloc(bytes), g->stage_b.p);
+        EpiArgs E;
+        E.kind = EPI_STORE;
+        E.out = dout;
+        launch_apply(*g, level, dtype_bytes, du, n, nrhs, E, g->stream);
+        if (!device_ptr) from_device(*g, out, dout, bytes);
+        else LP_HIP(hipStreamSynchronize(g->stream));
+        return LP_OK;
+    });
+}
+
+int lp_precondition(lp_grid* g, const double* r, double* z, int nrhs, int device_ptr) {
+    return guarded([&]() -> int {
+        if (!g || !g->built || nrhs < 1 || nrhs > g->max_rhs) return LP_ERR_ARG;
+        const size_t n = 3 * g->g0.nn;
+        const size_t bytes = n * nrhs * 8;
+        const double* dr = static_cast<const double*>(to_device(*g, g->stage_a, r, bytes, device_ptr));
+        double* dz = device_ptr ? z : (g->stage_b.alloc(bytes), g->stage_b.as<double>());
+        precondition(*g, dr, dz, nrhs);
+        if (!device_ptr) from_device(*g, z, dz, bytes);
+        else LP_HIP(hipStreamSynchronize(g->stream));
+        return LP_OK;
+    });
+}
+
+int lp_solve(lp_grid* g, int nrhs, const double* b, double* x, int warm, int device_ptr, int* iters, int* converged) {
+    return guarded([&]() -> int {
+        if (!g || !g->built || nrhs < 1 || nrhs > g->max_rhs || !b || !x || !iters || !converged) return LP_ERR_ARG;
+        Grid& G = *g;
+        const size_t n = 3 * G.g0.nn;
+        const size_t bytes = n * nrhs * 8;
+        auto load = [&]() {
+            if (device_ptr) {
+                LP_HIP(hipMemcpyAsync(G.b.p, b, bytes, hipMemcpyDeviceToDevice, G.stream));
+                if (warm) LP_HIP(hipMemcpyAsync(G.x.p, x, bytes, hipMemcpyDeviceToDevice, G.stream));
+            } else {
+                LP_HIP(hipMemcpyAsync(G.b.p, b, bytes, hipMemcpyHostToDevice, G.stream));
+                if (warm) LP_HIP(hipMemcpyAsync(G.x.p, x, bytes, hipMemcpyHostToDevice, G.stream));
+            }
+        };
+        load();
+        std::vector<int> it1(nrhs), ok1(nrhs);
+        const bool stall = !G.strict && G.nlevels >= 2;
+        const int full = G.opt.maxiter;
+        if (stall) G.opt.maxiter = std::min(full, 100);   // mg.py:928-965 STALL
+        pcg(G, nrhs, warm != 0, it1.data(), ok1.data());
+        G.opt.maxiter = full;
+        bool all_ok = true;
+        for (int k = 0; k < nrhs; ++k) all_ok &= ok1[k] != 0;
+        if (!all_ok && stall) {
+            G.strict = true;
+            const int rc = set_moduli_impl(G, false);
+            if (rc != LP_OK) return rc;
+            load();
+            std::vector<int> it2(nrhs), ok2(nrhs);
+            pcg(G, nrhs, warm != 0, it2.data(), ok2.data());
+            for (int k = 0; k < nrhs; ++k) {
+                it1[k] += it2[k];
+                ok1[k] = ok2[k];
+            }
+        }
+        all_ok = true;
+        for (int k = 0; k < nrhs; ++k) {
+            iters[k] = it1[k];
+            converged[k] = ok1[k];
+            all_ok &= ok1[k] != 0;
+        }
+        if (device_ptr) {
+            LP_HIP(hipMemcpyAsync(x, G.x.p, bytes, hipMemcpyDeviceToDevice, G.stream));
+            LP_HIP(hipStreamSynchronize(G.stream));
+        } else {
+            from_device(G, x, G.x.p, bytes);
+        }
+        if (!all_ok) {
+            set_error("linear solve did not converge; the structure may be a mechanism (check supports)");
+            return LP_NOT_CONVERGED;
+        }
+        return LP_OK;
+    });
+}
+
+int lp_strain_energy(lp_grid* g, int nrhs, const double* u, double* se, int device_ptr) {
+    return guarded([&]() -> int {
+        if (!g || nrhs < 1 || !u || !se) return LP_ERR_ARG;
+        Grid& G = *g;
+        const size_t n = 3 * G.g0.nn;
+        const double* du = static_cast<const double*>(to_device(G, G.stage_a, u, n * nrhs * 8, device_ptr));
+        const size_t sb = G.g0.ncell * nrhs * 8;
+        double* dse = device_ptr ? se : (G.stage_b.alloc(sb), G.stage_b.as<double>());
+        hipLaunchKernelGGL(k_strain_energy, dim3(nblk(G.g0.ncell), nrhs), dim3(kBlock), 0, G.stream, du,
+                           G.ke64.as<double>(), G.g0, dse, n);
+        LP_HIP(hipGetLastError());
+        if (!device_ptr) from_device(G, se, dse, sb);
+        else LP_HIP(hipStreamSynchronize(G.stream));
+        return LP_OK;
+    });
+}
+
+int lp_level_data(lp_grid* g, int level, int what, void* out) {
+    return guarded([&]() -> int {
+        if (!g || !g->built || level < 0 || level >= g->nlevels || !out) return LP_ERR_ARG;
+        Grid& G = *g;
+        Level& L = G.lv[level];
+        if (what == 0) {
+            from_device(G, out, L.mask.p, L.g.nn);
+        } else if (what == 1) {
+            from_device(G, out, L.dinv64.p, 3 * L.g.nn * 8);
+        } else if (what == 2) {
+            DBuf tmp;
+            tmp.alloc(L.g.ncell * 24 * 8);
+            level_cell_dofs(G, level, tmp.as<int64_t>());
+            from_device(G, out, tmp.p, L.g.ncell * 24 * 8);
+        } else if (what == 3) {
+            from_device(G, out, L.held.p, L.g.nn);
+        } else {
+            return LP_ERR_ARG;
+        }
+        return LP_OK;
+    });
+}
+
+int lp_prolong(const double* coarse, double* fine, int cx, int cy, int cz, int device_ptr) {
+    return guarded([&]() -> int {
+        if (!coarse || !fine || cx < 1 || cy < 1 || cz < 1) return LP_ERR_ARG;
+        const size_t nc = (size_t)cx * cy * cz, nf = (size_t)(2 * cx - 1) * (2 * cy - 1) * (2 * cz - 1);
+        DBuf a, b;
+        const double* dc = coarse;
+        double* df = fine;
+        if (!device_ptr) {
+            a.alloc(3 * nc * 8);
+            b.alloc(3 * nf * 8);
+            LP_HIP(hipMemcpy(a.p, coarse, 3 * nc * 8, hipMemcpyHostToDevice));
+            dc = a.as<double>();
+            df = b.as<double>();
+        }
+        for (int c = 0; c < 3; ++c) launch_prolong_plain(dc + c * nc, df + c * nf, cx, cy, cz, nullptr);
+        if (!device_ptr) LP_HIP(hipMemcpy(fine, df, 3 * nf * 8, hipMemcpyDeviceToHost));
+        else LP_HIP(hipDeviceSynchronize());
+        return LP_OK;
+    });
+}
+
+int lp_restrict(const double* fine, double* coarse, int cx, int cy, int cz, int device_ptr) {
+    return guarded([&]() -> int {
+        if (!coarse || !fine || cx < 1 || cy < 1 || cz < 1) return LP_ERR_ARG;
+        const size_t nc = (size_t)cx * cy * cz, nf = (size_t)(2 * cx - 1) * (2 * cy - 1) * (2 * cz - 1);
+        DBuf a, b;
+        const double* df = fine;
+        double* dc = coarse;
+        if (!device_ptr) {
+            a.alloc(3 * nf * 8);
+            b.alloc(3 * nc * 8);
+            LP_HIP(hipMemcpy(a.p, fine, 3 * nf * 8, hipMemcpyHostToDevice));
+            df = a.as<double>();
+            dc = b.as<double>();
+        }
+        for (int c = 0; c < 3; ++c) launch_restrict_plain(df + c * nf, dc + c * nc, cx, cy, cz, nullptr);
+        if (!device_ptr) LP_HIP(hipMemcpy(coarse, dc, 3 * nc * 8, hipMemcpyDeviceToHost));
+        else LP_HIP(hipDeviceSynchronize());
+        return LP_OK;
+    });
+}
+
+int lp_time_apply(lp_grid* g, int level, int dtype_bytes, int nrhs, int reps, double* ms_per_apply) {
+    return guarded([&]() -> int {
+        if (!g || !g->built || level < 0 || level >= g->nlevels || reps < 1 || !ms_per_apply) return LP_ERR_ARG;
+        if (nrhs < 1 || nrhs > g->max_rhs) return LP_ERR_ARG;
+        Grid& G = *g;
+        Level& L = G.lv[level];
+        const size_t n = 3 * L.g.nn;
+        DBuf u, o;
+        u.alloc(n * nrhs * dtype_bytes);
+        o.alloc(n * nrhs * dtype_bytes);
+        launch_fill(u.p, dtype_bytes, n * nrhs, 1.0, G.stream);
+        EpiArgs E;
+        E.kind = EPI_STORE;
+        E.out = o.p;
+        launch_apply(G, level, dtype_bytes, u.p, n, nrhs, E, G.stream);
+        hipEvent_t a, b;
+        LP_HIP(hipEventCreate(&a));
+        LP_HIP(hipEventCreate(&b));
+        LP_HIP(hipEventRecord(a, G.stream));
+        for (int k = 0; k < reps; ++k) launch_apply(G, level, dtype_bytes, u.p, n, nrhs, E, G.stream);
+        LP_HIP(hipEventRecord(b, G.stream));
+        LP_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        LP_HIP(hipEventElapsedTime(&ms, a, b));
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        *ms_per_apply = ms / reps;
+        return LP_OK;
+    });
+}
+
+}  // extern "C"

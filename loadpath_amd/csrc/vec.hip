@@ -1,0 +1,377 @@
+// Vector passes of the PCG and the smoothers, with deterministic two-stage
+// reductions (per-block partials in a fixed order, then one block per rhs).
+//
+// Reference: mgops._cg_step / _xpby / _dot / _cast_into / _cheb_start
+// (mgops.py:320-360, 557-585) and mg._pcg (mg.py:872-926). Scalars (alpha,
+// beta, |r|) stay on the device in CgScal; the host reads only the per-rhs
+// active / converged flags.
+#include "lp_internal.h"
+
+namespace lp {
+
+namespace {
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+__device__ double block_sum(double v, double* sred) {
+    v = wsum(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) sred[wv] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sred[i];
+    __syncthreads();
+    return s;  // valid on thread 0
+}
+
+template <typename T>
+__global__ void k_dot(const T* __restrict__ a, const T* __restrict__ b, size_t n, size_t stride,
+                      double* partials, const CgScal* scal) {
+    __shared__ double sred[4];
+    const int rhs = blockIdx.y;
+    double acc = 0;
+    if (!(scal && !scal[rhs].active)) {
+        const T* pa = a + rhs * stride;
+        const T* pb = b + rhs * stride;
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+            acc += (double)(pa[i] * pb[i]);
+    }
+    const double s = block_sum(acc, sred);
+    if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void k_reduce(const double* __restrict__ partials, int nb, double* dst, size_t dstride) {
+    __shared__ double sred[4];
+    const int rhs = blockIdx.x;
+    double acc = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partials[(size_t)rhs * nb + i];
+    const double s = block_sum(acc, sred);
+    if (threadIdx.x == 0) *(double*)((char*)dst + rhs * dstride) = s;
+}
+
+template <typename S, typename D>
+__global__ void k_cast(const S* __restrict__ src, D* __restrict__ dst, size_t n, size_t stride) {
+    const int rhs = blockIdx.y;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[rhs * stride + i] = (D)src[rhs * stride + i];
+}
+
+template <typename T> __global__ void k_fill(T* dst, size_t n, T v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = v;
+}
+
+// mode 0: v *= num[rhs]; mode 1: v /= (T)num[rhs] (num = sqrt of stored value when mode 2)
+template <typename T>
+__global__ void k_scale_dev(T* v, size_t n, size_t stride, const double* num, size_t nstride, int mode) {
+    const int rhs = blockIdx.y;
+    const double s = *(const double*)((const char*)num + rhs * nstride);
+    T f;
+    if (mode == 0) f = (T)s;
+    else f = (T)sqrt(s);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (mode == 0) v[rhs * stride + i] = v[rhs * stride + i] * f;
+        else v[rhs * stride + i] = f != T(0) ? v[rhs * stride + i] / f : v[rhs * stride + i];
+    }
+}
+
+// x = 0; r = b; d = (dinv b) s   (mgops._cheb_start, mgops.py:320-327)
+template <typename T>
+__global__ void k_cheb_start(const T* __restrict__ b, const T* __restrict__ dinv, T s, T* __restrict__ x,
+                             T* __restrict__ r, T* __restrict__ d, size_t n, size_t stride) {
+    const int rhs = blockIdx.y;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = rhs * stride + i;
+        const T bi = b[k];
+        x[k] = T(0);
+        r[k] = bi;
+        d[k] = (dinv[i] * bi) * s;
+    }
+}
+
+template <typename T>
+__global__ void k_axpy1(T* __restrict__ x, const T* __restrict__ d, size_t n, size_t stride) {
+    const int rhs = blockIdx.y;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        x[rhs * stride + i] += d[rhs * stride + i];
+}
+
+// first damped-Jacobi sweep from zero: x = w dinv b
+template <typename T>
+__global__ void k_jacobi_start(const T* __restrict__ b, const T* __restrict__ dinv, double w, T* __restrict__ x,
+                               size_t n, size_t stride) {
+    const int rhs = blockIdx.y;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        x[rhs * stride + i] = (T)(w * (double)dinv[i] * (double)b[rhs * stride + i]);
+}
+
+// x += alpha p; r -= alpha ap; partial |r|^2   (mgops._cg_step, mgops.py:346-354)
+__global__ void k_cg_step(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ p,
+                          const double* __restrict__ ap, size_t n, size_t stride, const CgScal* scal,
+                          double* partials) {
+    __shared__ double sred[4];
+    const int rhs = blockIdx.y;
+    double acc = 0;
+    if (scal[rhs].active) {
+        const double a = scal[rhs].alpha;
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+            const size_t k = rhs * stride + i;
+            x[k] += a * p[k];
+            const double ri = r[k] - a * ap[k];
+            r[k] = ri;
+            acc += ri * ri;
+        }
+    }
+    const double s = block_sum(acc, sred);
+    if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = s;
+}
+
+// p = z + beta p   (mgops._xpby, mgops.py:357-360)
+__global__ void k_xpby(const double* __restrict__ z, double* __restrict__ p, size_t n, size_t stride,
+                       const CgScal* scal) {
+    const int rhs = blockIdx.y;
+    if (!scal[rhs].active) return;
+    const double beta = scal[rhs].beta;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = rhs * stride + i;
+        p[k] = z[k] + beta * p[k];
+    }
+}
+
+// Finalize stages (one block per rhs): reduce the partials, update CgScal.
+__global__ void k_cg_finalize(const double* __restrict__ partials, int nb, CgScal* scal, int stage,
+                              double rtol) {
+    __shared__ double sred[4];
+    const int rhs = blockIdx.x;
+    double acc = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partials[(size_t)rhs * nb + i];
+    const double s = block_sum(acc, sred);
+    if (threadIdx.x != 0) return;
+    CgScal& c = scal[rhs];
+    switch (stage) {
+        case 0:  // pap = p.Ap -> alpha
+            if (!c.active) break;
+            c.pap = s;
+            if (!(s > 0)) {
+                c.failed = 1;
+                c.active = 0;
+            } else {
+                c.alpha = c.rz / s;
+            }
+            break;
+        case 1:  // rr -> convergence
+            if (!c.active) break;
+            c.rr = s;
+            c.iters += 1;
+            if (sqrt(s) <= c.tol) {
+                c.converged = 1;
+                c.active = 0;
+            }
+            break;
+        case 2:  // rz_new -> beta
+            if (!c.active) break;
+            c.beta = s / c.rz;
+            c.rz = s;
+            break;
+        case 3:  // |b|^2 -> bnorm, tol; reset counters
+            c.bnorm = sqrt(s);
+            c.tol = rtol * c.bnorm;
+            c.iters = 0;
+            c.failed = 0;
+            c.converged = c.bnorm == 0 ? 1 : 0;
+            c.active = c.bnorm == 0 ? 0 : 1;
+            c.rz = 0;
+            c.alpha = 0;
+            c.beta = 0;
+            break;
+        case 4:  // rz = r.z
+            if (!c.active) break;
+            c.rz = s;
+            break;
+        case 5:  // xax (warm start): stored in tmp
+            c.tmp = s;
+            break;
+        case 6:  // b.x -> warm-start scale s = bx / xax (or 0), stored in alpha
+            c.alpha = (c.tmp > 0) ? s / c.tmp : 0.0;
+            break;
+        case 7:  // |b - s Ax|^2 -> keep the scaled start (beta = 1) or drop it (beta = 0)
+            c.beta = (c.alpha > 0 && sqrt(s) < c.bnorm) ? 1.0 : 0.0;
+            break;
+        default: break;
+    }
+}
+
+// warm start: if beta==1: x *= s; r = b - s ax; else x = 0; r = b
+__global__ void k_warm_apply(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ b,
+                             const double* __restrict__ ax, size_t n, size_t stride, const CgScal* scal) {
+    const int rhs = blockIdx.y;
+    const bool keep = scal[rhs].beta == 1.0;
+    const double s = scal[rhs].alpha;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = rhs * stride + i;
+        if (keep) {
+            x[k] *= s;
+            r[k] = b[k] - s * ax[k];
+        } else {
+            x[k] = 0;
+            r[k] = b[k];
+        }
+    }
+}
+
+// |b - s ax|^2 partials for the warm-start test
+__global__ void k_warm_norm(const double* __restrict__ b, const double* __restrict__ ax, size_t n,
+                            size_t stride, const CgScal* scal, double* partials) {
+    __shared__ double sred[4];
+    const int rhs = blockIdx.y;
+    const double s = scal[rhs].alpha;
+    double acc = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = rhs * stride + i;
+        const double v = b[k] - s * ax[k];
+        acc += v * v;
+    }
+    const double t = block_sum(acc, sred);
+    if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = t;
+}
+
+inline int grid_for(size_t n) {
+    size_t b = (n + kBlock - 1) / kBlock;
+    if (b > 2048) b = 2048;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+}  // namespace
+
+void ensure_partials(Grid& G, size_t n) {
+    if (n > G.partials_cap) {
+        G.partials.alloc(n * sizeof(double));
+        G.partials_cap = n;
+    }
+}
+
+int dot_blocks(size_t n) {
+    size_t b = (n + 4 * kBlock - 1) / (4 * kBlock);
+    if (b > 1024) b = 1024;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+void launch_dot(const void* a, const void* b, int eb, size_t n, size_t stride, int nrhs, double* partials,
+                const CgScal* scal, hipStream_t s) {
+    dim3 grid(dot_blocks(n), nrhs);
+    if (eb == 8)
+        hipLaunchKernelGGL(k_dot<double>, grid, dim3(kBlock), 0, s, (const double*)a, (const double*)b, n, stride,
+                           partials, scal);
+    else
+        hipLaunchKernelGGL(k_dot<float>, grid, dim3(kBlock), 0, s, (const float*)a, (const float*)b, n, stride,
+                           partials, scal);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_reduce(const double* partials, int nb, int nrhs, double* dst, size_t dstride, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce, dim3(nrhs), dim3(kBlock), 0, s, partials, nb, dst, dstride);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_cast(const void* src, int sb, void* dst, int db, size_t n, size_t stride, int nrhs, hipStream_t s) {
+    dim3 grid(grid_for(n), nrhs);
+    if (sb == 8 && db == 4)
+        hipLaunchKernelGGL((k_cast<double, float>), grid, dim3(kBlock), 0, s, (const double*)src, (float*)dst, n, stride);
+    else if (sb == 4 && db == 8)
+        hipLaunchKernelGGL((k_cast<float, double>), grid, dim3(kBlock), 0, s, (const float*)src, (double*)dst, n, stride);
+    else if (sb == 8 && db == 8)
+        hipLaunchKernelGGL((k_cast<double, double>), grid, dim3(kBlock), 0, s, (const double*)src, (double*)dst, n, stride);
+    else
+        hipLaunchKernelGGL((k_cast<float, float>), grid, dim3(kBlock), 0, s, (const float*)src, (float*)dst, n, stride);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_fill(void* dst, int eb, size_t n, double v, hipStream_t s) {
+    if (eb == 8) hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(kBlock), 0, s, (double*)dst, n, v);
+    else if (eb == 4) hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(kBlock), 0, s, (float*)dst, n, (float)v);
+    else hipLaunchKernelGGL(k_fill<uint8_t>, dim3(grid_for(n)), dim3(kBlock), 0, s, (uint8_t*)dst, n, (uint8_t)v);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_scale_by_dev(void* v, int eb, size_t n, size_t stride, int nrhs, const double* num, size_t nstride,
+                         int mode, hipStream_t s) {
+    dim3 grid(grid_for(n), nrhs);
+    if (eb == 8)
+        hipLaunchKernelGGL(k_scale_dev<double>, grid, dim3(kBlock), 0, s, (double*)v, n, stride, num, nstride, mode);
+    else
+        hipLaunchKernelGGL(k_scale_dev<float>, grid, dim3(kBlock), 0, s, (float*)v, n, stride, num, nstride, mode);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_cheb_start(const void* b, const void* dinv, double s_, void* x, void* r, void* d, int eb, size_t n,
+                       size_t stride, int nrhs, hipStream_t st) {
+    dim3 grid(grid_for(n), nrhs);
+    if (eb == 8)
+        hipLaunchKernelGGL(k_cheb_start<double>, grid, dim3(kBlock), 0, st, (const double*)b, (const double*)dinv, s_,
+                           (double*)x, (double*)r, (double*)d, n, stride);
+    else
+        hipLaunchKernelGGL(k_cheb_start<float>, grid, dim3(kBlock), 0, st, (const float*)b, (const float*)dinv,
+                           (float)s_, (float*)x, (float*)r, (float*)d, n, stride);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_axpy_inplace(void* x, const void* d, int eb, size_t n, size_t stride, int nrhs, hipStream_t st) {
+    dim3 grid(grid_for(n), nrhs);
+    if (eb == 8) hipLaunchKernelGGL(k_axpy1<double>, grid, dim3(kBlock), 0, st, (double*)x, (const double*)d, n, stride);
+    else hipLaunchKernelGGL(k_axpy1<float>, grid, dim3(kBlock), 0, st, (float*)x, (const float*)d, n, stride);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_jacobi_start(const void* b, const void* dinv, double w, void* x, int eb, size_t n, size_t stride,
+                         int nrhs, hipStream_t st) {
+    dim3 grid(grid_for(n), nrhs);
+    if (eb == 8)
+        hipLaunchKernelGGL(k_jacobi_start<double>, grid, dim3(kBlock), 0, st, (const double*)b, (const double*)dinv, w,
+                           (double*)x, n, stride);
+    else
+        hipLaunchKernelGGL(k_jacobi_start<float>, grid, dim3(kBlock), 0, st, (const float*)b, (const float*)dinv, w,
+                           (float*)x, n, stride);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_cg_step(Grid& G, int nrhs, hipStream_t s) {
+    const size_t n = 3 * G.g0.nn;
+    const int nb = dot_blocks(n);
+    hipLaunchKernelGGL(k_cg_step, dim3(nb, nrhs), dim3(kBlock), 0, s, G.x.as<double>(), G.r.as<double>(),
+                       G.p.as<double>(), G.ap.as<double>(), n, n, G.scal.as<CgScal>(), G.partials.as<double>());
+    LP_HIP(hipGetLastError());
+}
+
+void launch_xpby(Grid& G, int nrhs, hipStream_t s) {
+    const size_t n = 3 * G.g0.nn;
+    hipLaunchKernelGGL(k_xpby, dim3(grid_for(n), nrhs), dim3(kBlock), 0, s, G.z.as<double>(), G.p.as<double>(), n,
+                       n, G.scal.as<CgScal>());
+    LP_HIP(hipGetLastError());
+}
+
+void launch_cg_finalize(Grid& G, int nrhs, int stage, int nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_finalize, dim3(nrhs), dim3(kBlock), 0, s, G.partials.as<double>(), nb,
+                       G.scal.as<CgScal>(), stage, G.opt.rtol);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_warm(Grid& G, int nrhs, int which, hipStream_t s) {
+    const size_t n = 3 * G.g0.nn;
+    if (which == 0) {
+        hipLaunchKernelGGL(k_warm_norm, dim3(dot_blocks(n), nrhs), dim3(kBlock), 0, s, G.b.as<double>(),
+                           G.ap.as<double>(), n, n, G.scal.as<CgScal>(), G.partials.as<double>());
+    } else {
+        hipLaunchKernelGGL(k_warm_apply, dim3(grid_for(n), nrhs), dim3(kBlock), 0, s, G.x.as<double>(),
+                           G.r.as<double>(), G.b.as<double>(), G.ap.as<double>(), n, n, G.scal.as<CgScal>());
+    }
+    LP_HIP(hipGetLastError());
+}
+
+}  // namespace lp

@@ -1,0 +1,87 @@
+"""Voxel hex8 element and mesh numbering (host side of the drop-in).
+
+- ``hex8_ke(nu, element)``: fem.hex8_ke (loadpath/fem.py:220-224), computed
+  by liblpmi's FP64 host routine (lp_hex8_ke).
+- ``hex8_element()``: the LOADPATH_HEX8 switch (fem.py:151-160).
+- ``HexMesh``: fem.HexMesh (fem.py:317-374): node ids ascend in C order over
+  the node grid, ``edof = 3 * node + component``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _lib
+
+CORNERS = np.array([[(c >> 0) & 1, (c >> 1) & 1, (c >> 2) & 1] for c in range(8)])
+_ELEMENTS = ("im", "plain")
+
+
+def hex8_element() -> str:
+    e = os.environ.get("LOADPATH_HEX8", "im").strip().lower() or "im"
+    if e not in _ELEMENTS:
+        raise ValueError(f"LOADPATH_HEX8 must be one of {_ELEMENTS}, not {e!r}")
+    return e
+
+
+def hex8_ke(nu: float, element: str | None = None) -> np.ndarray:
+    element = element or hex8_element()
+    if element not in _ELEMENTS:
+        raise ValueError(f"element must be one of {_ELEMENTS}, not {element!r}")
+    out = np.empty((24, 24))
+    _lib.check(_lib.lib().lp_hex8_ke(float(nu), 0 if element == "im" else 1, _lib.ptr(out)), "lp_hex8_ke")
+    return out
+
+
+class HexMesh:
+    """Active voxels of a grid and the nodes they use (fem.py:317-374)."""
+
+    def __init__(self, elem_ijk: np.ndarray, pitch: float, origin=(0.0, 0.0, 0.0)):
+        self.elem_ijk = np.asarray(elem_ijk, np.int64).reshape(-1, 3)
+        self.pitch = float(pitch)
+        self.origin = np.asarray(origin, float)
+        ne = len(self.elem_ijk)
+        if ne == 0:
+            raise ValueError("no active elements")
+        dims = self.elem_ijk.max(axis=0) + 2
+        self.node_dims = dims
+        e_flat = (self.elem_ijk[:, 0] * dims[1] + self.elem_ijk[:, 1]) * dims[2] + self.elem_ijk[:, 2]
+        offs = (CORNERS[:, 0] * dims[1] + CORNERS[:, 1]) * dims[2] + CORNERS[:, 2]
+        used_mark = np.zeros(int(np.prod(dims)), bool)
+        for o in offs:
+            used_mark[e_flat + o] = True
+        used = np.flatnonzero(used_mark)
+        self._lookup = np.full(int(np.prod(dims)), -1, np.int64)
+        self._lookup[used] = np.arange(len(used))
+        self.enode = self._lookup[e_flat[:, None] + offs[None, :]]
+        self.node_ijk = np.stack(np.unravel_index(used, tuple(dims)), axis=1).astype(np.int64)
+        self.n_nodes = len(used)
+        self.node_xyz = self.origin + self.node_ijk * self.pitch
+        self.n_dof = 3 * self.n_nodes
+        self._edof = None
+
+    @property
+    def n_elem(self):
+        return len(self.elem_ijk)
+
+    @property
+    def edof(self):
+        if self._edof is None:
+            self._edof = (3 * self.enode[:, :, None] + np.arange(3)).reshape(self.n_elem, 24)
+        return self._edof
+
+    def node_id(self, ijk) -> np.ndarray:
+        ijk = np.asarray(ijk, np.int64)
+        d = self.node_dims
+        ok = np.all((ijk >= 0) & (ijk < d), axis=-1)
+        flat = (ijk[..., 0] * d[1] + ijk[..., 1]) * d[2] + ijk[..., 2]
+        out = np.full(ijk.shape[:-1], -1, np.int64)
+        out[ok] = self._lookup[flat[ok]]
+        return out
+
+
+def box_mesh(nx: int, ny: int, nz: int, pitch: float = 1.0) -> HexMesh:
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    return HexMesh(np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1), pitch)

@@ -1,0 +1,196 @@
+"""HIP path vs the FP64 oracle on seeded inputs (tiny grids, all through the
+C-ABI via loadpath_amd). Integer/index work bit-exact; floating point within
+the tolerance written in each test."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from tests.conftest import cantilever_bar
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(mesh, fixed, e, **kw):
+    from loadpath_amd.mg import GridElasticity
+    from oracle.mg import GridOracle
+
+    vd = kw.pop("vdtype", np.float32)
+    el = kw.pop("element", "im")
+    g = GridElasticity(mesh, 0.3, fixed, dtype=vd, element=el, **kw)
+    o = GridOracle(mesh, 0.3, fixed, vdtype=vd, element=el, **kw)
+    g.set_moduli(e)
+    o.set_moduli(e)
+    return g, o
+
+
+def _voided(nx=12, ny=6, nz=6, seed=1):
+    mesh, fixed, f, e = cantilever_bar(nx, ny, nz, seed=seed)
+    rng = np.random.default_rng(seed + 10)
+    e = e.copy()
+    e[rng.random(mesh.n_elem) < 0.15] = 0.0          # removed cells: dead nodes
+    e[mesh.elem_ijk[:, 0] == 0] = 1.0                # keep the root
+    return mesh, fixed, f, e
+
+
+def test_cell_dofs_bit_exact(gpu):
+    from loadpath_amd.mg import GridElasticity
+    from oracle.fem import box_mesh
+
+    mesh = box_mesh(6, 4, 2)
+    g = GridElasticity(mesh, 0.3, np.zeros(mesh.n_dof, bool))
+    g.set_moduli(np.ones(mesh.n_elem))
+    nn = g.nn
+    ed = mesh.edof                                   # oracle numbering: 3 * node + comp
+    want = (ed % 3) * nn + ed // 3                   # component-major grid dofs
+    got = g.cell_dofs(0)
+    assert got.dtype == np.int64 and np.array_equal(got, want)
+
+
+def test_masks_and_levels_bit_exact(gpu):
+    mesh, fixed, f, e = _voided()
+    g, o = _pair(mesh, fixed, e, coarse_cells=20)
+    assert len(g.levels) == len(o.levels) >= 3
+    for i, lv in enumerate(o.levels):
+        assert g.levels[i].dims == lv.dims
+        m = g.level_mask(i)
+        assert np.array_equal(m, lv.mask), i
+    # dinv: same rule, FP64 rounding of 1/d (d summed in another order)
+    d0 = g.level_dinv(0)
+    assert np.allclose(d0, o.levels[0].dinv, rtol=1e-14, atol=0)
+    assert np.array_equal(g.live_dofs(), o.live_dofs())
+
+
+def test_fine_apply_fp64_matches_oracle_and_assembled(gpu):
+    from oracle.fem import assemble
+
+    mesh, fixed, f, e = _voided()
+    g, o = _pair(mesh, fixed, e, coarse_cells=20)
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal((2,) + o.levels[0].node_shape)
+    got = g.apply(u)
+    want = np.stack([o.levels[0].apply(v) for v in u])
+    scale = np.abs(want).max()
+    assert np.abs(got - want).max() <= 1e-13 * scale
+    # brute force: m K (m u) + (1 - m) u with K assembled by edof
+    k = assemble(mesh, o.ke, e)
+    m = o.levels[0].mask
+    for j in range(2):
+        um = o.from_grid(u[j] * m)
+        ku = o.to_grid(k @ um)
+        bf = ku * m + (1 - m) * u[j]
+        # nodes outside the mesh (none on a full box) keep identity
+        assert np.abs(got[j] - bf).max() <= 1e-13 * scale
+
+
+def test_level_operators_fp32(gpu):
+    mesh, fixed, f, e = _voided()
+    g, o = _pair(mesh, fixed, e, coarse_cells=20)
+    rng = np.random.default_rng(6)
+    for i, lv in enumerate(o.vlevels):
+        u = rng.standard_normal(lv.node_shape).astype(np.float32)
+        got = g.apply(u, level=i, dtype=np.float32)
+        want = o.vlevels[i].apply(u.astype(np.float64).astype(np.float32))
+        ref64 = o.levels[i].apply(u.astype(np.float64))
+        scale = np.abs(ref64).max()
+        assert np.abs(got - ref64).max() <= 2e-5 * scale, i
+        assert np.abs(want - ref64).max() <= 2e-5 * scale, i
+
+
+def test_fp64_hierarchy_operators(gpu):
+    mesh, fixed, f, e = _voided()
+    g, o = _pair(mesh, fixed, e, coarse_cells=20, vdtype=np.float64)
+    rng = np.random.default_rng(7)
+    for i, lv in enumerate(o.levels):
+        u = rng.standard_normal(lv.node_shape)
+        got = g.apply(u, level=i, dtype=np.float64)
+        want = lv.apply(u)
+        assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max(), i
+
+
+def test_lmax_matches_oracle(gpu):
+    mesh, fixed, f, e = _voided()
+    g, o = _pair(mesh, fixed, e, coarse_cells=20)
+    for i, lv in enumerate(o.vlevels):
+        assert g.levels[i].lmax == pytest.approx(lv.lmax, rel=1e-4), i
+
+
+def test_precondition_matches_oracle(gpu):
+    mesh, fixed, f, e = _voided()
+    for vd, tol in ((np.float64, 1e-9), (np.float32, 2e-4)):
+        g, o = _pair(mesh, fixed, e, coarse_cells=20, vdtype=vd)
+        rng = np.random.default_rng(8)
+        r = rng.standard_normal(o.levels[0].node_shape) * o.levels[0].mask
+        got = g.precondition(r[None])[0]
+        want = o.precondition(r)
+        assert np.abs(got - want).max() <= tol * np.abs(want).max(), vd
+
+
+@pytest.mark.parametrize("smoother", ["chebyshev", "jacobi"])
+def test_solve_matches_direct_and_oracle_counts(gpu, smoother):
+    mesh, fixed, f, e = cantilever_bar(16, 8, 8, seed=2, contrast=(smoother == "chebyshev"))
+    from oracle.fem import assemble
+
+    k = assemble(mesh, __import__("oracle.fem", fromlist=["hex8_ke"]).hex8_ke(0.3), e)
+    free = ~fixed
+    ref = np.zeros(mesh.n_dof)
+    ref[free] = spla.spsolve(k[free][:, free].tocsc(), f[free])
+    g, o = _pair(mesh, fixed, e, coarse_cells=100, rtol=1e-9, maxiter=2000, smoother=smoother)
+    u = g.solve_many([f])[0]
+    uo = o.solve_many([f])[0]
+    assert np.linalg.norm(u - ref) <= 1e-6 * np.linalg.norm(ref)
+    assert abs(g.last_iters - o.last_iters) <= 2, (g.last_iters, o.last_iters)
+    # warm start after a moduli change (mg.py:881-896)
+    g.set_moduli(e * 1.1)
+    o.set_moduli(e * 1.1)
+    w = g.solve_many([f], [u])[0]
+    assert np.linalg.norm(w - ref / 1.1) <= 1e-6 * np.linalg.norm(ref)
+
+
+def test_batched_rhs_equal_single(gpu):
+    mesh, fixed, f, e = cantilever_bar(16, 8, 8, seed=3)
+    rng = np.random.default_rng(4)
+    fs = [f, rng.standard_normal(mesh.n_dof) * ~fixed, -2.0 * f]
+    from loadpath_amd.mg import GridElasticity
+
+    g = GridElasticity(mesh, 0.3, fixed, rtol=1e-10, coarse_cells=100, maxiter=2000, max_rhs=4)
+    g.set_moduli(e)
+    many = g.solve_many(fs)
+    for fk, uk in zip(fs, many):
+        one = g.solve_many([fk])[0]
+        assert np.linalg.norm(uk - one) <= 1e-8 * np.linalg.norm(one)
+    assert np.linalg.norm(many[2] + 2.0 * many[0]) <= 1e-8 * np.linalg.norm(many[2])
+
+
+def test_strain_energy_matches_oracle(gpu):
+    mesh, fixed, f, e = _voided()
+    g, o = _pair(mesh, fixed, e, coarse_cells=20)
+    u = np.random.default_rng(9).standard_normal(mesh.n_dof)
+    assert np.allclose(g.strain_energy(u), o.strain_energy(u), rtol=1e-12, atol=1e-12)
+
+
+def test_transfers_match_oracle_and_are_adjoint(gpu):
+    from loadpath_amd.mg import prolong, restrict
+    from oracle import mg as om
+
+    rng = np.random.default_rng(1)
+    c = rng.random((3, 4, 5, 6))
+    fgrid = rng.random((3, 7, 9, 11))
+    pc, rf = prolong(c), restrict(fgrid)
+    assert np.allclose(pc, om.prolong(c), rtol=1e-15, atol=1e-15)
+    assert np.allclose(rf, om.restrict(fgrid), rtol=1e-14, atol=1e-14)
+    assert abs(np.vdot(pc, fgrid) - np.vdot(c, rf)) < 1e-10
+    x = np.arange(4.0)[None, :, None, None] * np.ones((3, 4, 5, 6))
+    assert np.array_equal(prolong(x)[0, :, 0, 0], np.arange(7) / 2.0)
+
+
+@pytest.mark.parametrize("r", [1.5, 2.0, 2.5])
+def test_filter_matches_oracle(gpu, r):
+    from loadpath_amd.simp import GridFilter
+    from oracle.simp import GridFilter as OF
+
+    dims = (9, 6, 7)
+    v = np.random.default_rng(2).random(int(np.prod(dims)))
+    h = GridFilter(dims, r)
+    ho = OF(dims, r, pad=False)
+    assert len(h.taps) == {1.5: 19, 2.0: 27, 2.5: 81}[r]
+    assert np.allclose(h @ v, ho @ v, rtol=1e-14, atol=1e-14)
