@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Headline benchmark: SIMP iterations per second on the 256x128x128 hex8
+multi-load workload (8 load cases, bottom face clamped, vf 0.15, rmin 2.5),
+mixed-precision MG-PCG vs plain FP64 on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+One step = one SIMP iteration on the device: moduli, hierarchy rebuild,
+8 right-hand sides solved in one batched PCG (rtol 1e-8), sensitivities,
+density filter, OC update with its volume bisection. Prints ONE JSON line.
+`value` is the mixed path's iterations/s; the same iterations are run in
+plain FP64 from the same start, and the largest relative difference in
+compliance between the two is reported (`compliance_rel_err`).
+
+`roofline` is the dominant kernel's (the V-cycle's fine-level operator)
+algorithmic bytes / device time, summed over every launch in the timed
+region (HIP events on the grid's stream). `cpu_baseline` times the FP64
+numpy oracle on a bounded sample (see its "sample" text).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+_CORES = len(os.sched_getaffinity(0))
+for _v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+    os.environ[_v] = str(_CORES)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = dict(nx=256, ny=128, nz=128, cases=8)
+
+
+def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None):
+    import ctypes as C
+
+    import numpy as np
+
+    from loadpath_amd import _lib
+    from loadpath_amd.problems import multiload_256x128x128
+    from loadpath_amd.simp import GridSimp
+
+    p = multiload_256x128x128(nx, ny, nz)
+    s = GridSimp(p, precision=precision, smoother="chebyshev", rtol=1e-8, maxiter=500, max_levels=6,
+                 coarse_cells=0, element="im", filter="top3d")
+    hist = []
+    for k in range(warmup):
+        hist += s.iterate(1)
+        if log:
+            log(f"  {precision} warmup {k}: {hist[-1]}")
+    lib = _lib.lib()
+    eb = 8 if precision == "fp64" else 4
+    if profile:
+        _lib.check(lib.lp_profile(s.grid._h, 0, eb, 1), "profile")
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    timed = s.iterate(steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    if log:
+        for k, st in enumerate(timed):
+            log(f"  {precision} step {k}: {st}")
+    prof = None
+    if profile:
+        n = C.c_longlong()
+        ms = C.c_double()
+        by = C.c_double()
+        rc = lib.lp_profile_stats(s.grid._h, C.byref(n), C.byref(ms), C.byref(by))
+        prof = dict(launches=n.value, ms=ms.value, bytes=by.value, complete=(rc == 0))
+        lib.lp_profile(s.grid._h, 0, eb, 0)
+    levels = [lv.dims for lv in s.grid.levels]
+    del s
+    return dict(seconds=dt, iters_per_s=steps / dt, hist=hist + timed, timed=timed, prof=prof, levels=levels)
+
+
+def cpu_baseline(gpu_timed, log=None):
+    """The FP64 numpy oracle on 1/8 of the cells (128x64x64, same supports
+    and load construction): hierarchy build + 2 PCG iterations of one load
+    case, timed; a SIMP iteration at 256x128x128 is extrapolated as
+    8 x (build + PCG-iteration time x the CG iterations all 8 cases took in
+    the GPU run's timed iterations)."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from loadpath_amd.problems import multiload_256x128x128
+    from oracle.fem import box_mesh
+    from oracle.mg import GridOracle
+
+    nx, ny, nz = 128, 64, 64
+    p = multiload_256x128x128(nx, ny, nz)
+    mesh = box_mesh(nx, ny, nz)
+    fg = p.fixed_grid()
+    fdof = np.empty(mesh.n_dof, bool)
+    for i in range(3):
+        fdof[i::3] = fg[i].ravel()
+    g = GridOracle(mesh, 0.3, fdof, rtol=1e-8, maxiter=2, coarse_cells=200)
+    e = np.full(mesh.n_elem, 1e-9 + 0.15 ** 3 * (1 - 1e-9))
+    t0 = time.perf_counter()
+    g.set_moduli(e)
+    t_build = time.perf_counter() - t0
+    b = p.load_grids()[0] * g.levels[0].mask
+    t0 = time.perf_counter()
+    _, it, _ = g._pcg(b, None)
+    t_it = (time.perf_counter() - t0) / max(it, 1)
+    cg_total = float(np.mean([st["cg_iters_total"] for st in gpu_timed]))
+    t_simp = 8.0 * (t_build + t_it * cg_total)
+    if log:
+        log(f"  cpu: build {t_build:.2f}s, {t_it:.3f}s per PCG iteration at 1/8 scale, {cg_total:.0f} CG its/SIMP it")
+    return dict(value=1.0 / t_simp, unit="SIMP iter/s", cores=_CORES, kind="port",
+                sample=f"oracle (numpy FP64 CG + FP32 V-cycle) at 128x64x64 (1/8 of the cells): hierarchy build "
+                       f"{t_build:.1f}s + {it} PCG iterations ({t_it:.2f}s each) of one load case, extrapolated x8 "
+                       f"by cells and by the {cg_total:.0f} CG iterations per SIMP iteration of the 8 cases")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--size", type=int, nargs=3, default=None, help="override the grid (testing)")
+    ap.add_argument("-v", "--verbose", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the multi-load solve shards by load case with no exchange during the "
+              "solve (see DESIGN.md); runs on more than one GPU are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+    log = (lambda m: print(m, file=sys.stderr, flush=True)) if a.verbose else None
+    nx, ny, nz = a.size or (WORKLOAD["nx"], WORKLOAD["ny"], WORKLOAD["nz"])
+
+    mixed = run_gpu("mixed", a.steps, a.warmup, nx, ny, nz, profile=True, log=log)
+    fp64 = None if a.no_fp64 else run_gpu("fp64", a.steps, a.warmup, nx, ny, nz, log=log)
+    err = None
+    if fp64 is not None:
+        err = max(abs(m["compliance"] - f["compliance"]) / abs(f["compliance"])
+                  for m, f in zip(mixed["hist"], fp64["hist"]))
+    prof = mixed["prof"]
+    roof = None
+    if prof and prof["launches"]:
+        gbs = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9
+        roof = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
+                    traffic=None, kernel="k_apply_fine<float> (V-cycle fine-level operator + smoother epilogue)",
+                    launches=prof["launches"], avg_us=round(prof["ms"] * 1e3 / prof["launches"], 2),
+                    avg_bytes=round(prof["bytes"] / prof["launches"]))
+    cpu = None if a.no_cpu else cpu_baseline(mixed["timed"], log)
+    ms = 1e3 * mixed["seconds"] / a.steps
+    out = {
+        "metric": "SIMP iter/s at 256x128x128 hex8, mixed vs FP64 GPU, <=1e-6 rel compliance err",
+        "value": round(mixed["iters_per_s"], 4),
+        "unit": "SIMP iter/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp64 CG + fp32 V-cycle",
+        "data": "synthetic (seeded load cases, uniform start density)",
+        "config": {"workload": f"multiload_{nx}x{ny}x{nz}: 8 load cases (unit force, seeded direction, 8x8-node top "
+                               "patch), bottom face clamped, vf 0.15, rmin 2.5, SIMP p=3 Emin 1e-9, top3d density "
+                               "filter + OC bisection (move 0.2), batched MG-PCG (6 levels, Chebyshev(2), rtol 1e-8)",
+                   "cells": nx * ny * nz, "load_cases": 8, "levels": mixed["levels"]},
+        "fp64_value": round(fp64["iters_per_s"], 4) if fp64 else None,
+        "speedup_vs_fp64": round(mixed["iters_per_s"] / fp64["iters_per_s"], 3) if fp64 else None,
+        "compliance_rel_err": err,
+        "cg_iters_per_step": [st["cg_iters_total"] for st in mixed["timed"]],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -129,6 +129,7 @@ typedef struct lp_simp_stats {
     double change;        /* max |xnew - x| */
     double volume;        /* mean xphys after the update */
     int cg_iters;         /* max PCG iterations over the load cases */
+    int cg_iters_total;   /* PCG iterations summed over the load cases */
     int oc_evals;         /* bisection evaluations */
     double seconds;
 } lp_simp_stats;
@@ -158,6 +159,21 @@ int lp_version(void);
 /* average device time (ms) of the most recent timed kernel calls: see
  * lp_time_apply; used by bench.py for the roofline figures. */
 int lp_time_apply(lp_grid* g, int level, int dtype_bytes, int nrhs, int reps, double* ms_per_apply);
+
+/* Launch profiling (no reference counterpart: measurement for the
+ * roofline). While enabled, every launch of the operator at (level,
+ * dtype_bytes) is bracketed by HIP events on the grid's stream (up to 8192
+ * launches); lp_profile_stats returns their count, summed device time and
+ * summed algorithmic bytes (node vectors read and written by the launch,
+ * cell moduli, mask at one bit per dof). Returns LP_NOT_CONVERGED if
+ * launches were dropped past the event pool. */
+int lp_profile(lp_grid* g, int level, int dtype_bytes, int enable);
+int lp_profile_stats(lp_grid* g, long long* launches, double* total_ms, double* total_bytes);
+
+/* Launch-shape tuning (no reference counterpart: the autotuner's knob).
+ * param 0: fine-level nodes per thread along y (1 or 2); 1: fine-level
+ * nodes per thread along x (x-chunk, 1..64). Returns the previous value. */
+int lp_tune(lp_grid* g, int param, int value);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,7 @@ class SimpStats(C.Structure):
         ("change", C.c_double),
         ("volume", C.c_double),
         ("cg_iters", C.c_int),
+        ("cg_iters_total", C.c_int),
         ("oc_evals", C.c_int),
         ("seconds", C.c_double),
     ]
@@ -86,6 +87,9 @@ _SIGS = {
     "lp_last_error": (C.c_char_p, []),
     "lp_version": (_I, []),
     "lp_time_apply": (_I, [_P, _I, _I, _I, _I, _P]),
+    "lp_tune": (_I, [_P, _I, _I]),
+    "lp_profile": (_I, [_P, _I, _I, _I]),
+    "lp_profile_stats": (_I, [_P, _P, _P, _P]),
 }
 
 EXPORTS = tuple(_SIGS)

@@ -57,6 +57,8 @@ def _compile(src: Path, verbose: bool) -> Path:
 
 def build(verbose: bool = False, jobs: int | None = None) -> Path:
     BUILD.mkdir(exist_ok=True)
+    # the Ke symmetry header the fine kernels compile against
+    subprocess.run([sys.executable, str(CSRC / "gen_ke_pattern.py")], check=True, capture_output=not verbose)
     srcs = sources()
     jobs = jobs or min(8, len(srcs))
     with cf.ThreadPoolExecutor(jobs) as ex:

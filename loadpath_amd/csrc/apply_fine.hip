@@ -23,22 +23,72 @@ AGeom make_apply_geom(const Grid& Gd, int level) {
     G.fnx = lv.fine.nx; G.fny = lv.fine.ny; G.fnz = lv.fine.nz;
     G.nn = lv.g.nn;
     G.ncell = lv.g.ncell;
-    G.xchunk = lv.kind == LK_FINE ? 8 : 4;
-    G.nyzb = (G.Ny * G.Nz + kBlock - 1) / kBlock;
-    G.nxc = (G.Nx + G.xchunk - 1) / G.xchunk;
-    G.nblk = G.nyzb * G.nxc;
-    G.per = (G.nblk + 7) / 8;
+    if (lv.kind == LK_FINE) {
+        G.ny_per = 1;
+        G.xchunk = Gd.tune_fine_xchunk > 0 ? Gd.tune_fine_xchunk : 16;
+        G.nyzb = (G.Ny * G.Nz + kBlock - 1) / kBlock;
+        G.nxc = (G.Nx + G.xchunk - 1) / G.xchunk;
+        G.nblk = G.nyzb * G.nxc;
+        G.per = (G.nblk + 7) / 8;
+    } else {
+        G.ny_per = 1;
+        G.xchunk = 1;
+        G.nblk = (int)((G.nn + (kBlock / 8) - 1) / (kBlock / 8));
+        G.per = G.nblk;
+    }
     return G;
 }
 
-int apply_blocks(const Grid& Gd, int level) { return 8 * make_apply_geom(Gd, level).per; }
+int apply_blocks(const Grid& Gd, int level) {
+    const AGeom G = make_apply_geom(Gd, level);
+    return Gd.lv[level].kind == LK_FINE ? 8 * G.per : G.nblk;
+}
+
+// algorithmic bytes of one operator launch: the node vectors the epilogue
+// reads and writes, the input, the cell moduli (fine level) and the mask at
+// one bit per dof (the per-apply unit of work's compulsory traffic)
+static double apply_bytes(const Grid& G, int level, int eb, int nrhs, int epi) {
+    int vecs = 2;
+    switch (epi) {
+        case EPI_CHEB: vecs = 7; break;
+        case EPI_RESID_SCALED: vecs = 5; break;
+        case EPI_RESID_MASK: vecs = 3; break;
+        case EPI_JACOBI: vecs = 4; break;
+        case EPI_POWER: vecs = 3; break;
+        case EPI_RESID: vecs = 3; break;
+        default: vecs = 2; break;
+    }
+    const Level& L = G.lv[level];
+    double b = (double)nrhs * vecs * 3.0 * (double)L.g.nn * eb + 3.0 * (double)L.g.nn / 8.0;
+    if (L.kind == LK_FINE) b += (double)L.g.ncell * eb;
+    return b;
+}
 
 int launch_apply(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs, const EpiArgs& E,
                  hipStream_t s) {
+    Grid::Prof& P = G.prof;
+    const bool rec = P.on && P.level == level && P.eb == eb;
+    if (rec) {
+        if (2 * (P.used + 1) > P.ev.size()) {
+            P.dropped++;
+        } else {
+            LP_HIP(hipEventRecord(P.ev[2 * P.used], s));
+        }
+    }
     switch (G.lv[level].kind) {
-        case LK_FINE: launch_apply_fine(G, level, eb, u, vstride, nrhs, E, s); break;
+        case LK_FINE:
+            if (eb == 4 && G.opt.precision == LP_PREC_MIXED)
+                launch_apply_fine_mfma(G, level, u, vstride, nrhs, E, s);
+            else
+                launch_apply_fine(G, level, eb, u, vstride, nrhs, E, s);
+            break;
         case LK_WEIGHTS: launch_apply_weights(G, level, eb, u, vstride, nrhs, E, s); break;
         default: launch_apply_explicit(G, level, eb, u, vstride, nrhs, E, s); break;
+    }
+    if (rec && 2 * (P.used + 1) <= P.ev.size()) {
+        LP_HIP(hipEventRecord(P.ev[2 * P.used + 1], s));
+        P.bytes.push_back(apply_bytes(G, level, eb, nrhs, E.kind));
+        P.used++;
     }
     return apply_blocks(G, level);
 }

@@ -4,26 +4,32 @@
 // numba kernels mgops._product_* + _scatter_into (mgops.py:66-217).
 //
 // MI355X design. The reference forms the (24, cells) product buffer and
-// scatters it node by node. Here every thread owns one node column position
-// (y, z) and marches along x: at each node it gathers the 27 neighbour nodes
-// (three node planes held in registers as a sliding window, so each plane is
-// loaded once per column) and sums, over the node's 8 cells, the 3 rows of
-// the cell matrix that belong to the node's corner times the cell's 24
-// corner values. No product buffer, no atomics, one writer per node, and the
-// node's vector update (CG / smoother step) runs in the same pass as an
-// epilogue. Lanes run along the flattened (y, z) plane, so every load and
-// store of a wave is a contiguous run of z. Blocks are remapped so each XCD
-// (8 on MI355X, round-robin block dispatch) owns a contiguous range of
-// (x-chunk, yz) tiles and the halo rows stay in its own L2.
+// scatters it node by node. Here each node's value is summed where it is
+// written, with no product buffer, no atomics and one writer per node, and
+// the node's vector update (CG / smoother step) runs in the same pass as an
+// epilogue.
 //
-// Cell matrices: "fine" e_c * Ke (Ke wave-uniform, scalar loads); "weights"
-// (first coarse level) sum_q w_q M_q with the 8 child moduli read from the
-// finer level's E and M_q staged in LDS, or an explicit patch matrix for
-// cells touching held dofs (mg.py:609-642); "explicit" K_c (576, cells) for
-// deeper levels (mg.py:643-666).
+// Fine level (cell matrix e_c * Ke): every thread owns NY nodes of one
+// (y, z) column position and marches along x one cell layer at a time. A
+// layer's cells (2 x (NY+1) of them) give the rows of the node below them
+// (finishing it) and of the node above (starting it), so only two node
+// planes are live in registers. Ke is wave-uniform and constant: it is read
+// through the constant address space into SGPRs and used as the FMA's scalar
+// operand; with NY = 2 each scalar value feeds two nodes. Lanes run along z,
+// so every load and store of a wave is a contiguous run. Blocks are
+// remapped so each XCD (8 on MI355X, round-robin dispatch) owns a contiguous
+// range of (x-chunk, yz) tiles and the halo rows stay in its own L2.
+//
+// Coarse levels ("weights": sum_q E_child(q) M_q read on the fly, plus the
+// explicit patch of cells touching held dofs, mg.py:609-642; "explicit":
+// K_c stored (576, cells), mg.py:643-666): one thread per (node, cell of
+// its 8) computes the node's 3 rows of that cell's matrix; the 8 lanes of a
+// node reduce by shuffles. 8x the parallelism of a node per thread, which
+// the small coarse grids need, and coalesced K_c reads.
 //
 // This header is included by one translation unit per cell-matrix kind.
 #pragma once
+#include "ke_pattern.h"
 #include "lp_internal.h"
 
 namespace lp {
@@ -33,6 +39,13 @@ struct AGeom {
     int fnx, fny, fnz;      // finer level cells (weights kind)
     size_t nn, ncell;
     int xchunk, nyzb, nxc, nblk, per;
+    int ny_per;             // nodes per thread along y (fine kernel)
+};
+
+// Ke as its KE_NV orbit values (see ke_pattern.h): kernel arguments, so
+// they sit in SGPRs for the whole launch
+template <typename T> struct KeVals {
+    T v[KE_NV];
 };
 
 template <typename T> struct CSrc {
@@ -49,15 +62,26 @@ AGeom make_apply_geom(const Grid& Gd, int level);
 
 namespace apply_detail {
 
-template <int N> struct IC { static constexpr int v = N; };
-
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     return v;
 }
 
-// node epilogue: y = finished operator value at node n (all 3 components)
+__device__ __forceinline__ void block_partial(double part, EpiArgs& E, int rhs, int L) {
+    __shared__ double sred[8];
+    part = wave_sum(part);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) sred[wv] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sred[i];
+        E.partials[(size_t)rhs * gridDim.x + L] = s;
+    }
+}
+
+// node epilogue: yv = finished operator value at node n (3 components)
 template <typename T>
 __device__ __forceinline__ void epilogue(const EpiArgs& E, int m, size_t o, size_t nn, size_t n, const T (&yv)[3],
                                          const T (&ur)[3], double& part) {
@@ -119,224 +143,343 @@ __device__ __forceinline__ void epilogue(const EpiArgs& E, int m, size_t o, size
     }
 }
 
-template <typename T, int KIND>
-__global__ void __launch_bounds__(256) k_apply(const T* __restrict__ u0, size_t vstride,
-                                               const uint8_t* __restrict__ mask, AGeom G, CSrc<T> C,
-                                               EpiArgs E) {
-    __shared__ T smq[KIND == LK_WEIGHTS ? 8 * 576 : 1];
-    __shared__ double sred[4];
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+
+// Raw buffer loads: 32-bit byte offsets against an SGPR base, and the
+// hardware range check returns 0 for an offset past num_records, so halo
+// positions outside the grid load as zero with no branch. kOOB is past any
+// buffer here (all < 2 GiB per right-hand side).
+constexpr unsigned kOOB = 0x80000000u;
+constexpr int kRsrcWord3 = 0x00020000;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, kRsrcWord3);
+}
+template <typename T> __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <> __device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ int bload_u8(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// fine level: LDS-staged cell-layer march
+// ---------------------------------------------------------------------------
+// A block owns 256 consecutive nodes of the flattened (y, z) plane (one per
+// thread) and an x-chunk. Three x-planes of its range plus a halo of Nz+1
+// nodes on either side are staged in LDS (masked u, 3 components padded to
+// 4); the plane two layers ahead is loaded into registers while the current
+// layer computes, then stored, with one barrier per layer. Each thread takes
+// its 4 cells of the layer one at a time (24 values live): the lo rows
+// finish the node below the layer, the hi rows start the node above.
+template <typename T>
+struct FineSmem {
+    static constexpr int kPad = 4;
+};
+
+template <typename T, bool SYM>
+__global__ void __launch_bounds__(256) k_apply_fine(const T* __restrict__ u0, size_t vstride,
+                                                    const uint8_t* __restrict__ mask, AGeom G,
+                                                    const T* __restrict__ ecell, const T* __restrict__ ke,
+                                                    KeVals<T> kv, EpiArgs E) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* lds = reinterpret_cast<T*>(smem_raw);
     const int rhs = blockIdx.z;
     const int L = blockIdx.x;
     const int logical = (L & 7) * G.per + (L >> 3);
     double part = 0.0;
     const bool skip = (E.scal != nullptr && !E.scal[rhs].active) || logical >= G.nblk;
-    if (KIND == LK_WEIGHTS) {
-        for (int i = threadIdx.x; i < 8 * 576; i += blockDim.x) smq[i] = C.mq[i];
-        __syncthreads();
-    }
     const bool need_raw_all = E.kind == EPI_STORE_DOT || E.kind == EPI_JACOBI || E.kind == EPI_POWER;
     if (!skip) {
         const int xc = logical / G.nyzb;
         const int yb = logical - xc * G.nyzb;
         const int Ny = G.Ny, Nz = G.Nz, Nx = G.Nx;
         const int nyz = Ny * Nz;
-        const int t = yb * blockDim.x + threadIdx.x;
-        if (t < nyz) {
-            const int y = t / Nz;
-            const int z = t - y * Nz;
-            const int x0 = xc * G.xchunk;
-            const int x1 = min(x0 + G.xchunk, Nx);
-            const T* __restrict__ u = u0 + (size_t)rhs * vstride;
-            const size_t nn = G.nn;
-            const bool vy[3] = {y > 0, true, y + 1 < Ny};
-            const bool vz[3] = {z > 0, true, z + 1 < Nz};
-            const bool cyv[2] = {y > 0, y < G.ny};
-            const bool czv[2] = {z > 0, z < G.nz};
+        const int tbase = yb * (int)blockDim.x;
+        const int t = tbase + (int)threadIdx.x;
+        const bool active = t < nyz;
+        const int y = active ? t / Nz : 0;
+        const int z = active ? t - y * Nz : 0;
+        const int x0 = xc * G.xchunk;
+        const int x1 = min(x0 + G.xchunk, Nx);
+        const T* __restrict__ u = u0 + (size_t)rhs * vstride;
+        const unsigned nnu = (unsigned)G.nn;
+        const int halo = Nz + 1;
+        const int W = (int)blockDim.x + 2 * halo;         // staged nodes per plane
+        constexpr int PAD = FineSmem<T>::kPad;
+        const int planeT = W * PAD;
+        const __amdgpu_buffer_rsrc_t ru = make_rsrc(u, 3u * nnu * (unsigned)sizeof(T));
+        const __amdgpu_buffer_rsrc_t rm = make_rsrc(mask, nnu);
+        const __amdgpu_buffer_rsrc_t re = make_rsrc(ecell, (unsigned)G.ncell * (unsigned)sizeof(T));
+        const bool cyv[2] = {active && y > 0, active && y < G.ny};
+        const bool czv[2] = {active && z > 0, active && z < G.nz};
+        constexpr int MAXE = 4;                            // staged entries per thread (W <= 4 * 256)
+        T pre[MAXE][3];
 
-            T U[3][9][3];
-            T Ec[3][4];
-
-            auto load_plane = [&](auto pslot, int x) {
-                constexpr int P = decltype(pslot)::v;
-                const bool inx = x >= 0 && x < Nx;
+        // load plane x's staged range into registers (masked), zero outside
+        auto fetch = [&](int x) {
+            const bool inx = x >= 0 && x < Nx;
 #pragma unroll
-                for (int dy = 0; dy < 3; ++dy)
+            for (int k = 0; k < MAXE; ++k) {
+                const int e = (int)threadIdx.x + k * (int)blockDim.x;
+                const int f = tbase - halo + e;
+                const bool ok = inx && e < W && f >= 0 && f < nyz;
+                const unsigned n = (unsigned)(x * nyz + f);
+                const int m = bload_u8(rm, ok ? n : kOOB);
 #pragma unroll
-                    for (int dz = 0; dz < 3; ++dz) {
-                        const bool ok = inx && vy[dy] && vz[dz];
-                        const size_t n = (size_t)x * nyz + t + (dy - 1) * Nz + (dz - 1);
-                        const int m = ok ? (int)mask[n] : 0;
+                for (int c = 0; c < 3; ++c)
+                    pre[k][c] = bload<T>(ru, ((m >> c) & 1) ? (unsigned)((c * nnu + n) * sizeof(T)) : kOOB);
+            }
+        };
+        auto store = [&](int buf) {
+            T* p = lds + buf * planeT;
 #pragma unroll
-                        for (int c = 0; c < 3; ++c)
-                            U[P][dy * 3 + dz][c] = ((m >> c) & 1) ? u[c * nn + n] : T(0);
-                    }
-            };
-            auto load_cells = [&](auto pslot, int X) {
-                constexpr int P = decltype(pslot)::v;
-                if (KIND == LK_FINE) {
-                    const bool inx = X >= 0 && X < G.nx;
+            for (int k = 0; k < MAXE; ++k) {
+                const int e = (int)threadIdx.x + k * (int)blockDim.x;
+                if (e < W) {
 #pragma unroll
-                    for (int cy = 0; cy < 2; ++cy)
-#pragma unroll
-                        for (int cz = 0; cz < 2; ++cz) {
-                            const bool ok = inx && cyv[cy] && czv[cz];
-                            const size_t ci = ((size_t)X * G.ny + (y - 1 + cy)) * G.nz + (z - 1 + cz);
-                            Ec[P][cy * 2 + cz] = ok ? C.e[ci] : T(0);
-                        }
+                    for (int c = 0; c < 3; ++c) p[e * PAD + c] = pre[k][c];
                 }
-            };
+            }
+        };
 
-            auto node = [&](auto p0, auto p1, auto p2, int x) {
-                constexpr int P0 = decltype(p0)::v, P1 = decltype(p1)::v, P2 = decltype(p2)::v;
-                // Ke is wave-uniform and constant for the launch: read it
-                // through the constant address space (scalar loads into SGPRs,
-                // used directly as the FMA's scalar operand), one cell's 72
-                // values at a time; the opaque pointer keeps the compiler from
-                // hoisting all 576 values out of the x loop.
-                unsigned long long kea = (unsigned long long)C.ke;
-                T acc0 = 0, acc1 = 0, acc2 = 0;
+        T acc0 = 0, acc1 = 0, acc2 = 0, nx0 = 0, nx1 = 0, nx2 = 0;
+        unsigned long long kea = (unsigned long long)ke;
+        // cell layer X between staged planes in buffers bl (X) and bh (X+1)
+        auto layer = [&](int X, int bl, int bh, bool lo) {
+            nx0 = 0; nx1 = 0; nx2 = 0;
+            const bool inx = X >= 0 && X < G.nx;
+            const T* PL = lds + bl * planeT;
+            const T* PH = lds + bh * planeT;
+            const int me = (int)threadIdx.x + halo;
 #pragma unroll
-                for (int cx = 0; cx < 2; ++cx)
+            for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
-                    for (int cy = 0; cy < 2; ++cy)
+                for (int cz = 0; cz < 2; ++cz) {
+                    // one cell's 24 values live at a time: keep the scheduler
+                    // from hoisting the next cell's LDS reads above this one
+                    asm volatile("" ::: "memory");
+                    const bool ok = inx && cyv[cy] && czv[cz];
+                    const unsigned ci = (unsigned)((X * G.ny + (y - 1 + cy)) * G.nz + (z - 1 + cz));
+                    const T ec = bload<T>(re, ok ? ci * (unsigned)sizeof(T) : kOOB);
+                    T ub[24];
 #pragma unroll
-                        for (int cz = 0; cz < 2; ++cz) {
-                            const int a = (1 - cx) | ((1 - cy) << 1) | ((1 - cz) << 2);
-                            const int X = x - 1 + cx;
-                            const bool cok = X >= 0 && X < G.nx && cyv[cy] && czv[cz];
-                            const size_t ci = cok ? ((size_t)X * G.ny + (y - 1 + cy)) * G.nz + (z - 1 + cz) : 0;
-                            T t0 = 0, t1 = 0, t2 = 0;
-                            // t[i] += sum_{b,j} K(3a+i, 3b+j) u_b[j]
-                            auto rows = [&](auto&& K) {
+                    for (int b = 0; b < 8; ++b) {
+                        const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
+                        const int idx = me + (cy + by - 1) * Nz + (cz + bz - 1);
+                        const T* q = (bx ? PH : PL) + idx * PAD;
 #pragma unroll
-                                for (int b = 0; b < 8; ++b) {
-                                    const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
-                                    const int pl = cx + bx;
-                                    const int w = (cy + by) * 3 + (cz + bz);
+                        for (int c = 0; c < 3; ++c) ub[3 * b + c] = q[c];
+                    }
+                    if (!SYM) asm volatile("" : "+s"(kea));
+                    cptr<T> Kp = (cptr<T>)kea;
+                    auto K = [&](int e) -> T {
+                        if (SYM) return KE_SGN[e] > 0 ? kv.v[KE_IDX[e]] : (KE_SGN[e] < 0 ? -kv.v[KE_IDX[e]] : T(0));
+                        return Kp[e];
+                    };
 #pragma unroll
-                                    for (int j = 0; j < 3; ++j) {
-                                        const T uv = pl == 0 ? U[P0][w][j] : (pl == 1 ? U[P1][w][j] : U[P2][w][j]);
-                                        t0 += K(3 * a + 0, 3 * b + j) * uv;
-                                        t1 += K(3 * a + 1, 3 * b + j) * uv;
-                                        t2 += K(3 * a + 2, 3 * b + j) * uv;
-                                    }
-                                }
-                            };
-                            if (KIND == LK_FINE) {
-                                asm volatile("" : "+s"(kea));
-                                const __attribute__((address_space(4))) T* kep =
-                                    (const __attribute__((address_space(4))) T*)kea;
-                                rows([&](int i, int j) { return kep[i * 24 + j]; });
-                                const T e = cx == 0 ? Ec[P0][cy * 2 + cz] : Ec[P1][cy * 2 + cz];
-                                acc0 += e * t0;
-                                acc1 += e * t1;
-                                acc2 += e * t2;
-                            } else if (KIND == LK_EXPLICIT) {
-                                if (cok) {
-                                    const T* __restrict__ kc = C.k + ci;
-                                    const size_t s = G.ncell;
-                                    rows([&](int i, int j) { return kc[(size_t)(i * 24 + j) * s]; });
-                                    acc0 += t0;
-                                    acc1 += t1;
-                                    acc2 += t2;
-                                }
-                            } else {  // weights
-                                if (cok) {
-                                    const int pi = C.pidx[ci];
-                                    if (pi >= 0) {
-                                        const T* __restrict__ kp = C.patch + (size_t)pi * 576;
-                                        rows([&](int i, int j) { return kp[i * 24 + j]; });
-                                        acc0 += t0;
-                                        acc1 += t1;
-                                        acc2 += t2;
-                                    } else {
-                                        const int I = X, J = y - 1 + cy, Kc = z - 1 + cz;
-#pragma unroll 1
-                                        for (int q = 0; q < 8; ++q) {
-                                            const int fx = 2 * I + (q & 1), fy = 2 * J + ((q >> 1) & 1),
-                                                      fz = 2 * Kc + ((q >> 2) & 1);
-                                            const bool ok = fx < G.fnx && fy < G.fny && fz < G.fnz;
-                                            const T wq = ok ? C.fe[((size_t)fx * G.fny + fy) * G.fnz + fz] : T(0);
-                                            if (wq != T(0)) {
-                                                t0 = 0; t1 = 0; t2 = 0;
-                                                const T* mq = smq + q * 576;
-                                                rows([&](int i, int j) { return mq[i * 24 + j]; });
-                                                acc0 += wq * t0;
-                                                acc1 += wq * t1;
-                                                acc2 += wq * t2;
-                                            }
-                                        }
-                                    }
-                                }
-                            }
+                    for (int side = 0; side < 2; ++side) {
+                        if (side == 0 && !lo) continue;
+                        const int a = side | ((1 - cy) << 1) | ((1 - cz) << 2);
+                        T t0 = 0, t1 = 0, t2 = 0;
+#pragma unroll
+                        for (int j = 0; j < 24; ++j) {
+                            t0 += K((3 * a + 0) * 24 + j) * ub[j];
+                            t1 += K((3 * a + 1) * 24 + j) * ub[j];
+                            t2 += K((3 * a + 2) * 24 + j) * ub[j];
                         }
-                // finish (out = m*Ku + (1-m)*u) and epilogue at node (x, y, z)
-                const size_t n = (size_t)x * nyz + t;
+                        if (side == 0) {
+                            acc0 += ec * t0; acc1 += ec * t1; acc2 += ec * t2;
+                        } else {
+                            nx0 += ec * t0; nx1 += ec * t1; nx2 += ec * t2;
+                        }
+                    }
+                }
+        };
+
+        // prologue: planes x0-1, x0, x0+1 -> buffers 0, 1, 2
+        fetch(x0 - 1);
+        store(0);
+        fetch(x0);
+        store(1);
+        fetch(x0 + 1);
+        store(2);
+        __syncthreads();
+        layer(x0 - 1, 0, 1, false);
+        acc0 = nx0; acc1 = nx1; acc2 = nx2;
+        int bl = 1;
+        for (int X = x0; X < x1; ++X) {
+            const int bh = bl == 2 ? 0 : bl + 1;
+            const int bn = bh == 2 ? 0 : bh + 1;          // buffer of plane X-1, refilled with X+2
+            const bool more = X + 1 < x1;
+            if (more) fetch(X + 2);
+            layer(X, bl, bh, true);
+            if (active) {
+                const size_t n = (size_t)X * nyz + t;
                 const int m = mask[n];
                 T yv[3] = {acc0, acc1, acc2};
+                const T* q = lds + bl * planeT + ((int)threadIdx.x + halo) * PAD;
                 T ur[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const bool mc = (m >> c) & 1;
-                    ur[c] = (!mc || need_raw_all) ? u[c * nn + n] : U[P1][4][c];
+                    ur[c] = (!mc || need_raw_all) ? u[c * (size_t)nnu + n] : q[c];
                     if (!mc) yv[c] = ur[c];
                 }
-                epilogue<T>(E, m, (size_t)rhs * vstride + n, nn, n, yv, ur, part);
-            };
+                epilogue<T>(E, m, (size_t)rhs * vstride + n, G.nn, n, yv, ur, part);
+            }
+            acc0 = nx0; acc1 = nx1; acc2 = nx2;
+            if (more) {
+                store(bn);
+                __syncthreads();
+            }
+            bl = bh;
+        }
+    }
+    if (E.partials != nullptr) block_partial(part, E, rhs, L);
+}
 
-            load_plane(IC<0>{}, x0 - 1);
-            load_plane(IC<1>{}, x0);
-            load_cells(IC<0>{}, x0 - 1);
-            int x = x0;
-            while (true) {
-                load_plane(IC<2>{}, x + 1);
-                load_cells(IC<1>{}, x);
-                node(IC<0>{}, IC<1>{}, IC<2>{}, x);
-                if (++x >= x1) break;
-                load_plane(IC<0>{}, x + 1);
-                load_cells(IC<2>{}, x);
-                node(IC<1>{}, IC<2>{}, IC<0>{}, x);
-                if (++x >= x1) break;
-                load_plane(IC<1>{}, x + 1);
-                load_cells(IC<0>{}, x);
-                node(IC<2>{}, IC<0>{}, IC<1>{}, x);
-                if (++x >= x1) break;
+// ---------------------------------------------------------------------------
+// coarse levels: one thread per (node, cell), shuffle reduction over the 8
+// ---------------------------------------------------------------------------
+template <typename T, int KIND>
+__global__ void __launch_bounds__(256) k_apply_cells(const T* __restrict__ u0, size_t vstride,
+                                                     const uint8_t* __restrict__ mask, AGeom G, CSrc<T> C,
+                                                     EpiArgs E) {
+    __shared__ T smq[KIND == LK_WEIGHTS ? 8 * 576 : 1];
+    const int rhs = blockIdx.z;
+    const int L = blockIdx.x;
+    double part = 0.0;
+    const bool skip = E.scal != nullptr && !E.scal[rhs].active;
+    if (KIND == LK_WEIGHTS) {
+        for (int i = threadIdx.x; i < 8 * 576; i += blockDim.x) smq[i] = C.mq[i];
+        __syncthreads();
+    }
+    const bool need_raw_all = E.kind == EPI_STORE_DOT || E.kind == EPI_JACOBI || E.kind == EPI_POWER;
+    const int c8 = threadIdx.x & 7;
+    const size_t nid = (size_t)L * (blockDim.x >> 3) + (threadIdx.x >> 3);
+    const size_t nn = G.nn;
+    const bool valid = !skip && nid < nn;
+    T t0 = 0, t1 = 0, t2 = 0;
+    const T* __restrict__ u = u0 + (size_t)rhs * vstride;
+    int x = 0, y = 0, z = 0;
+    if (valid) {
+        z = (int)(nid % G.Nz);
+        y = (int)((nid / G.Nz) % G.Ny);
+        x = (int)(nid / ((size_t)G.Nz * G.Ny));
+        const int cx = c8 & 1, cy = (c8 >> 1) & 1, cz = (c8 >> 2) & 1;
+        const int X = x - 1 + cx, Y = y - 1 + cy, Z = z - 1 + cz;
+        if (X >= 0 && Y >= 0 && Z >= 0 && X < G.nx && Y < G.ny && Z < G.nz) {
+            const int a = (1 - cx) | ((1 - cy) << 1) | ((1 - cz) << 2);
+            const size_t ci = ((size_t)X * G.ny + Y) * G.nz + Z;
+            T ub[24];
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const size_t n = ((size_t)(X + (b & 1)) * G.Ny + (Y + ((b >> 1) & 1))) * G.Nz + (Z + ((b >> 2) & 1));
+                const int m = mask[n];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) ub[3 * b + q] = ((m >> q) & 1) ? u[q * nn + n] : T(0);
+            }
+            auto rows = [&](auto&& K, T& r0, T& r1, T& r2) {
+#pragma unroll
+                for (int j = 0; j < 24; ++j) {
+                    r0 += K((3 * a + 0) * 24 + j) * ub[j];
+                    r1 += K((3 * a + 1) * 24 + j) * ub[j];
+                    r2 += K((3 * a + 2) * 24 + j) * ub[j];
+                }
+            };
+            if (KIND == LK_EXPLICIT) {
+                const T* __restrict__ kc = C.k + ci;
+                const size_t s = G.ncell;
+                rows([&](int e) { return kc[(size_t)e * s]; }, t0, t1, t2);
+            } else {
+                const int pi = C.pidx[ci];
+                if (pi >= 0) {
+                    const T* __restrict__ kp = C.patch + (size_t)pi * 576;
+                    rows([&](int e) { return kp[e]; }, t0, t1, t2);
+                } else {
+#pragma unroll 1
+                    for (int q = 0; q < 8; ++q) {
+                        const int fx = 2 * X + (q & 1), fy = 2 * Y + ((q >> 1) & 1), fz = 2 * Z + ((q >> 2) & 1);
+                        const bool ok = fx < G.fnx && fy < G.fny && fz < G.fnz;
+                        const T wq = ok ? C.fe[((size_t)fx * G.fny + fy) * G.fnz + fz] : T(0);
+                        if (wq != T(0)) {
+                            T s0 = 0, s1 = 0, s2 = 0;
+                            const T* mq = smq + q * 576;
+                            rows([&](int e) { return mq[e]; }, s0, s1, s2);
+                            t0 += wq * s0;
+                            t1 += wq * s1;
+                            t2 += wq * s2;
+                        }
+                    }
+                }
             }
         }
     }
-    if (E.partials != nullptr) {
-        part = wave_sum(part);
-        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        if (lane == 0) sred[wv] = part;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double s = 0;
-            for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sred[i];
-            E.partials[(size_t)rhs * gridDim.x + L] = s;
-        }
+    // reduce the node's 8 cells (lanes c8 = 0..7 are adjacent)
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        t0 += __shfl_xor(t0, o, 64);
+        t1 += __shfl_xor(t1, o, 64);
+        t2 += __shfl_xor(t2, o, 64);
     }
+    if (valid && c8 == 0) {
+        const size_t n = nid;
+        const int m = mask[n];
+        T yv[3] = {t0, t1, t2};
+        T ur[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const bool mc = (m >> c) & 1;
+            ur[c] = (!mc || need_raw_all) ? u[c * nn + n] : T(0);
+            if (!mc) yv[c] = ur[c];
+        }
+        epilogue<T>(E, m, (size_t)rhs * vstride + n, nn, n, yv, ur, part);
+    }
+    if (E.partials != nullptr) block_partial(part, E, rhs, L);
 }
 
 template <typename T, int KIND>
 void launch_kind(const Grid& Gd, int level, const T* u, size_t vstride, int nrhs, const EpiArgs& E, hipStream_t s) {
     const Level& lv = Gd.lv[level];
     AGeom G = make_apply_geom(Gd, level);
-    CSrc<T> C{};
     const bool f64 = sizeof(T) == 8;
     if (KIND == LK_FINE) {
-        C.e = static_cast<const T*>(f64 ? lv.e64.p : lv.e32.p);
-        C.ke = static_cast<const T*>(f64 ? Gd.ke64.p : Gd.ke32.p);
-    } else if (KIND == LK_WEIGHTS) {
-        const Level& fl = Gd.lv[level - 1];
-        C.fe = static_cast<const T*>(f64 ? fl.e64.p : fl.e32.p);
-        C.mq = static_cast<const T*>(f64 ? Gd.mq64.p : Gd.mq32.p);
-        C.pidx = lv.patch_idx.as<int>();
-        C.patch = static_cast<const T*>(f64 ? lv.patch64.p : lv.patch32.p);
+        const T* e = static_cast<const T*>(f64 ? lv.e64.p : lv.e32.p);
+        const T* ke = static_cast<const T*>(f64 ? Gd.ke64.p : Gd.ke32.p);
+        dim3 grid(8 * G.per, 1, nrhs), block(kBlock);
+        KeVals<T> kv{};
+        for (int i = 0; i < KE_NV; ++i) kv.v[i] = (T)Gd.ke_orbit[i];
+        const uint8_t* m = lv.mask.as<uint8_t>();
+        const int W = kBlock + 2 * (G.Nz + 1);
+        if (W > 4 * kBlock) {
+            set_error("fine level: z extent too large for the staged plane");
+            throw ArgFailure();
+        }
+        const size_t smem = (size_t)3 * W * FineSmem<T>::kPad * sizeof(T);
+        if (Gd.ke_sym)
+            hipLaunchKernelGGL((k_apply_fine<T, true>), grid, block, smem, s, u, vstride, m, G, e, ke, kv, E);
+        else
+            hipLaunchKernelGGL((k_apply_fine<T, false>), grid, block, smem, s, u, vstride, m, G, e, ke, kv, E);
     } else {
-        C.k = static_cast<const T*>(f64 ? lv.k64.p : lv.k32.p);
+        CSrc<T> C{};
+        if (KIND == LK_WEIGHTS) {
+            const Level& fl = Gd.lv[level - 1];
+            C.fe = static_cast<const T*>(f64 ? fl.e64.p : fl.e32.p);
+            C.mq = static_cast<const T*>(f64 ? Gd.mq64.p : Gd.mq32.p);
+            C.pidx = lv.patch_idx.as<int>();
+            C.patch = static_cast<const T*>(f64 ? lv.patch64.p : lv.patch32.p);
+        } else {
+            C.k = static_cast<const T*>(f64 ? lv.k64.p : lv.k32.p);
+        }
+        dim3 grid(G.nblk, 1, nrhs), block(kBlock);
+        hipLaunchKernelGGL((k_apply_cells<T, KIND>), grid, block, 0, s, u, vstride, lv.mask.as<uint8_t>(), G, C, E);
     }
-    dim3 grid(8 * G.per, 1, nrhs), block(kBlock);
-    hipLaunchKernelGGL((k_apply<T, KIND>), grid, block, 0, s, u, vstride, lv.mask.as<uint8_t>(), G, C, E);
     LP_HIP(hipGetLastError());
 }
 

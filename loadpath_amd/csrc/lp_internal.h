@@ -118,7 +118,24 @@ struct Grid {
     // staging for host pointers
     DBuf stage_a, stage_b;
     int max_rhs = 1;
+    bool ke_sym = false;        // Ke has the cube's symmetry (ke_pattern.h): orbit values used
+    double ke_orbit[32] = {};
+    int tune_fine_ny = 1;       // fine-level nodes per thread along y (1 or 2)
+    int tune_fine_xchunk = 16;  // fine-level nodes per thread along x
     int vbytes() const { return opt.precision == LP_PREC_FP64 ? 8 : 4; }
+    // launch profiling of one level operator (HIP events on the grid's stream)
+    struct Prof {
+        bool on = false;
+        int level = 0, eb = 0;
+        std::vector<hipEvent_t> ev;      // pairs
+        std::vector<double> bytes;       // algorithmic bytes per recorded launch
+        size_t used = 0;
+        long long dropped = 0;
+    };
+    mutable Prof prof;
+    ~Grid() {
+        for (hipEvent_t e : prof.ev) hipEventDestroy(e);
+    }
 };
 
 // Per right-hand-side CG scalars, device resident.
@@ -170,6 +187,9 @@ void launch_apply_weights(const Grid& G, int level, int eb, const void* u, size_
                           const EpiArgs& E, hipStream_t s);
 void launch_apply_explicit(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs,
                            const EpiArgs& E, hipStream_t s);
+// fine level of the mixed-precision V-cycle: FP16 MFMA Ke contraction, FP32 vectors
+void launch_apply_fine_mfma(const Grid& G, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,
+                            hipStream_t s);
 
 // ---- vec.hip -------------------------------------------------------------
 void ensure_partials(Grid& G, size_t n);

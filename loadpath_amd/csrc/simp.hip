@@ -409,6 +409,8 @@ int lp_simp_iterate(lp_simp* S, int niter, lp_simp_stats* stats) {
             st.change = h[1];
             st.volume = 0;
             st.cg_iters = *std::max_element(iters.begin(), iters.end());
+            st.cg_iters_total = 0;
+            for (int k = 0; k < S->nrhs; ++k) st.cg_iters_total += iters[k];
             st.oc_evals = bh.evals;
             st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             if (!bh.done) {

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <mutex>
 
+#include "ke_pattern.h"
 #include "lp_internal.h"
 
 namespace lp {
@@ -403,11 +404,29 @@ int lp_grid_create(lp_grid** out, int nx, int ny, int nz, const uint8_t* fixed_g
             if (G->opt.max_rhs < 1) G->opt.max_rhs = 1;
             if (G->opt.degree < 1) G->opt.degree = 2;
             if (G->opt.maxiter < 1) G->opt.maxiter = 500;
-            if (G->opt.precision == LP_PREC_MIXED) G->opt.precision = LP_PREC_FP32;
             G->max_rhs = G->opt.max_rhs;
             G->g0.set(nx, ny, nz);
             LP_HIP(hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking));
             std::memcpy(G->ke, ke, sizeof(G->ke));
+            // Ke from its symmetry orbits (ke_pattern.h) when it has the cube's
+            // symmetry to rounding; otherwise the fine kernel reads all 576
+            {
+                double cnt[KE_NV] = {}, kmax = 0;
+                for (int e = 0; e < 576; ++e) {
+                    kmax = std::max(kmax, std::fabs(ke[e]));
+                    if (KE_SGN[e]) {
+                        G->ke_orbit[KE_IDX[e]] += KE_SGN[e] * ke[e];
+                        cnt[KE_IDX[e]] += 1;
+                    }
+                }
+                for (int v = 0; v < KE_NV; ++v) G->ke_orbit[v] /= std::max(cnt[v], 1.0);
+                double dev = 0;
+                for (int e = 0; e < 576; ++e) {
+                    const double r = KE_SGN[e] ? KE_SGN[e] * G->ke_orbit[KE_IDX[e]] : 0.0;
+                    dev = std::max(dev, std::fabs(r - ke[e]));
+                }
+                G->ke_sym = dev <= 1e-13 * kmax;
+            }
             const size_t nn = G->g0.nn;
             // fixed bits per node
             std::vector<uint8_t> fb(nn, 0);
@@ -687,6 +706,61 @@ int lp_restrict(const double* fine, double* coarse, int cx, int cy, int cz, int 
         else LP_HIP(hipDeviceSynchronize());
         return LP_OK;
     });
+}
+
+int lp_profile(lp_grid* g, int level, int dtype_bytes, int enable) {
+    return guarded([&]() -> int {
+        if (!g) return LP_ERR_ARG;
+        Grid::Prof& P = g->prof;
+        LP_HIP(hipStreamSynchronize(g->stream));
+        P.on = enable != 0;
+        P.level = level;
+        P.eb = dtype_bytes;
+        P.used = 0;
+        P.dropped = 0;
+        P.bytes.clear();
+        if (P.on && P.ev.empty()) {
+            P.ev.resize(2 * 8192);
+            for (hipEvent_t& e : P.ev) LP_HIP(hipEventCreate(&e));
+        }
+        return LP_OK;
+    });
+}
+
+int lp_profile_stats(lp_grid* g, long long* launches, double* total_ms, double* total_bytes) {
+    return guarded([&]() -> int {
+        if (!g || !launches || !total_ms || !total_bytes) return LP_ERR_ARG;
+        Grid::Prof& P = g->prof;
+        LP_HIP(hipStreamSynchronize(g->stream));
+        double ms = 0, by = 0;
+        for (size_t i = 0; i < P.used; ++i) {
+            float t = 0;
+            LP_HIP(hipEventElapsedTime(&t, P.ev[2 * i], P.ev[2 * i + 1]));
+            ms += t;
+            by += P.bytes[i];
+        }
+        *launches = (long long)P.used;
+        *total_ms = ms;
+        *total_bytes = by;
+        return P.dropped ? LP_NOT_CONVERGED : LP_OK;
+    });
+}
+
+int lp_tune(lp_grid* g, int param, int value) {
+    if (!g) return LP_ERR_ARG;
+    if (param == 0) {
+        if (value != 1 && value != 2) return LP_ERR_ARG;
+        const int old = g->tune_fine_ny;
+        g->tune_fine_ny = value;
+        return old;
+    }
+    if (param == 1) {
+        if (value < 1 || value > 64) return LP_ERR_ARG;
+        const int old = g->tune_fine_xchunk;
+        g->tune_fine_xchunk = value;
+        return old;
+    }
+    return LP_ERR_ARG;
 }
 
 int lp_time_apply(lp_grid* g, int level, int dtype_bytes, int nrhs, int reps, double* ms_per_apply) {

@@ -114,8 +114,8 @@ class GridSimp:
         out = []
         for k in range(n):
             s = stats[k]
-            out.append(dict(compliance=s.compliance, change=s.change, cg_iters=s.cg_iters, oc_evals=s.oc_evals,
-                            seconds=s.seconds))
+            out.append(dict(compliance=s.compliance, change=s.change, cg_iters=s.cg_iters,
+                            cg_iters_total=s.cg_iters_total, oc_evals=s.oc_evals, seconds=s.seconds))
         return out
 
     def _get(self, what):

@@ -252,20 +252,25 @@ def corner_mask(mask, a, b, ny, nz):
 
 def galerkin(fine: Level, masked: bool) -> Level:
     """Coarse cells (half, padded even) with K_c = sum_k P_k^T K_child P_k,
-    K_child masked by the fine mask when `masked` (mg.py:601-677)."""
+    K_child masked by the fine mask when `masked` (mg.py:601-677). Processed
+    by coarse x-slabs to bound memory."""
     nx, ny, nz = fine.dims
     half = (nx // 2, ny // 2, nz // 2)
     cdims = tuple(h + h % 2 if h > 1 else h for h in half)
     k = np.zeros((24, 24) + cdims)
-    kf = fine.cell_matrices(0, nx)
-    if masked:
-        dm = corner_mask(fine.mask, 0, nx, ny, nz)
-        kf = kf * dm[:, None] * dm[None, :]
     hx, hy, hz = half
-    for ch, (ox, oy, oz) in enumerate(CORNER_OFF):
-        sub = kf[:, :, ox:2 * hx:2, oy:2 * hy:2, oz:2 * hz:2]
-        t = np.einsum("ai,abxyz,bj->ijxyz", PK[ch], sub, PK[ch], optimize=True)
-        k[:, :, :hx, :hy, :hz] += t
+    per = 24 * 24 * 8 * ny * nz * 8
+    step = max(1, int(256e6 // max(per, 1)))
+    for c0 in range(0, hx, step):
+        c1 = min(c0 + step, hx)
+        kf = fine.cell_matrices(2 * c0, 2 * c1)
+        if masked:
+            dm = corner_mask(fine.mask, 2 * c0, 2 * c1, ny, nz)
+            kf = kf * dm[:, None] * dm[None, :]
+        for ch, (ox, oy, oz) in enumerate(CORNER_OFF):
+            sub = kf[:, :, ox::2, oy:2 * hy:2, oz:2 * hz:2]
+            t = np.einsum("ai,abxyz,bj->ijxyz", PK[ch], sub, PK[ch], optimize=True)
+            k[:, :, c0:c1, :hy, :hz] += t
     lv = Level("explicit", cdims, np.float64)
     lv.k = k
     return lv
