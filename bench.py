@@ -128,7 +128,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the multi-load solve shards by load case with no exchange during the "
-              "solve (see DESIGN.md); runs on more than one GPU are not part of this build.", file=sys.stderr)
+              "solve; runs on more than one GPU are not measured by this script.", file=sys.stderr)
         sys.exit(2)
     log = (lambda m: print(m, file=sys.stderr, flush=True)) if a.verbose else None
     nx, ny, nz = a.size or (WORKLOAD["nx"], WORKLOAD["ny"], WORKLOAD["nz"])
@@ -144,7 +144,8 @@ def main():
     if prof and prof["launches"]:
         gbs = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9
         roof = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
-                    traffic=None, kernel="k_apply_fine<float> (V-cycle fine-level operator + smoother epilogue)",
+                    traffic=None, kernel="k_apply_fine_mfma (V-cycle fine-level operator, FP16 MFMA contraction, "
+                                         "FP32 smoother epilogue)",
                     launches=prof["launches"], avg_us=round(prof["ms"] * 1e3 / prof["launches"], 2),
                     avg_bytes=round(prof["bytes"] / prof["launches"]))
     cpu = None if a.no_cpu else cpu_baseline(mixed["timed"], log)
@@ -160,7 +161,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp64 CG + fp32 V-cycle",
+        "dtype": "f64 CG / f32 V-cycle / f16 MFMA fine operator",
         "data": "synthetic (seeded load cases, uniform start density)",
         "config": {"workload": f"multiload_{nx}x{ny}x{nz}: 8 load cases (unit force, seeded direction, 8x8-node top "
                                "patch), bottom face clamped, vf 0.15, rmin 2.5, SIMP p=3 Emin 1e-9, top3d density "

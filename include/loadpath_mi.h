@@ -4,7 +4,8 @@
  * This is the drop-in boundary for loadpath's voxel SIMP hot path. Every
  * entry point replaces one reference interface, cited as file:line into the
  * loadpath repository. The Python host package (loadpath_amd) binds these
- * with ctypes; INTEGRATION.md shows the binding a loadpath maintainer adds.
+ * with ctypes; a loadpath-side binding is a thin ctypes wrapper over the same
+ * symbols.
  *
  * Conventions
  *  - Plain pointers and sizes only. Node-grid fields are component-major
@@ -65,6 +66,12 @@ typedef struct lp_grid lp_grid;
  * element: 0 = "im" (Wilson bubbles condensed), 1 = "plain" trilinear. */
 int lp_hex8_ke(double nu, int element, double* ke_out /* 576 */);
 
+/* C B at the stress sample points, E = 1, unit pitch (constrained.py:383-387):
+ * points 0 = the centre, fem.hex8_b_center (fem.py:307-308), 1 point;
+ * points 1 = the 8 corners, x outermost, fem.hex8_b with the condensed
+ * bubbles (fem.py:227-237). out (npts, 6, 24); divide by the pitch. */
+int lp_hex8_cb(double nu, int element, int points, double* out);
+
 /* ---- grid handle: mg.GridElasticity (mg.py:411-511) --------------------- */
 /* fixed_grid: (3, nx+1, ny+1, nz+1) uint8 (mg.py:447-451); ke: 576 doubles
  * already scaled by the pitch (mg.py:446). */
@@ -105,6 +112,25 @@ int lp_solve(lp_grid* g, int nrhs, const double* b, double* x, int warm, int dev
 /* GridElasticity.strain_energy (mg.py:975-988) / femops.element_energy
  * (femops.py:236-242): se (nrhs, nx, ny, nz) = u_c^T Ke u_c. */
 int lp_strain_energy(lp_grid* g, int nrhs, const double* u, double* se, int device_ptr);
+
+/* The bilinear form a_c^T Ke b_c per cell (the adjoint term of
+ * constrained.py:659, lambda_e^T Ke u_e): out (nrhs, nx, ny, nz). */
+int lp_element_product(lp_grid* g, int nrhs, const double* a, const double* b, double* out, int device_ptr);
+
+/* p-mean von Mises stress aggregate, constrained.py:607-664 (the
+ * non-averaged branch), one load case per rhs on the fine grid:
+ *   sigma_k = cb_k u_c (cb from lp_hex8_cb / pitch, npts 1 or 8),
+ *   vm = sqrt(max(sigma^T V sigma, 1e-30)), s = x_c^q vm / lim (0 where
+ *   counted == 0), pm = sc (sum (s / sc)^p / nc)^(1/p), sc = max(s_max, 1e-30).
+ * u (nrhs, 3, nodes), x (nx, ny, nz) the material share, counted (nx, ny, nz)
+ * uint8 or NULL (every cell), nc the number of counted samples. Outputs pm
+ * and s_max (host, nrhs); optional rhs (nrhs, 3, nodes) = d(kappa pm)/du,
+ * zero on fixed dofs (the adjoint right-hand side), and gx (nrhs, nx, ny,
+ * nz) = the explicit d(kappa pm)/dx at fixed u. The full gradient is
+ * gx - dE/dx * lp_element_product(lambda, u) with K lambda = rhs. */
+int lp_stress_pnorm(lp_grid* g, int nrhs, const double* u, const double* x, const unsigned char* counted,
+                    const double* cb, int npts, double p, double q, double lim, double kappa, double nc,
+                    double* pm, double* smax, double* rhs, double* gx, int device_ptr);
 
 /* Level data for parity checks: what = 0 mask bits (uint8, per node),
  * 1 dinv (double, 3 x nodes), 2 cell dof indices (int64, (ncells, 24), the

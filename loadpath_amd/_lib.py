@@ -74,6 +74,9 @@ _SIGS = {
     "lp_precondition": (_I, [_P, _P, _P, _I, _I]),
     "lp_solve": (_I, [_P, _I, _P, _P, _I, _I, _P, _P]),
     "lp_strain_energy": (_I, [_P, _I, _P, _P, _I]),
+    "lp_element_product": (_I, [_P, _I, _P, _P, _P, _I]),
+    "lp_stress_pnorm": (_I, [_P, _I, _P, _P, _P, _P, _I, _D, _D, _D, _D, _D, _P, _P, _P, _P, _I]),
+    "lp_hex8_cb": (_I, [_D, _I, _I, _P]),
     "lp_level_data": (_I, [_P, _I, _I, _P]),
     "lp_prolong": (_I, [_P, _P, _I, _I, _I, _I]),
     "lp_restrict": (_I, [_P, _P, _I, _I, _I, _I]),

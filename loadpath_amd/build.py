@@ -23,7 +23,8 @@ ARCH = "gfx950"
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
-    # packed FP32 (v_pk_*_f32) off: see DESIGN.md (it has given wrong results)
+    # packed FP32 (v_pk_*_f32) off: some of its forms have given wrong results
+    # on this hardware with no error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unused-command-line-argument",
 ]

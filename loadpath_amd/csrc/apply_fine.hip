@@ -39,7 +39,9 @@ AGeom make_apply_geom(const Grid& Gd, int level) {
     return G;
 }
 
-int apply_blocks(const Grid& Gd, int level) {
+int apply_blocks(const Grid& Gd, int level, int eb) {
+    if (galerkin1_path(Gd, level, eb)) return (int)((Gd.lv[level].g.nn + kBlock - 1) / kBlock);
+    if (Gd.lv[level].kind == LK_FINE && eb == 4 && Gd.opt.precision == LP_PREC_MIXED) return mfma_blocks(Gd, level);
     const AGeom G = make_apply_geom(Gd, level);
     return Gd.lv[level].kind == LK_FINE ? 8 * G.per : G.nblk;
 }
@@ -82,7 +84,12 @@ int launch_apply(const Grid& G, int level, int eb, const void* u, size_t vstride
             else
                 launch_apply_fine(G, level, eb, u, vstride, nrhs, E, s);
             break;
-        case LK_WEIGHTS: launch_apply_weights(G, level, eb, u, vstride, nrhs, E, s); break;
+        case LK_WEIGHTS:
+            if (galerkin1_path(G, level, eb))
+                launch_apply_galerkin1(G, level, u, vstride, nrhs, E, s);
+            else
+                launch_apply_weights(G, level, eb, u, vstride, nrhs, E, s);
+            break;
         default: launch_apply_explicit(G, level, eb, u, vstride, nrhs, E, s); break;
     }
     if (rec && 2 * (P.used + 1) <= P.ev.size()) {
@@ -90,7 +97,7 @@ int launch_apply(const Grid& G, int level, int eb, const void* u, size_t vstride
         P.bytes.push_back(apply_bytes(G, level, eb, nrhs, E.kind));
         P.used++;
     }
-    return apply_blocks(G, level);
+    return apply_blocks(G, level, eb);
 }
 
 }  // namespace lp

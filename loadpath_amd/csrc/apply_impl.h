@@ -40,6 +40,7 @@ struct AGeom {
     size_t nn, ncell;
     int xchunk, nyzb, nxc, nblk, per;
     int ny_per;             // nodes per thread along y (fine kernel)
+    int ntz;                // node tiles along z (fine MFMA kernel)
 };
 
 // Ke as its KE_NV orbit values (see ke_pattern.h): kernel arguments, so
@@ -81,66 +82,115 @@ __device__ __forceinline__ void block_partial(double part, EpiArgs& E, int rhs, 
     }
 }
 
-// node epilogue: yv = finished operator value at node n (3 components)
-template <typename T>
-__device__ __forceinline__ void epilogue(const EpiArgs& E, int m, size_t o, size_t nn, size_t n, const T (&yv)[3],
-                                         const T (&ur)[3], double& part) {
+// node epilogue, in two phases so a kernel can issue the loads early:
+// epi_load fetches the vectors the epilogue kind reads at node n (rhs offset
+// o), epi_finish combines them with yv = the finished operator value
+template <typename T> struct EpiIn {
+    T v[4][3];
+};
+
+// K >= 0 fixes the epilogue kind at compile time (straight-line code, the
+// prefetched values stay in registers); K < 0 reads E.kind at run time
+template <typename T, int K = -1>
+__device__ __forceinline__ void epi_load(const EpiArgs& E, size_t o, size_t nn, size_t n, EpiIn<T>& in) {
+    const int kind = K >= 0 ? K : E.kind;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const size_t i = o + c * nn;
+        switch (kind) {
+            case EPI_CHEB:
+                in.v[0][c] = static_cast<const T*>(E.d)[i];
+                in.v[1][c] = static_cast<const T*>(E.x)[i];
+                in.v[2][c] = static_cast<const T*>(E.r)[i];
+                in.v[3][c] = static_cast<const T*>(E.dinv)[c * nn + n];
+                break;
+            case EPI_RESID_SCALED:
+            case EPI_JACOBI:
+                in.v[0][c] = static_cast<const T*>(E.b)[i];
+                in.v[3][c] = static_cast<const T*>(E.dinv)[c * nn + n];
+                break;
+            case EPI_RESID_MASK:
+            case EPI_RESID:
+                in.v[0][c] = static_cast<const T*>(E.b)[i];
+                break;
+            case EPI_POWER:
+                in.v[3][c] = static_cast<const T*>(E.dinv)[c * nn + n];
+                break;
+            default: break;
+        }
+    }
+}
+
+// AR: the type of the smoother/residual arithmetic (double for the FP32
+// reference split, as the reference computes its float32 smoother with
+// float64 coefficients; float inside the FP16-MFMA operator)
+template <typename T, int K = -1, typename AR = double>
+__device__ __forceinline__ void epi_finish(const EpiArgs& E, int m, size_t o, size_t nn, const T (&yv)[3],
+                                           const T (&ur)[3], const EpiIn<T>& in, double& part) {
+    const int kind = K >= 0 ? K : E.kind;
+    const AR c1 = (AR)E.c1, c2 = (AR)E.c2;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const size_t i = o + c * nn;
         const T y_ = yv[c];
-        switch (E.kind) {
+        switch (kind) {
             case EPI_STORE: static_cast<T*>(E.out)[i] = y_; break;
             case EPI_STORE_DOT:
                 static_cast<T*>(E.out)[i] = y_;
                 part += (double)ur[c] * (double)y_;
                 break;
             case EPI_CHEB: {
-                const T di = static_cast<const T*>(E.d)[i];
-                T* X_ = static_cast<T*>(E.x);
-                T* R_ = static_cast<T*>(E.r);
-                X_[i] = X_[i] + di;
-                const T ri = R_[i] - y_;
-                R_[i] = ri;
-                const T dv = static_cast<const T*>(E.dinv)[c * nn + n];
-                static_cast<T*>(E.d2)[i] = (T)(E.c1 * (double)di + E.c2 * (double)dv * (double)ri);
+                const T di = in.v[0][c];
+                static_cast<T*>(E.x)[i] = in.v[1][c] + di;
+                const T ri = in.v[2][c] - y_;
+                static_cast<T*>(E.r)[i] = ri;
+                static_cast<T*>(E.d2)[i] = (T)(c1 * (AR)di + c2 * (AR)in.v[3][c] * (AR)ri);
                 break;
             }
             case EPI_RESID_SCALED: {
-                const T ri = static_cast<const T*>(E.b)[i] - y_;
+                const T ri = in.v[0][c] - y_;
                 static_cast<T*>(E.r)[i] = ri;
-                const T dv = static_cast<const T*>(E.dinv)[c * nn + n];
-                static_cast<T*>(E.d2)[i] = (T)(E.c1 * (double)dv * (double)ri);
+                static_cast<T*>(E.d2)[i] = (T)(c1 * (AR)in.v[3][c] * (AR)ri);
                 break;
             }
             case EPI_RESID_MASK: {
                 const bool mc = (m >> c) & 1;
-                static_cast<T*>(E.out)[i] = mc ? (T)(static_cast<const T*>(E.b)[i] - y_) : T(0);
+                static_cast<T*>(E.out)[i] = mc ? (T)(in.v[0][c] - y_) : T(0);
                 break;
             }
-            case EPI_JACOBI: {
-                const T dv = static_cast<const T*>(E.dinv)[c * nn + n];
-                const T bi = static_cast<const T*>(E.b)[i];
-                static_cast<T*>(E.out)[i] = (T)((double)ur[c] + E.c1 * (double)dv * (double)(bi - y_));
+            case EPI_JACOBI:
+                static_cast<T*>(E.out)[i] =
+                    (T)((AR)ur[c] + c1 * (AR)in.v[3][c] * (AR)(in.v[0][c] - y_));
                 break;
-            }
             case EPI_POWER: {
                 const bool mc = (m >> c) & 1;
-                const T dv = static_cast<const T*>(E.dinv)[c * nn + n];
-                const T av = mc ? (T)(dv * y_) : T(0);
+                const T av = mc ? (T)(in.v[3][c] * y_) : T(0);
                 static_cast<T*>(E.out)[i] = av;
                 part += (double)ur[c] * (double)av;
                 break;
             }
             case EPI_RESID: {
-                const T ri = static_cast<const T*>(E.b)[i] - y_;
+                const T ri = in.v[0][c] - y_;
                 static_cast<T*>(E.out)[i] = ri;
                 part += (double)ri * (double)ri;
+                break;
+            }
+            case EPI_MASKED: {
+                const bool mc = (m >> c) & 1;
+                static_cast<T*>(E.out)[i] = mc ? y_ : T(0);
                 break;
             }
             default: break;
         }
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void epilogue(const EpiArgs& E, int m, size_t o, size_t nn, size_t n, const T (&yv)[3],
+                                         const T (&ur)[3], double& part) {
+    EpiIn<T> in;
+    epi_load<T>(E, o, nn, n, in);
+    epi_finish<T>(E, m, o, nn, yv, ur, in, part);
 }
 
 template <typename T>

@@ -60,28 +60,36 @@ void add_atcb(double w, const double* a, int p, const double c[6][6], const doub
 
 }  // namespace
 
-void hex8_ke_host(double nu, int element, double* ke) {
+// trilinear (6 x 24) and bubble (6 x 9) strain operators of the unit cube at
+// natural point nat (fem._b_matrix / _b_bubble, fem.py:127-189)
+static void b_at(const double nat[3], double* b, double* ba) {
+    double grad[8][3];
+    for (int k = 0; k < 8; ++k) {
+        const double s[3] = {(k & 1) ? 1.0 : -1.0, (k & 2) ? 1.0 : -1.0, (k & 4) ? 1.0 : -1.0};
+        const double f0 = 1.0 + nat[0] * s[0], f1 = 1.0 + nat[1] * s[1], f2 = 1.0 + nat[2] * s[2];
+        grad[k][0] = 0.25 * s[0] * f1 * f2;
+        grad[k][1] = 0.25 * s[1] * f0 * f2;
+        grad[k][2] = 0.25 * s[2] * f0 * f1;
+    }
+    double gb[3][3] = {{-4.0 * nat[0], 0, 0}, {0, -4.0 * nat[1], 0}, {0, 0, -4.0 * nat[2]}};
+    strain_rows(grad, 8, b);
+    strain_rows(gb, 3, ba);
+}
+
+// Ke (24 x 24) and, for the "im" element, the bubble recovery r (9 x 24):
+// bubble amplitudes = r u (fem._hex8_parts, fem.py:191-217)
+static void hex8_parts_host(double nu, int element, double* ke, double* r_out) {
     double c[6][6];
     isotropic(nu, c);
     const double g = 1.0 / std::sqrt(3.0);
-    static double kuu[24 * 24], kua[24 * 9], kaa[9 * 9];
+    double kuu[24 * 24], kua[24 * 9], kaa[9 * 9];
     std::memset(kuu, 0, sizeof(kuu));
     std::memset(kua, 0, sizeof(kua));
     std::memset(kaa, 0, sizeof(kaa));
     for (int gp = 0; gp < 8; ++gp) {
         const double nat[3] = {(gp & 4) ? g : -g, (gp & 2) ? g : -g, (gp & 1) ? g : -g};
-        double grad[8][3];
-        for (int k = 0; k < 8; ++k) {
-            const double s[3] = {(k & 1) ? 1.0 : -1.0, (k & 2) ? 1.0 : -1.0, (k & 4) ? 1.0 : -1.0};
-            const double f0 = 1.0 + nat[0] * s[0], f1 = 1.0 + nat[1] * s[1], f2 = 1.0 + nat[2] * s[2];
-            grad[k][0] = 0.25 * s[0] * f1 * f2;
-            grad[k][1] = 0.25 * s[1] * f0 * f2;
-            grad[k][2] = 0.25 * s[2] * f0 * f1;
-        }
-        double gb[3][3] = {{-4.0 * nat[0], 0, 0}, {0, -4.0 * nat[1], 0}, {0, 0, -4.0 * nat[2]}};
         double b[6 * 24], ba[6 * 9];
-        strain_rows(grad, 8, b);
-        strain_rows(gb, 3, ba);
+        b_at(nat, b, ba);
         add_atcb(0.125, b, 24, c, b, 24, kuu);
         add_atcb(0.125, b, 24, c, ba, 9, kua);
         add_atcb(0.125, ba, 9, c, ba, 9, kaa);
@@ -117,9 +125,51 @@ void hex8_ke_host(double nu, int element, double* ke) {
                 for (int a = 0; a < 9; ++a) s += kua[i * 9 + a] * (-m[a][9 + j]);
                 k[i * 24 + j] += s;
             }
+        if (r_out)
+            for (int a = 0; a < 9; ++a)
+                for (int j = 0; j < 24; ++j) r_out[a * 24 + j] = -m[a][9 + j];
+    } else if (r_out) {
+        std::memset(r_out, 0, sizeof(double) * 9 * 24);
     }
     for (int i = 0; i < 24; ++i)
         for (int j = 0; j < 24; ++j) ke[i * 24 + j] = 0.5 * (k[i * 24 + j] + k[j * 24 + i]);
+}
+
+void hex8_ke_host(double nu, int element, double* ke) { hex8_parts_host(nu, element, ke, nullptr); }
+
+// C B at the stress sample points (constrained.py:383-387): the centre with
+// the trilinear B alone (fem.hex8_b_center, fem.py:307-308), or the 8 corners
+// (x outermost, then y, then z) with the bubbles' strain added (fem.hex8_b,
+// fem.py:227-237). out (npts, 6, 24) at E = 1 and unit pitch.
+void hex8_cb_host(double nu, int element, int points, double* out) {
+    double c[6][6];
+    isotropic(nu, c);
+    double ke[24 * 24], r[9 * 24];
+    hex8_parts_host(nu, element, ke, r);
+    const int npts = points == 0 ? 1 : 8;
+    for (int k = 0; k < npts; ++k) {
+        double nat[3] = {0.0, 0.0, 0.0};
+        if (points != 0) {
+            nat[0] = (k & 4) ? 1.0 : -1.0;
+            nat[1] = (k & 2) ? 1.0 : -1.0;
+            nat[2] = (k & 1) ? 1.0 : -1.0;
+        }
+        double b[6 * 24], ba[6 * 9];
+        b_at(nat, b, ba);
+        if (points != 0 && element == 0)
+            for (int i = 0; i < 6; ++i)
+                for (int j = 0; j < 24; ++j) {
+                    double s = 0;
+                    for (int a = 0; a < 9; ++a) s += ba[i * 9 + a] * r[a * 24 + j];
+                    b[i * 24 + j] += s;
+                }
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 24; ++j) {
+                double s = 0;
+                for (int a = 0; a < 6; ++a) s += c[i][a] * b[a * 24 + j];
+                out[(k * 6 + i) * 24 + j] = s;
+            }
+    }
 }
 
 void child_prolongations_host(double* pk) {
@@ -138,6 +188,15 @@ void child_prolongations_host(double* pk) {
 }
 
 }  // namespace lp
+
+extern "C" int lp_hex8_cb(double nu, int element, int points, double* out) {
+    if (!out || (element != 0 && element != 1) || (points != 0 && points != 1) || !(nu > -1.0 && nu < 0.5)) {
+        lp::set_error("lp_hex8_cb: bad argument");
+        return LP_ERR_ARG;
+    }
+    lp::hex8_cb_host(nu, element, points, out);
+    return LP_OK;
+}
 
 extern "C" int lp_hex8_ke(double nu, int element, double* ke_out) {
     if (!ke_out || (element != 0 && element != 1) || !(nu > -1.0 && nu < 0.5)) {

@@ -10,7 +10,7 @@ that symmetry satisfies
 
 so its 576 entries fall into a few orbits: within an orbit every entry is
 +-1 times one value, and an orbit that maps an entry onto itself with sign
--1 is identically zero. Both elements of this build (plain trilinear and
+-1 is identically zero. Both elements of this library (plain trilinear and
 "im" with condensed bubbles) have the symmetry for an isotropic material;
 the host checks an actual Ke against the pattern before the fine kernel
 uses it (an anisotropic Ke falls back to reading all 576 values).

@@ -476,6 +476,10 @@ int build_hierarchy(Grid& G) {
     long long n_active = (long long)read_counter(counter, s);
     L0.n_active = n_active;
     alloc_vectors(G, L0);
+    if (G.opt.precision == LP_PREC_MIXED && !G.gal_a.p) {
+        G.gal_a.alloc(3 * L0.g.nn * (size_t)G.max_rhs * 4);
+        G.gal_b.alloc(3 * L0.g.nn * (size_t)G.max_rhs * 4);
+    }
     int nl = 1;
     const bool need64 = G.opt.precision == LP_PREC_FP64;
 

@@ -27,6 +27,22 @@ void set_error(const std::string& msg);
 struct HipFailure {};
 struct ArgFailure {};
 
+// C-ABI entry guard: HIP failures, argument failures and other exceptions
+// become status codes (the message is already in the error slot for the
+// first two)
+template <typename F> int api_call(F&& f) {
+    try {
+        return f();
+    } catch (const HipFailure&) {
+        return -1;
+    } catch (const ArgFailure&) {
+        return -2;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return -3;
+    }
+}
+
 // Owning device allocation.
 struct DBuf {
     void* p = nullptr;
@@ -108,6 +124,8 @@ struct Grid {
     // CG vectors (FP64), max_rhs each
     DBuf x, r, p, ap, z, b;
     DBuf rv, zv;                // V-cycle precision copies of r and z at level 0
+    DBuf vscale;                // per-rhs V-cycle input scale (mixed path)
+    DBuf gal_a, gal_b;          // fine-grid FP32 scratch of the Galerkin level-1 apply (mixed path)
     // reductions
     DBuf partials;              // doubles
     size_t partials_cap = 0;
@@ -146,6 +164,7 @@ struct CgScal {
 
 // ---- element.cpp -------------------------------------------------------
 void hex8_ke_host(double nu, int element, double* ke);
+void hex8_cb_host(double nu, int element, int points, double* out /* npts*6*24 */);
 void child_prolongations_host(double* pk /* 8*576 */);
 
 // ---- apply.hip ---------------------------------------------------------
@@ -159,6 +178,7 @@ enum EpiKind {
     EPI_JACOBI = 5,       // out = u + w dinv (b - y)
     EPI_POWER = 6,        // out = dinv * y * m; partial += u . out
     EPI_RESID = 7,        // out = b - y; partial += out . out (double)
+    EPI_MASKED = 8,       // out = m ? y : 0 (the masked operator m K m u, no identity rows)
 };
 
 struct EpiArgs {
@@ -180,7 +200,9 @@ struct EpiArgs {
 // Returns the number of blocks per rhs (for partial sums).
 int launch_apply(const Grid& G, int level, int elem_bytes, const void* u, size_t vstride, int nrhs,
                  const EpiArgs& epi, hipStream_t s);
-int apply_blocks(const Grid& G, int level);
+// blocks (= reduction partials per rhs) of one operator launch at element size eb
+int apply_blocks(const Grid& G, int level, int eb);
+int mfma_blocks(const Grid& G, int level);
 void launch_apply_fine(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs,
                        const EpiArgs& E, hipStream_t s);
 void launch_apply_weights(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs,
@@ -202,6 +224,12 @@ void launch_dot(const void* a, const void* b, int elem_bytes, size_t n, size_t s
 void launch_cast(const void* src, int src_bytes, void* dst, int dst_bytes, size_t n, size_t stride,
                  int nrhs, hipStream_t s);
 void launch_fill(void* dst, int elem_bytes, size_t n, double v, hipStream_t s);
+// scale[rhs] = max |v_rhs| (1 if zero)
+void launch_absmax(const double* v, size_t n, size_t stride, int nrhs, double* partials, double* scale,
+                   hipStream_t s);
+// dst = src / scale (invert 0) or src * scale (invert 1), per rhs
+void launch_cast_scaled(const void* src, int sb, void* dst, int db, size_t n, size_t stride, int nrhs,
+                        const double* scale, int invert, hipStream_t s);
 void launch_scale_by_dev(void* v, int elem_bytes, size_t n, size_t stride, int nrhs,
                          const double* num, size_t num_stride, int mode, hipStream_t s);
 void launch_cheb_start(const void* b, const void* dinv, double s_, void* x, void* r, void* d,
@@ -225,6 +253,14 @@ void launch_prolong_add(const void* xc, const uint8_t* cmask, const Geom& coarse
                         void* xf, const Geom& fine, int elem_bytes, int nrhs, size_t cstride,
                         size_t fstride, hipStream_t s);
 void launch_prolong_plain(const double* c, double* f, int cx, int cy, int cz, hipStream_t s);
+// xf = prolong(xc * cm) on the fine grid (overwrites xf)
+void launch_prolong_set(const void* xc, const uint8_t* cmask, const Geom& coarse, void* xf, const Geom& fine, int eb,
+                        int nrhs, size_t cstride, size_t fstride, hipStream_t s);
+// mixed path, first coarse level applied as P^T (m K0 m) P through the fine
+// MFMA operator: prolong, masked fine apply, restrict + finish + epilogue
+void launch_apply_galerkin1(const Grid& G, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,
+                            hipStream_t s);
+bool galerkin1_path(const Grid& G, int level, int eb);
 void launch_restrict_plain(const double* f, double* c, int cx, int cy, int cz, hipStream_t s);
 
 // ---- hierarchy.hip -------------------------------------------------------

@@ -147,7 +147,7 @@ static double estimate_lmax_level(Grid& G, int i, const double* start_host) {
     av.alloc(n * eb);
     const void* dinv = eb == 8 ? L.dinv64.p : L.dinv32.p;
     const int nbd = dot_blocks(n);
-    const int nba = apply_blocks(G, i);
+    const int nba = apply_blocks(G, i, eb);
     ensure_partials(G, std::max(nbd, nba));
     double* S = sc.as<double>();
     for (int it = 0; it < its; ++it) {
@@ -273,6 +273,17 @@ static void precondition(Grid& G, const double* r, double* z, int nrhs) {
     const int eb = G.vbytes();
     Level& L = G.lv[0];
     const size_t n = 3 * L.g.nn;
+    if (G.opt.precision == LP_PREC_MIXED) {
+        // the V-cycle is linear: run it on r / max|r| so the FP16 operands of
+        // the fine-level contraction stay in range, and scale z back
+        G.vscale.alloc(G.max_rhs * sizeof(double));
+        ensure_partials(G, (size_t)nrhs * dot_blocks(n));
+        launch_absmax(r, n, n, nrhs, G.partials.as<double>(), G.vscale.as<double>(), G.stream);
+        launch_cast_scaled(r, 8, L.vb.p, eb, n, n, nrhs, G.vscale.as<double>(), 0, G.stream);
+        vcycle(G, 0, nrhs);
+        launch_cast_scaled(L.vx.p, eb, z, 8, n, n, nrhs, G.vscale.as<double>(), 1, G.stream);
+        return;
+    }
     launch_cast(r, 8, L.vb.p, eb, n, n, nrhs, G.stream);
     vcycle(G, 0, nrhs);
     launch_cast(L.vx.p, eb, z, 8, n, n, nrhs, G.stream);
@@ -290,7 +301,7 @@ void pcg(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
     hipStream_t s = G.stream;
     const size_t n = 3 * G.g0.nn;
     const int nbd = dot_blocks(n);
-    const int nba = apply_blocks(G, 0);
+    const int nba = apply_blocks(G, 0, 8);
     ensure_partials(G, (size_t)nrhs * std::max(nbd, nba));
     double* P = G.partials.as<double>();
     CgScal* S = G.scal.as<CgScal>();

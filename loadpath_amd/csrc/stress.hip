@@ -1,0 +1,317 @@
+// p-mean von Mises stress aggregate over the fine grid's cells and the parts
+// of its gradient that need no solve: the explicit derivative in the cell
+// densities and the adjoint right-hand side dg/du; plus the element bilinear
+// form a_c^T Ke b_c the adjoint sensitivity uses.
+//
+// Reference: constrained.py:607-664 (the non-averaged stress branch):
+//   sigma_k = C B_k u_c / pitch at the sample points (centre or 8 corners),
+//   vm = sqrt(max(sigma^T V sigma, 1e-30)), s = x^q vm / lim,
+//   pm = sc (sum (s / sc)^P / nc)^(1/P), sc = max(s_max, 1e-30), g = kappa pm,
+//   dg/ds = kappa (s / pm)^(P-1) / nc,
+//   dg/dx (explicit) = sum_k dg/ds q x^(q-1) vm / lim,
+//   dg/du = sum_k dg/ds x^q / (vm lim) (V sigma)^T C B_k, zero on fixed dofs;
+//   and the adjoint term lambda_c^T Ke u_c (constrained.py:659).
+//
+// Cells run one per thread in FP64 (the aggregate is evaluated once per
+// design iteration; its cost is a few field passes). The adjoint right-hand
+// side is assembled node-centrically from a per-cell buffer: each node sums
+// its <= 8 cells in a fixed order (no atomics, deterministic).
+#include "lp_internal.h"
+
+namespace lp {
+
+namespace {
+
+struct PnArgs {
+    const double* u;         // (nrhs, 3, Nx, Ny, Nz)
+    size_t ustride;
+    const double* x;         // (nx, ny, nz)
+    const uint8_t* counted;  // (nx, ny, nz) or null = every cell
+    const double* cb;        // (npts, 6, 24)
+    int npts;
+    double p, q, lim, kappa, nc;
+};
+
+__device__ __forceinline__ void gather_cell(const double* __restrict__ u, const Geom& g, size_t cell, double ue[24]) {
+    const int Z = (int)(cell % g.nz), Y = (int)((cell / g.nz) % g.ny), X = (int)(cell / ((size_t)g.nz * g.ny));
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const size_t node = ((size_t)(X + (b & 1)) * g.Ny + (Y + ((b >> 1) & 1))) * g.Nz + (Z + ((b >> 2) & 1));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ue[3 * b + c] = u[c * g.nn + node];
+    }
+}
+
+// sigma = CB_k ue, sv = V sigma; returns vm
+__device__ __forceinline__ double point_stress(const double* __restrict__ cbk, const double ue[24], double sv[6]) {
+    double s[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double a = 0.0;
+#pragma unroll
+        for (int j = 0; j < 24; ++j) a += cbk[i * 24 + j] * ue[j];
+        s[i] = a;
+    }
+    sv[0] = s[0] - 0.5 * s[1] - 0.5 * s[2];
+    sv[1] = -0.5 * s[0] + s[1] - 0.5 * s[2];
+    sv[2] = -0.5 * s[0] - 0.5 * s[1] + s[2];
+    sv[3] = 3.0 * s[3];
+    sv[4] = 3.0 * s[4];
+    sv[5] = 3.0 * s[5];
+    double v2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) v2 += sv[i] * s[i];
+    return sqrt(fmax(v2, 1e-30));
+}
+
+__device__ __forceinline__ double srel_of(const PnArgs& A, double xc, double vm, bool cnt) {
+    return cnt ? pow(xc, A.q) * vm / A.lim : 0.0;
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double sred[kBlock / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sred[i];
+    return s;
+}
+
+// MODE 0: s_max per rhs (atomic max on the bits of non-negative doubles,
+// order independent); MODE 1: block partial sums of (s / sc)^P
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_pnorm_scan(PnArgs A, Geom g, const double* __restrict__ sc,
+                                                       double* __restrict__ out) {
+    const size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int rhs = blockIdx.y;
+    double acc = 0.0;
+    if (cell < g.ncell) {
+        double ue[24];
+        gather_cell(A.u + rhs * A.ustride, g, cell, ue);
+        const double xc = A.x[cell];
+        const bool cnt = A.counted == nullptr || A.counted[cell] != 0;
+        for (int k = 0; k < A.npts; ++k) {
+            double sv[6];
+            const double vm = point_stress(A.cb + k * 144, ue, sv);
+            const double sr = srel_of(A, xc, vm, cnt);
+            if (MODE == 0)
+                acc = fmax(acc, sr);
+            else
+                acc += pow(sr / sc[rhs], A.p);
+        }
+    }
+    if (MODE == 0) {
+        for (int o = 32; o > 0; o >>= 1) acc = fmax(acc, __shfl_xor(acc, o));
+        if ((threadIdx.x & 63) == 0)
+            atomicMax(reinterpret_cast<unsigned long long*>(out + rhs),
+                      (unsigned long long)__double_as_longlong(acc));
+    } else {
+        const double s = block_sum(acc);
+        if (threadIdx.x == 0) out[(size_t)rhs * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// one rhs: explicit gradient gx (per cell) and the element adjoint
+// right-hand sides re (ncell, 24)
+__global__ void __launch_bounds__(kBlock) k_pnorm_cell(PnArgs A, Geom g, int rhs, double pm, double* __restrict__ gx,
+                                                       double* __restrict__ re) {
+    const size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= g.ncell) return;
+    double ue[24];
+    gather_cell(A.u + rhs * A.ustride, g, cell, ue);
+    const double xc = A.x[cell];
+    const bool cnt = A.counted == nullptr || A.counted[cell] != 0;
+    double r[24];
+#pragma unroll
+    for (int j = 0; j < 24; ++j) r[j] = 0.0;
+    double ex = 0.0;
+    for (int k = 0; k < A.npts; ++k) {
+        const double* cbk = A.cb + k * 144;
+        double sv[6];
+        const double vm = point_stress(cbk, ue, sv);
+        const double sr = srel_of(A, xc, vm, cnt);
+        const double dg = A.kappa * pow(sr / fmax(pm, 1e-30), A.p - 1.0) / A.nc;
+        ex += dg * (A.q * pow(fmax(xc, 1e-12), A.q - 1.0) * vm / A.lim);
+        const double coef = cnt ? dg * (pow(xc, A.q) / (vm * A.lim)) : 0.0;
+#pragma unroll
+        for (int j = 0; j < 24; ++j) {
+            double a = 0.0;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) a += sv[i] * cbk[i * 24 + j];
+            r[j] += coef * a;
+        }
+    }
+    if (gx) gx[(size_t)rhs * g.ncell + cell] = ex;
+#pragma unroll
+    for (int j = 0; j < 24; ++j) re[cell * 24 + j] = r[j];
+}
+
+// node-centric assembly of the element right-hand sides (bincount over
+// edof, constrained.py:651-653), zero on fixed dofs
+__global__ void __launch_bounds__(kBlock) k_pnorm_node(const double* __restrict__ re, Geom g,
+                                                       const uint8_t* __restrict__ fixed, double* __restrict__ out) {
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nn) return;
+    const int z = (int)(n % g.Nz), y = (int)((n / g.Nz) % g.Ny), x = (int)(n / ((size_t)g.Nz * g.Ny));
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int b = 0; b < 8; ++b) {
+        const int cx = x - (b & 1), cy = y - ((b >> 1) & 1), cz = z - ((b >> 2) & 1);
+        if (cx < 0 || cy < 0 || cz < 0 || cx >= g.nx || cy >= g.ny || cz >= g.nz) continue;
+        const double* e = re + (((size_t)cx * g.ny + cy) * g.nz + cz) * 24 + 3 * b;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[c] += e[c];
+    }
+    const int f = fixed[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c * g.nn + n] = ((f >> c) & 1) ? 0.0 : s[c];
+}
+
+// a_c^T Ke b_c per cell (Ke includes the pitch)
+__global__ void __launch_bounds__(kBlock) k_element_product(const double* __restrict__ a, const double* __restrict__ b,
+                                                            const double* __restrict__ ke, Geom g, size_t vstride,
+                                                            double* __restrict__ out) {
+    const size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int rhs = blockIdx.y;
+    if (cell >= g.ncell) return;
+    double ae[24], be[24];
+    gather_cell(a + rhs * vstride, g, cell, ae);
+    gather_cell(b + rhs * vstride, g, cell, be);
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 24; ++i) {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < 24; ++j) t += ke[i * 24 + j] * be[j];
+        acc += ae[i] * t;
+    }
+    out[(size_t)rhs * g.ncell + cell] = acc;
+}
+
+inline unsigned nblk(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+const void* upload(Grid& G, DBuf& stage, const void* p, size_t bytes, bool device_ptr) {
+    if (device_ptr || p == nullptr) return p;
+    stage.alloc(bytes);
+    LP_HIP(hipMemcpyAsync(stage.p, p, bytes, hipMemcpyHostToDevice, G.stream));
+    return stage.p;
+}
+
+}  // namespace
+
+}  // namespace lp
+
+using namespace lp;
+
+struct lp_grid : public Grid {};
+
+extern "C" {
+
+int lp_stress_pnorm(lp_grid* g, int nrhs, const double* u, const double* x, const uint8_t* counted,
+                    const double* cb, int npts, double p, double q, double lim, double kappa, double nc, double* pm,
+                    double* smax, double* rhs, double* gx, int device_ptr) {
+    return api_call([&]() -> int {
+        if (!g || nrhs < 1 || !u || !x || !cb || !pm || !smax || (npts != 1 && npts != 8) || !(p >= 1.0) ||
+            !(q > 0.0) || !(lim > 0.0) || !(nc > 0.0)) {
+            set_error("lp_stress_pnorm: bad argument");
+            return LP_ERR_ARG;
+        }
+        Grid& G = *g;
+        const Geom& g0 = G.g0;
+        hipStream_t s = G.stream;
+        const size_t ustride = 3 * g0.nn;
+        DBuf su, sx, sc_, scb, sm, part, re, out_r, out_g;
+        PnArgs A;
+        A.u = static_cast<const double*>(upload(G, su, u, ustride * nrhs * 8, device_ptr));
+        A.ustride = ustride;
+        A.x = static_cast<const double*>(upload(G, sx, x, g0.ncell * 8, device_ptr));
+        A.counted = static_cast<const uint8_t*>(upload(G, sc_, counted, g0.ncell, device_ptr));
+        A.cb = static_cast<const double*>(upload(G, scb, cb, (size_t)npts * 144 * 8, false));
+        A.npts = npts;
+        A.p = p;
+        A.q = q;
+        A.lim = lim;
+        A.kappa = kappa;
+        A.nc = nc;
+        // s_max, then sc = max(s_max, 1e-30) and the sums of (s / sc)^P
+        sm.alloc(2 * nrhs * sizeof(double));
+        LP_HIP(hipMemsetAsync(sm.p, 0, nrhs * sizeof(double), s));
+        const dim3 grid(nblk(g0.ncell), nrhs);
+        hipLaunchKernelGGL(k_pnorm_scan<0>, grid, dim3(kBlock), 0, s, A, g0, (const double*)nullptr,
+                           sm.as<double>());
+        LP_HIP(hipGetLastError());
+        std::vector<double> h(2 * nrhs);
+        LP_HIP(hipMemcpyAsync(h.data(), sm.p, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
+        LP_HIP(hipStreamSynchronize(s));
+        for (int r = 0; r < nrhs; ++r) {
+            smax[r] = h[r];
+            h[nrhs + r] = std::max(h[r], 1e-30);
+        }
+        LP_HIP(hipMemcpyAsync(sm.as<double>() + nrhs, h.data() + nrhs, nrhs * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+        part.alloc((size_t)nrhs * grid.x * sizeof(double));
+        hipLaunchKernelGGL(k_pnorm_scan<1>, grid, dim3(kBlock), 0, s, A, g0, (const double*)(sm.as<double>() + nrhs),
+                           part.as<double>());
+        LP_HIP(hipGetLastError());
+        std::vector<double> hp((size_t)nrhs * grid.x);
+        LP_HIP(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        LP_HIP(hipStreamSynchronize(s));
+        for (int r = 0; r < nrhs; ++r) {
+            double sum = 0.0;
+            for (unsigned b = 0; b < grid.x; ++b) sum += hp[(size_t)r * grid.x + b];
+            const double scr = h[nrhs + r];
+            pm[r] = scr * std::pow(sum / nc, 1.0 / p);
+        }
+        if (!rhs && !gx) return LP_OK;
+        // per rhs: explicit gradient and the node-assembled adjoint right-hand side
+        re.alloc(g0.ncell * 24 * sizeof(double));
+        double* drhs = rhs;
+        double* dgx = gx;
+        if (!device_ptr) {
+            if (rhs) drhs = static_cast<double*>(out_r.alloc(ustride * nrhs * 8), out_r.p);
+            if (gx) dgx = static_cast<double*>(out_g.alloc(g0.ncell * nrhs * 8), out_g.p);
+        }
+        for (int r = 0; r < nrhs; ++r) {
+            hipLaunchKernelGGL(k_pnorm_cell, dim3(nblk(g0.ncell)), dim3(kBlock), 0, s, A, g0, r, pm[r], dgx,
+                               re.as<double>());
+            LP_HIP(hipGetLastError());
+            if (drhs) {
+                hipLaunchKernelGGL(k_pnorm_node, dim3(nblk(g0.nn)), dim3(kBlock), 0, s, (const double*)re.as<double>(),
+                                   g0, G.fixed.as<uint8_t>(), drhs + r * ustride);
+                LP_HIP(hipGetLastError());
+            }
+        }
+        if (!device_ptr) {
+            if (rhs) LP_HIP(hipMemcpyAsync(rhs, drhs, ustride * nrhs * 8, hipMemcpyDeviceToHost, s));
+            if (gx) LP_HIP(hipMemcpyAsync(gx, dgx, g0.ncell * nrhs * 8, hipMemcpyDeviceToHost, s));
+        }
+        LP_HIP(hipStreamSynchronize(s));
+        return LP_OK;
+    });
+}
+
+int lp_element_product(lp_grid* g, int nrhs, const double* a, const double* b, double* out, int device_ptr) {
+    return api_call([&]() -> int {
+        if (!g || nrhs < 1 || !a || !b || !out) {
+            set_error("lp_element_product: bad argument");
+            return LP_ERR_ARG;
+        }
+        Grid& G = *g;
+        const Geom& g0 = G.g0;
+        hipStream_t s = G.stream;
+        const size_t n = 3 * g0.nn;
+        DBuf sa, sb, so;
+        const double* da = static_cast<const double*>(upload(G, sa, a, n * nrhs * 8, device_ptr));
+        const double* db = static_cast<const double*>(upload(G, sb, b, n * nrhs * 8, device_ptr));
+        double* dout = device_ptr ? out : static_cast<double*>((so.alloc(g0.ncell * nrhs * 8), so.p));
+        hipLaunchKernelGGL(k_element_product, dim3(nblk(g0.ncell), nrhs), dim3(kBlock), 0, s, da, db,
+                           G.ke64.as<double>(), g0, n, dout);
+        LP_HIP(hipGetLastError());
+        if (!device_ptr) LP_HIP(hipMemcpyAsync(out, dout, g0.ncell * nrhs * 8, hipMemcpyDeviceToHost, s));
+        LP_HIP(hipStreamSynchronize(s));
+        return LP_OK;
+    });
+}
+
+}  // extern "C"

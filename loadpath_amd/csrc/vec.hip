@@ -240,6 +240,46 @@ __global__ void k_warm_norm(const double* __restrict__ b, const double* __restri
     if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = t;
 }
 
+// per-block max |v| (the V-cycle input scale of the mixed path)
+__global__ void k_absmax(const double* __restrict__ v, size_t n, size_t stride, double* partials) {
+    __shared__ double s[256];
+    const int rhs = blockIdx.y;
+    double m = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(v[rhs * stride + i]));
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = s[0];
+}
+
+__global__ void k_absmax_final(const double* __restrict__ partials, int nb, double* scale) {
+    __shared__ double s[256];
+    const int rhs = blockIdx.x;
+    double m = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) m = fmax(m, partials[(size_t)rhs * nb + i]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) scale[rhs] = s[0] > 0 ? s[0] : 1.0;
+}
+
+// dst = src / scale[rhs] (invert = 0) or src * scale[rhs] (invert = 1)
+template <typename S, typename D>
+__global__ void k_cast_scaled(const S* __restrict__ src, D* __restrict__ dst, size_t n, size_t stride,
+                              const double* __restrict__ scale, int invert) {
+    const int rhs = blockIdx.y;
+    const double f = invert ? scale[rhs] : 1.0 / scale[rhs];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[rhs * stride + i] = (D)((double)src[rhs * stride + i] * f);
+}
+
 inline int grid_for(size_t n) {
     size_t b = (n + kBlock - 1) / kBlock;
     if (b > 2048) b = 2048;
@@ -290,6 +330,30 @@ void launch_cast(const void* src, int sb, void* dst, int db, size_t n, size_t st
         hipLaunchKernelGGL((k_cast<double, double>), grid, dim3(kBlock), 0, s, (const double*)src, (double*)dst, n, stride);
     else
         hipLaunchKernelGGL((k_cast<float, float>), grid, dim3(kBlock), 0, s, (const float*)src, (float*)dst, n, stride);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_absmax(const double* v, size_t n, size_t stride, int nrhs, double* partials, double* scale,
+                   hipStream_t s) {
+    const int nb = dot_blocks(n);
+    hipLaunchKernelGGL(k_absmax, dim3(nb, nrhs), dim3(256), 0, s, v, n, stride, partials);
+    hipLaunchKernelGGL(k_absmax_final, dim3(nrhs), dim3(256), 0, s, partials, nb, scale);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_cast_scaled(const void* src, int sb, void* dst, int db, size_t n, size_t stride, int nrhs,
+                        const double* scale, int invert, hipStream_t s) {
+    dim3 grid(grid_for(n), nrhs);
+    if (sb == 8 && db == 4)
+        hipLaunchKernelGGL((k_cast_scaled<double, float>), grid, dim3(kBlock), 0, s, (const double*)src, (float*)dst, n,
+                           stride, scale, invert);
+    else if (sb == 4 && db == 8)
+        hipLaunchKernelGGL((k_cast_scaled<float, double>), grid, dim3(kBlock), 0, s, (const float*)src, (double*)dst, n,
+                           stride, scale, invert);
+    else {
+        set_error("cast_scaled: unsupported types");
+        throw ArgFailure();
+    }
     LP_HIP(hipGetLastError());
 }
 

@@ -16,7 +16,7 @@ coarsest level is factored on the GPU (FP64 Cholesky -> inverse) instead of
 SuperLU on the host, so ``lu_max_dofs`` only bounds its size; ``dtype``
 float64 selects the all-FP64 V-cycle. Extra keywords: ``smoother``
 ("chebyshev" | "jacobi"), ``omega``, ``max_rhs``, ``max_levels``.
-``springs`` / ``lowrank`` (extra sparse terms) are outside this build.
+``springs`` / ``lowrank`` (extra sparse terms) are outside this package.
 """
 from __future__ import annotations
 

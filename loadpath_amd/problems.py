@@ -1,7 +1,7 @@
 """Box problems on unit voxel grids: supports, loads and seeded inputs.
 
 Textbook cases follow loadpath/benchmarks/problems.py (cantilever_3d
-:107-119, bridge_3d :122-136). The SIMP workloads of this build are named by
+:107-119, bridge_3d :122-136). The SIMP workloads of this package are named by
 what they are:
 
 - ``cantilever_48x24x24``: clamped x = 0 face, unit downward point load at

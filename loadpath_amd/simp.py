@@ -111,6 +111,8 @@ class GridSimp:
         stats = (_lib.SimpStats * max(n, 1))()
         _lib.check(_lib.lib().lp_simp_iterate(self._h, n, stats), "simp_iterate")
         self.iterations += n
+        # the hierarchy was rebuilt on the device: refresh the level table
+        self.grid._refresh(lmax=False)
         out = []
         for k in range(n):
             s = stats[k]

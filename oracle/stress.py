@@ -1,0 +1,141 @@
+"""Oracle: the p-mean von Mises stress aggregate and its adjoint sensitivity
+(FP64, numpy).
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+Follows loadpath/constrained.py, the non-averaged stress branch:
+- sample points: the centre (fem.hex8_b_center, fem.py:307-308) or the 8
+  corners, x outermost, with the condensed bubbles (constrained.py:384-387;
+  fem.hex8_b, fem.py:227-237); sigma = C B u_e / pitch
+- vm = sqrt(max(s^T V s, 1e-30)), V of constrained.py:107-114
+- s_rel = x^q vm / lim; pm = sc (sum (s_rel / sc)^P / nc)^(1/P), sc = max
+  (s_max, 1e-30); g = kappa pm                          constrained.py:622-641
+- dg/ds = kappa (s_rel / pm)^(P-1) / nc; explicit part dg/dx at fixed u;
+  adjoint right-hand side sum dg/ds x^q / (vm lim) (V sigma)^T C B, zero on
+  fixed dofs                                            constrained.py:643-653
+- dg/dx -= dE/dx lambda_e^T Ke u_e, K lambda = rhs      constrained.py:655-659
+
+Pinned by (tests/test_oracle_stress.py):
+- centre stress of an affine displacement field equals C eps exactly (the
+  B matrix reproduces any constant strain), and vm of a uniaxial stress is
+  |s|, of pure shear sqrt(3)|t| (closed forms);
+- pm -> s_max as P grows, pm = s_max when every cell reads the same stress;
+- the whole gradient (explicit + adjoint through a direct sparse solve)
+  against FP64 central finite differences of the forward functional, on
+  seeded elements of a small clamped bar, centre and corner sampling;
+- the compliance gradient -p x^(p-1) (1-Emin) u_e^T Ke u_e against the same
+  finite differences (the self-adjoint case).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .fem import b_matrix, hex8_b, isotropic_c, von_mises_matrix
+
+VM = von_mises_matrix()
+P_DEFAULT = 8.0     # constrained._P (constrained.py:46)
+Q_DEFAULT = 1.0     # constrained._Q (constrained.py:45)
+
+
+def sample_cb(nu: float, points: str = "center", element: str = "im", pitch: float = 1.0) -> np.ndarray:
+    """(npts, 6, 24) C B / pitch at the stress sample points
+    (constrained.py:383-387)."""
+    c0 = isotropic_c(1.0, nu)
+    if points == "center":
+        return (c0 @ b_matrix(0.0, 0.0, 0.0) / pitch)[None]
+    if points == "corners":
+        return np.array([c0 @ hex8_b(a, b, c, nu, element) / pitch
+                         for a in (-1.0, 1.0) for b in (-1.0, 1.0) for c in (-1.0, 1.0)])
+    raise ValueError(points)
+
+
+def pnorm_forward(ue, x, cbs, p=P_DEFAULT, q=Q_DEFAULT, lim=1.0, kappa=1.0, nc=None):
+    """ue (ne, 24) element displacements, x (ne,) material share.
+    Returns (g, pm, smax, parts) with parts per sample point
+    (cb, sv, vm, s_rel) (constrained.py:618-641)."""
+    ue = np.asarray(ue, float)
+    x = np.asarray(x, float)
+    if nc is None:
+        nc = len(x) * len(cbs)
+    parts = []
+    for cb in cbs:
+        sig = ue @ cb.T
+        sv = sig @ VM
+        vm = np.sqrt(np.maximum(np.einsum("ij,ij->i", sv, sig), 1e-30))
+        parts.append((cb, sv, vm, x ** q * vm / lim))
+    smax = max((float(sr.max()) for *_, sr in parts if len(sr)), default=0.0)
+    sc = max(smax, 1e-30)
+    pm = sc * (sum(float(np.sum((sr / sc) ** p)) for *_, sr in parts) / nc) ** (1.0 / p)
+    return kappa * pm, pm, smax, parts
+
+
+def pnorm_partials(ue, x, cbs, p=P_DEFAULT, q=Q_DEFAULT, lim=1.0, kappa=1.0, nc=None):
+    """(g, pm, smax, g_explicit (ne,), rhs_e (ne, 24)): the explicit
+    derivative in x at fixed u and the element adjoint right-hand sides
+    dg/du_e (constrained.py:643-652, dmat = 1)."""
+    x = np.asarray(x, float)
+    if nc is None:
+        nc = len(x) * len(cbs)
+    g, pm, smax, parts = pnorm_forward(ue, x, cbs, p, q, lim, kappa, nc)
+    gx = np.zeros(len(x))
+    rhs_e = np.zeros((len(x), 24))
+    for cb, sv, vm, sr in parts:
+        dg_ds = kappa * (sr / max(pm, 1e-30)) ** (p - 1) / nc
+        gx += dg_ds * (q * np.maximum(x, 1e-12) ** (q - 1) * vm / lim)
+        coef = dg_ds * (x ** q / (vm * lim))
+        rhs_e += coef[:, None] * (sv @ cb)
+    return g, pm, smax, gx, rhs_e
+
+
+def assemble_rhs(edof, rhs_e, n_dof, fixed=None):
+    rhs = np.bincount(np.asarray(edof).ravel(), weights=np.asarray(rhs_e).ravel(), minlength=n_dof)
+    if fixed is not None:
+        rhs[np.asarray(fixed, bool)] = 0.0
+    return rhs
+
+
+def element_product(a, b, edof, ke):
+    """a_e^T Ke b_e per element (constrained.py:659; the bilinear form of
+    femops.element_energy)."""
+    return np.einsum("ij,jk,ik->i", a[edof], ke, b[edof])
+
+
+def pnorm_gradient(mesh, ke, fixed, f, xphys, cbs, penal=3.0, emin=1e-9, p=P_DEFAULT, q=Q_DEFAULT, lim=1.0,
+                   kappa=1.0):
+    """The whole objective and its gradient in xphys through a direct FP64
+    solve: (g, dg/dx, u, lambda). E = emin + x^penal (1 - emin); the stress
+    reads the material share x itself (constrained.py:423-428, share None)."""
+    import scipy.sparse.linalg as spla
+
+    from .fem import assemble
+
+    x = np.asarray(xphys, float)
+    e = emin + x ** penal * (1 - emin)
+    de = penal * x ** (penal - 1) * (1 - emin)
+    k = assemble(mesh, ke, e)
+    free = ~np.asarray(fixed, bool)
+    kf = k[free][:, free].tocsc()
+    lu = spla.splu(kf)
+    u = np.zeros(mesh.n_dof)
+    u[free] = lu.solve(f[free])
+    edof = mesh.edof
+    g, pm, smax, gx, rhs_e = pnorm_partials(u[edof], x, cbs, p, q, lim, kappa)
+    rhs = assemble_rhs(edof, rhs_e, mesh.n_dof, fixed)
+    lam = np.zeros(mesh.n_dof)
+    lam[free] = lu.solve(rhs[free])
+    grad = gx - de * element_product(lam, u, edof, ke)
+    return g, grad, u, lam
+
+
+def pnorm_value(mesh, ke, fixed, f, xphys, cbs, penal=3.0, emin=1e-9, p=P_DEFAULT, q=Q_DEFAULT, lim=1.0,
+                kappa=1.0):
+    import scipy.sparse.linalg as spla
+
+    from .fem import assemble
+
+    x = np.asarray(xphys, float)
+    k = assemble(mesh, ke, emin + x ** penal * (1 - emin))
+    free = ~np.asarray(fixed, bool)
+    u = np.zeros(mesh.n_dof)
+    u[free] = spla.spsolve(k[free][:, free].tocsc(), f[free])
+    return pnorm_forward(u[mesh.edof], x, cbs, p, q, lim, kappa)[0]

@@ -1,0 +1,108 @@
+"""The mixed-precision path: FP16-MFMA fine-level contraction inside an FP32
+V-cycle under the FP64 CG. The operator is checked against the FP64 oracle
+to FP16 tolerance (with and without a large rigid offset, which the
+per-cell translation removal must cancel); the solve and the SIMP loop must
+land on the FP64 answers to the solve tolerance."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from tests.conftest import cantilever_bar
+
+pytestmark = pytest.mark.gpu
+
+# FP16 operands (11-bit significand) with FP32 accumulation: relative error
+# of a Ke.u product ~ sqrt(24) * 2^-11 ~ 2.4e-3 of its magnitude at worst
+TOL_FP16 = 4e-3
+
+
+def _grids(mesh, fixed, e, **kw):
+    from loadpath_amd.mg import GridElasticity
+    from oracle.mg import GridOracle
+
+    g = GridElasticity(mesh, 0.3, fixed, precision="mixed", **kw)
+    o = GridOracle(mesh, 0.3, fixed, vdtype=np.float64, **kw)
+    g.set_moduli(e)
+    o.set_moduli(e)
+    return g, o
+
+
+def test_mfma_operator_matches_fp64():
+    mesh, fixed, f, e = cantilever_bar(12, 10, 9, seed=11)
+    g, o = _grids(mesh, fixed, e, coarse_cells=20)
+    rng = np.random.default_rng(1)
+    shape = o.levels[0].node_shape
+    u = rng.standard_normal((2,) + shape).astype(np.float32)
+    got = g.apply(u, level=0, dtype=np.float32)
+    want = np.stack([o.levels[0].apply(v.astype(np.float64)) for v in u])
+    assert np.abs(got - want).max() <= TOL_FP16 * np.abs(want).max()
+
+
+def test_mfma_operator_cancels_rigid_offset():
+    """A large common offset on an unsupported grid (every dof free): each
+    cell's FP16 operand is taken relative to its corner 0, so the product
+    keeps FP16 accuracy relative to the deformation, not to the offset."""
+    mesh, fixed, f, e = cantilever_bar(12, 10, 8, seed=12)
+    free = np.zeros_like(fixed)
+    # a single level: the unsupported operator is singular (rigid modes), so
+    # no coarsest factorization; only the fine operator is exercised
+    g, o = _grids(mesh, free, e, coarse_cells=10 ** 9)
+    assert o.levels[0].mask.all()
+    rng = np.random.default_rng(2)
+    shape = o.levels[0].node_shape
+    d = rng.standard_normal(shape)
+    u = (1000.0 + d).astype(np.float32)
+    got = g.apply(u[None], level=0, dtype=np.float32)[0]
+    want = o.levels[0].apply(u.astype(np.float64))
+    assert np.abs(got - want).max() <= TOL_FP16 * np.abs(want).max()
+
+
+def test_galerkin_level1_matches_oracle():
+    """Level 1 applied as P^T (m K0 m) P through the fine MFMA operator
+    equals the oracle's stored Galerkin level (FP64) to FP16 tolerance, on
+    a grid with supports and removed cells (patch cells present)."""
+    mesh, fixed, f, e = cantilever_bar(16, 10, 8, seed=13)
+    rng = np.random.default_rng(14)
+    e = e.copy()
+    e[rng.random(mesh.n_elem) < 0.1] = 0.0
+    e[mesh.elem_ijk[:, 0] == 0] = 1.0
+    g, o = _grids(mesh, fixed, e, coarse_cells=20)
+    lv = o.levels[1]
+    u = rng.standard_normal((2,) + lv.node_shape).astype(np.float32)
+    got = g.apply(u, level=1, dtype=np.float32)
+    want = np.stack([lv.apply(v.astype(np.float64)) for v in u])
+    assert np.abs(got - want).max() <= TOL_FP16 * np.abs(want).max()
+
+
+def test_mixed_solve_matches_direct():
+    from oracle.fem import assemble, hex8_ke
+
+    mesh, fixed, f, e = cantilever_bar(16, 8, 8, seed=2)
+    k = assemble(mesh, hex8_ke(0.3), e)
+    free = ~fixed
+    ref = np.zeros(mesh.n_dof)
+    ref[free] = spla.spsolve(k[free][:, free].tocsc(), f[free])
+    from loadpath_amd.mg import GridElasticity
+
+    g = GridElasticity(mesh, 0.3, fixed, precision="mixed", rtol=1e-10, coarse_cells=100, maxiter=2000)
+    g.set_moduli(e)
+    u = g.solve_many([f, 3.0 * f])
+    assert np.linalg.norm(u[0] - ref) <= 1e-7 * np.linalg.norm(ref)
+    assert np.linalg.norm(u[1] - 3.0 * ref) <= 1e-7 * np.linalg.norm(3.0 * ref)
+    g32 = GridElasticity(mesh, 0.3, fixed, precision="fp32", rtol=1e-10, coarse_cells=100, maxiter=2000)
+    g32.set_moduli(e)
+    g32.solve_many([f])
+    assert g.last_iters <= 1.3 * g32.last_iters + 2, (g.last_iters, g32.last_iters)
+
+
+def test_mixed_simp_tracks_fp64():
+    from loadpath_amd.problems import multiload_256x128x128
+    from loadpath_amd.simp import GridSimp
+
+    p = multiload_256x128x128(32, 16, 16)
+    a = GridSimp(p, precision="mixed", rtol=1e-9, coarse_cells=0, max_levels=4)
+    b = GridSimp(p, precision="fp64", rtol=1e-9, coarse_cells=0, max_levels=4)
+    ha, hb = a.iterate(3), b.iterate(3)
+    for sa, sb in zip(ha, hb):
+        assert sa["compliance"] == pytest.approx(sb["compliance"], rel=1e-7)
+    assert np.abs(a.xphys - b.xphys).max() < 1e-6
