@@ -1,0 +1,81 @@
+"""torch.autograd bindings of the device objectives: compliance and the
+p-mean von Mises stress aggregate as functions of the element densities.
+
+The forward sets E = Emin + x^p (1 - Emin) on a GridElasticity and solves
+with its MG-PCG on the GPU; the backward is the analytic sensitivity
+(loadpath/simp.py:1897-1904 for compliance, whose adjoint is the state
+itself) or, for the stress aggregate, an adjoint solve on the same
+hierarchy (constrained.py:607-664). The functions compose with torch
+operations on x (filters, projections), so a design pipeline written in
+torch differentiates end to end.
+
+    c = compliance(x, grid, f)            # x: (n_elem,) float64 tensor
+    s = stress_pnorm(x, agg, f)           # agg: stress.StressPNorm
+    (c + w * s).backward()
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+__all__ = ["compliance", "stress_pnorm", "Compliance", "StressPNormObjective"]
+
+
+def _np(x: torch.Tensor) -> np.ndarray:
+    return x.detach().to("cpu", torch.float64).numpy()
+
+
+def _set(grid, e):
+    if getattr(grid, "_e", None) is not e:
+        grid.set_moduli(e)
+
+
+class Compliance(torch.autograd.Function):
+    """c = f . u(x); dc/dx = -p x^(p-1) (1 - Emin) u_e^T Ke u_e."""
+
+    @staticmethod
+    def forward(ctx, x, grid, f, penal, emin):
+        xd = _np(x)
+        e = emin + xd ** penal * (1 - emin)
+        grid.set_moduli(e)
+        u = grid.solve_many([f])[0]
+        ce = grid.strain_energy(u)
+        ctx.dc = -penal * (1 - emin) * xd ** (penal - 1) * ce
+        return x.new_tensor(float(np.dot(f, u)))
+
+    @staticmethod
+    def backward(ctx, gout):
+        return gout * torch.as_tensor(ctx.dc, dtype=gout.dtype, device=gout.device), None, None, None, None
+
+
+class StressPNormObjective(torch.autograd.Function):
+    """g = kappa pm(u(x), x); the backward solves K lambda = dg/du on the
+    same hierarchy and returns explicit - dE/dx lambda_e^T Ke u_e."""
+
+    @staticmethod
+    def forward(ctx, x, agg, f, penal, emin, kappa):
+        xd = _np(x)
+        e = emin + xd ** penal * (1 - emin)
+        agg.grid.set_moduli(e)
+        u = agg.grid.solve_many([f])[0]
+        ev = agg.evaluate([u], xd, kappa)
+        ctx.agg, ctx.e, ctx.u = agg, e, u
+        ctx.rhs, ctx.gx = ev["rhs"][0], ev["gx"][0]
+        ctx.de = penal * xd ** (penal - 1) * (1 - emin)
+        return x.new_tensor(float(ev["g"][0]))
+
+    @staticmethod
+    def backward(ctx, gout):
+        grid = ctx.agg.grid
+        _set(grid, ctx.e)
+        lam = grid.solve_many([ctx.rhs])[0]
+        grad = ctx.gx - ctx.de * grid.element_product(lam, ctx.u)
+        return (gout * torch.as_tensor(grad, dtype=gout.dtype, device=gout.device), None, None, None, None, None)
+
+
+def compliance(x: torch.Tensor, grid, f, penal: float = 3.0, emin: float = 1e-9) -> torch.Tensor:
+    return Compliance.apply(x, grid, np.asarray(f, float), float(penal), float(emin))
+
+
+def stress_pnorm(x: torch.Tensor, agg, f, penal: float = 3.0, emin: float = 1e-9, kappa: float = 1.0):
+    return StressPNormObjective.apply(x, agg, np.asarray(f, float), float(penal), float(emin), float(kappa))

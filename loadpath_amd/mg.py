@@ -298,6 +298,16 @@ class GridElasticity:
         _lib.check(_lib.lib().lp_strain_energy(self._h, 1, _lib.ptr(g), _lib.ptr(se), 0), "strain_energy")
         return se[0][tuple(self.elem_ijk.T)]
 
+    def element_product(self, a, b):
+        """Per element a_e^T Ke b_e, mesh order (the adjoint term of
+        constrained.py:659; strain_energy when a is b)."""
+        ga = np.ascontiguousarray(self.to_grid(a)[None])
+        gb = ga if b is a else np.ascontiguousarray(self.to_grid(b)[None])
+        out = np.empty((1,) + self.cells)
+        _lib.check(_lib.lib().lp_element_product(self._h, 1, _lib.ptr(ga), _lib.ptr(gb), _lib.ptr(out), 0),
+                   "element_product")
+        return out[0][tuple(self.elem_ijk.T)]
+
     def time_apply(self, level=0, dtype=np.float64, nrhs=1, reps=20) -> float:
         """Mean device time (ms) of one operator apply (HIP events)."""
         ms = C.c_double()

@@ -55,6 +55,29 @@ class BoxProblem:
                 g[_AX[c], nd[:, 0], nd[:, 1], nd[:, 2]] = True
         return g
 
+    def mesh(self):
+        """HexMesh of the present cells (unit pitch)."""
+        from .fem import HexMesh
+
+        if self.present is None:
+            ii, jj, kk = np.meshgrid(*[np.arange(d) for d in self.dims], indexing="ij")
+            cells = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1)
+        else:
+            cells = np.argwhere(self.present)
+        return HexMesh(cells, 1.0)
+
+    def dof_vectors(self, mesh):
+        """(fixed (n_dof,) bool, loads (ncases, n_dof)) on a mesh of this box."""
+        at = tuple(mesh.node_ijk.T)
+        fg = self.fixed_grid()
+        lg = self.load_grids()
+        fixed = np.empty(mesh.n_dof, bool)
+        loads = np.empty((len(self.cases), mesh.n_dof))
+        for c in range(3):
+            fixed[c::3] = fg[c][at]
+            loads[:, c::3] = lg[:, c][(slice(None),) + at]
+        return fixed, loads
+
     def load_grids(self) -> np.ndarray:
         """(ncases, 3, nx+1, ny+1, nz+1) FP64 nodal forces."""
         out = np.zeros((len(self.cases), 3) + self.node_shape)
@@ -100,15 +123,17 @@ def mbb_192x64x64(seed=2):
                       [[(top, "z", -1.0 / len(top))]], density=rho)
 
 
-def lbracket_128x64x64(seed=3):
-    nx, ny, nz = 128, 64, 64
+def lbracket_128x64x64(seed=3, nx=128, ny=64, nz=64):
+    """Scaled copies (other nx, ny, nz) keep the shape: the block
+    x >= nx/2, z >= nz/2 removed, the top face x <= nx/2 clamped, the load
+    on the x = nx face nodes with z <= nz/16."""
     present = np.ones((nx, ny, nz), bool)
-    present[64:, :, 32:] = False
-    tip = nodes(nx, range(ny + 1), range(0, 5))
+    present[nx // 2:, :, nz // 2:] = False
+    tip = nodes(nx, range(ny + 1), range(0, max(nz // 16, 1) + 1))
     rng = np.random.default_rng(seed)
     rho = rng.uniform(0.1, 1.0, (nx, ny, nz))
-    return BoxProblem("lbracket-128x64x64", (nx, ny, nz), 0.3, 1.5,
-                      [(nodes(range(0, 65), range(ny + 1), nz), "xyz")],
+    return BoxProblem(f"lbracket-{nx}x{ny}x{nz}", (nx, ny, nz), 0.3, 1.5,
+                      [(nodes(range(0, nx // 2 + 1), range(ny + 1), nz), "xyz")],
                       [[(tip, "z", -1.0 / len(tip))]], present=present, density=rho)
 
 
