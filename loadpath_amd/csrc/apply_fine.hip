@@ -40,7 +40,7 @@ AGeom make_apply_geom(const Grid& Gd, int level) {
 }
 
 int apply_blocks(const Grid& Gd, int level, int eb) {
-    if (galerkin1_path(Gd, level, eb)) return (int)((Gd.lv[level].g.nn + kBlock - 1) / kBlock);
+    if (weights_mfma_path(Gd, level, eb)) return mfma_blocks(Gd, level);
     if (Gd.lv[level].kind == LK_FINE && eb == 4 && Gd.opt.precision == LP_PREC_MIXED) return mfma_blocks(Gd, level);
     const AGeom G = make_apply_geom(Gd, level);
     return Gd.lv[level].kind == LK_FINE ? 8 * G.per : G.nblk;
@@ -85,8 +85,8 @@ int launch_apply(const Grid& G, int level, int eb, const void* u, size_t vstride
                 launch_apply_fine(G, level, eb, u, vstride, nrhs, E, s);
             break;
         case LK_WEIGHTS:
-            if (galerkin1_path(G, level, eb))
-                launch_apply_galerkin1(G, level, u, vstride, nrhs, E, s);
+            if (weights_mfma_path(G, level, eb))
+                launch_apply_weights_mfma(G, level, u, vstride, nrhs, E, s);
             else
                 launch_apply_weights(G, level, eb, u, vstride, nrhs, E, s);
             break;

@@ -1,10 +1,16 @@
-// Fine-level operator for the mixed-precision V-cycle: the shared 24x24 Ke
-// times every cell's corner vector as an FP16 MFMA contraction with FP32
-// accumulation, then a node-centric sum in LDS.
+// Level operators of the mixed-precision V-cycle as FP16 MFMA contractions
+// with FP32 accumulation, then a node-centric sum in LDS:
+//  - the fine level: every cell's corner vector times the shared 24x24 Ke,
+//    scaled by the cell's modulus;
+//  - the first coarse ("weights") level: every coarse cell's corner vector
+//    times sum_q w_q M_q, M_q = P_q^T Ke P_q the child maps and w_q the
+//    moduli of its 8 fine children; cells touching held dofs use their
+//    explicit patch matrices (products from k_patch_prod, run first).
 //
-// Reference: the same operator as mg._Level.apply on the fine level
-// (mg.py:251-346; mgops._product_scaled + _scatter_into, mgops.py:80-217),
-// which the reference evaluates in FP32 inside the V-cycle.
+// Reference: the same operators as mg._Level.apply on the fine and first
+// coarse level (mg.py:251-346; mgops._product_scaled + _scatter_into,
+// mgops.py:80-217; the weights level mg.py:609-642), which the reference
+// evaluates in FP32 inside the V-cycle.
 //
 // Tiling: a block owns a 15 x 15 tile of nodes in the (y, z) plane and marches
 // an x-chunk of node planes. The 16 x 16 = 256 cells touching the tile are
@@ -14,17 +20,19 @@
 //
 // Per cell layer:
 //  - B (32 cells x 24) = each cell's corner vector relative to its corner 0
-//    (Ke annihilates translations, so the product is unchanged) times a
-//    power of two chosen per cell from its largest entry, so FP16 carries the
-//    cell's deformation at full relative precision whatever its magnitude (a
-//    void cell's correction can be 1e9 times a solid one's);
-//  - A = Ke with its rows permuted so a lane's 12 output rows are the 4
-//    corners on one side of the layer (x-low in lanes 0-31, x-high in lanes
-//    32-63): 3 v_mfma_f32_32x32x8_f16 per 32 cells;
-//  - the products, times E / scale (exact), go to LDS as (row, cell) with a
-//    row stride of 8 mod 16 floats (conflict-free in both half-waves); every
-//    node then sums its 4 cells' x-low rows (finishing the node plane) and
-//    x-high rows (starting the next plane), and runs the epilogue.
+//    (Ke and every M_q annihilate translations, so the product is unchanged)
+//    times a power of two chosen per cell from its largest entry, so FP16
+//    carries the cell's deformation at full relative precision whatever its
+//    magnitude (a void cell's correction can be 1e9 times a solid one's);
+//  - A = Ke (or M_q) with its rows permuted so a lane's 12 output rows are
+//    the 4 corners on one side of the layer (x-low in lanes 0-31, x-high in
+//    lanes 32-63): 3 v_mfma_f32_32x32x8_f16 per 32 cells (per child q on the
+//    weights level, whose products are summed with the weights w_q);
+//  - the products go to LDS as (row, cell) with a row stride of 8 mod 16
+//    floats (conflict-free in both half-waves), with one factor per cell
+//    (E / scale); every node then sums its 4 cells' x-low rows (finishing
+//    the node plane) and x-high rows (starting the next plane), and runs the
+//    epilogue.
 #include "apply_impl.h"
 
 namespace lp {
@@ -42,6 +50,7 @@ constexpr int PSTR = 292;           // component stride of a staged plane (float
 constexpr int NCELL = TC * TC;      // 256 cells = 8 tiles of 32
 constexpr int FSTR = 264;           // row stride of the product buffer (= 8 mod 16)
 constexpr int kThreads = 256;
+constexpr int kWeightsXChunk = 8;   // node planes per block on the weights level
 static_assert(NCELL == 8 * 32, "cells per layer must fill the MFMA tiles");
 static_assert(2 * kThreads >= NSTAGE, "two staging entries per thread");
 
@@ -52,14 +61,25 @@ __host__ __device__ constexpr int ke_row_of(int R) {
     return 3 * (2 * (((R >> 3) * 4 + (R & 3)) / 3) + ((R >> 2) & 1)) + ((R >> 3) * 4 + (R & 3)) % 3;
 }
 
-template <int EK>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) k_apply_fine_mfma(const float* __restrict__ u0, size_t vstride,
-                                                              const uint8_t* __restrict__ mask, AGeom G,
-                                                              const float* __restrict__ ecell,
-                                                              const float* __restrict__ ke, EpiArgs E) {
+// Ke row of product slot q (0..11) on side hh of the layer
+__device__ __forceinline__ int row_of_slot(int hh, int q) { return ke_row_of(8 * (q >> 2) + 4 * hh + (q & 3)); }
+
+struct MfmaSrc {
+    const float* e;       // fine: cell moduli; weights: the finer level's cell moduli
+    const float* a;       // fine: Ke (576); weights: M_q (8 x 576)
+    const int* pidx;      // weights: patch slot per cell or -1
+    const float* pprod;   // weights: (nrhs, npatch, 24) patch products of this apply
+    int npatch;
+};
+
+template <int EK, bool WTS>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WTS ? 3 : 4, 4)))
+k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __restrict__ mask, AGeom G, MfmaSrc S,
+             EpiArgs E) {
     __shared__ float planes[3 * 3 * PSTR];
     __shared__ float fl[24 * FSTR];
-    __shared__ float fsc[NCELL];        // per cell: E / (its FP16 operand scale)
+    __shared__ float fsc[NCELL];        // per cell: its products' common factor
+    __shared__ __align__(8) _Float16 alds[WTS ? 8 * 3 * 64 * 4 : 4];
     using namespace apply_detail;
     const int rhs = blockIdx.z;
     const int L = blockIdx.x;
@@ -81,18 +101,28 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
         const unsigned nnu = (unsigned)G.nn;
         const __amdgpu_buffer_rsrc_t ru = make_rsrc(u, 3u * nnu * 4u);
         const __amdgpu_buffer_rsrc_t rm = make_rsrc(mask, nnu);
-        const __amdgpu_buffer_rsrc_t re = make_rsrc(ecell, (unsigned)G.ncell * 4u);
+        const size_t ne = WTS ? (size_t)G.fnx * G.fny * G.fnz : G.ncell;
+        const __amdgpu_buffer_rsrc_t re = make_rsrc(S.e, (unsigned)ne * 4u);
         const int tid = (int)threadIdx.x;
         const int lane = tid & 63;
         const int wave = tid >> 6;
         const int jc = lane & 31;          // cell column of the MFMA tile / output row of A
         const int hh = lane >> 5;          // K half (A, B) and output row half (D)
         half4 A[3];
+        if (WTS) {
+            // the 8 child maps as FP16 A operands, lane-ordered: (q, s, lane, i)
+            for (int k = tid; k < 8 * 3 * 64 * 4; k += kThreads) {
+                const int i = k & 3, l = (k >> 2) & 63, s = (k >> 8) % 3, q = k / 768;
+                const int row = l & 31, col = 8 * s + 4 * (l >> 5) + i;
+                alds[k] = row < 24 ? (_Float16)S.a[q * 576 + ke_row_of(row) * 24 + col] : (_Float16)0.0f;
+            }
+        } else {
 #pragma unroll
-        for (int s = 0; s < 3; ++s)
+            for (int s = 0; s < 3; ++s)
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                A[s][i] = jc < 24 ? (_Float16)ke[ke_row_of(jc) * 24 + 8 * s + 4 * hh + i] : (_Float16)0.0f;
+                for (int i = 0; i < 4; ++i)
+                    A[s][i] = jc < 24 ? (_Float16)S.a[ke_row_of(jc) * 24 + 8 * s + 4 * hh + i] : (_Float16)0.0f;
+        }
 
         // staging: entries tid and tid + 256 of the 17 x 17 patch
         float pre[2][3];
@@ -123,16 +153,17 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
                 }
             }
         };
-        // products of cell layer X (node planes in buffers bl = X, bh = X + 1)
-        // moduli of this lane's two cells (slots wave*32 + jc and that + 128)
-        // of cell layer X; 0 outside the grid
+        // fine level: moduli of this lane's two cells (slots wave*32 + jc and
+        // that + 128) of cell layer X, loaded a layer ahead; 0 outside the grid
         auto ecell_load = [&](int X, int tt) -> float {
+            if (WTS) return 0.0f;
             const int c = (wave + 4 * tt) * 32 + jc;
             const int cy = y0 - 1 + (c >> 4), cz = z0 - 1 + (c & 15);
             const bool cok = X >= 0 && X < G.nx && cy >= 0 && cy < G.ny && cz >= 0 && cz < G.nz;
             return bload<float>(re, cok ? (unsigned)((X * G.ny + cy) * G.nz + cz) * 4u : kOOB);
         };
         float ec0 = 0.0f, ec1 = 0.0f;
+        // products of cell layer X (node planes in buffers bl = X, bh = X + 1)
         auto contract = [&](int X, int bl, int bh) {
             const bool inx = X >= 0 && X < G.nx;
             const float* PL = planes + bl * 3 * PSTR;
@@ -141,10 +172,6 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
             for (int tt = 0; tt < 2; ++tt) {
                 const int c = (wave + 4 * tt) * 32 + jc;      // cell slot
                 const int cj = c >> 4, ck = c & 15;
-                // a cell outside the grid has E = 0 (its staged corners are
-                // finite), so its products are exact zeros
-                const float ec = inx ? ec0 : 0.0f;
-                ec0 = ec1;                                    // the next tile's cell
                 const int s0 = cj * TS + ck;                  // staged index of corner 0
                 float d[12];
                 float dmax = 0.0f;
@@ -163,21 +190,56 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
                 int ex;
                 (void)frexpf(dmax, &ex);                      // dmax = f 2^ex, f in [0.5, 1)
                 const float sc = ldexpf(1.0f, 15 - ex);       // |d| sc < 2^15 < FP16 max
-                if (hh == 0) fsc[c] = ec * ldexpf(1.0f, ex - 15);
                 half4 B[3];
 #pragma unroll
                 for (int s = 0; s < 3; ++s)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) B[s][i] = (_Float16)(d[4 * s + i] * sc);
-                floatx16 acc = {};
-#pragma unroll
-                for (int s = 0; s < 3; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x8f16(A[s], B[s], acc, 0, 0, 0);
-                // lane holds output rows 8 g + 4 hh + i (g < 3): product slots
-                // q = 4 g + i of its side of the layer (unscaled; the gather
-                // applies the cell's fsc)
                 float* dst = fl + hh * 12 * FSTR + c;
+                if (!WTS) {
+                    // a cell outside the grid has E = 0 (its staged corners
+                    // are finite), so its products are exact zeros
+                    const float ec = inx ? ec0 : 0.0f;
+                    ec0 = ec1;                                // the next tile's cell
+                    floatx16 acc = {};
 #pragma unroll
-                for (int q = 0; q < 12; ++q) dst[q * FSTR] = acc[q];
+                    for (int s = 0; s < 3; ++s)
+                        acc = __builtin_amdgcn_mfma_f32_32x32x8f16(A[s], B[s], acc, 0, 0, 0);
+                    if (hh == 0) fsc[c] = ec * ldexpf(1.0f, ex - 15);
+                    // lane holds output rows 8 g + 4 hh + i (g < 3): slots q = 4 g + i
+#pragma unroll
+                    for (int q = 0; q < 12; ++q) dst[q * FSTR] = acc[q];
+                } else {
+                    const int cy = y0 - 1 + cj, cz = z0 - 1 + ck;
+                    const bool cok = inx && cy >= 0 && cy < G.ny && cz >= 0 && cz < G.nz;
+                    const int cell = (X * G.ny + cy) * G.nz + cz;
+                    const int pi = cok ? S.pidx[cell] : -1;
+                    float y[12];
+#pragma unroll
+                    for (int q = 0; q < 12; ++q) y[q] = 0.0f;
+#pragma unroll 2
+                    for (int ch = 0; ch < 8; ++ch) {
+                        const int fx = 2 * X + (ch & 1), fy = 2 * cy + ((ch >> 1) & 1), fz = 2 * cz + ((ch >> 2) & 1);
+                        const bool wok = cok && pi < 0 && fx < G.fnx && fy < G.fny && fz < G.fnz;
+                        const float w = bload<float>(re, wok ? (unsigned)((fx * G.fny + fy) * G.fnz + fz) * 4u : kOOB);
+                        floatx16 acc = {};
+#pragma unroll
+                        for (int s = 0; s < 3; ++s) {
+                            const half4 a = *reinterpret_cast<const half4*>(alds + ((ch * 3 + s) * 64 + lane) * 4);
+                            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a, B[s], acc, 0, 0, 0);
+                        }
+#pragma unroll
+                        for (int q = 0; q < 12; ++q) y[q] = fmaf(w, acc[q], y[q]);
+                    }
+                    if (pi >= 0) {
+                        const float* pp = S.pprod + ((size_t)rhs * S.npatch + pi) * 24;
+#pragma unroll
+                        for (int q = 0; q < 12; ++q) y[q] = pp[row_of_slot(hh, q)];
+                    }
+                    if (hh == 0) fsc[c] = pi >= 0 ? 1.0f : ldexpf(1.0f, ex - 15);
+#pragma unroll
+                    for (int q = 0; q < 12; ++q) dst[q * FSTR] = y[q];
+                }
             }
         };
         const bool active = tid < TN * TN;
@@ -275,8 +337,35 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
     if (E.partials != nullptr) block_partial(part, E, rhs, L);
 }
 
+// patch cells of the weights level: (patch, row) per thread, the explicit
+// 24x24 patch matrix times the cell's masked corner vector (FP32)
+__global__ void __launch_bounds__(kBlock) k_patch_prod(const float* __restrict__ u0, size_t vstride,
+                                                       const uint8_t* __restrict__ mask, Geom g,
+                                                       const int* __restrict__ cells, const float* __restrict__ pm,
+                                                       int npatch, float* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int rhs = blockIdx.y;
+    if (t >= npatch * 24) return;
+    const int p = t / 24, row = t - 24 * (t / 24);
+    const int cell = cells[p];
+    const int Z = cell % g.nz, Y = (cell / g.nz) % g.ny, X = cell / (g.nz * g.ny);
+    const float* u = u0 + (size_t)rhs * vstride;
+    const float* m = pm + (size_t)p * 576 + row * 24;
+    float acc = 0.0f;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const size_t n = ((size_t)(X + (b & 1)) * g.Ny + (Y + ((b >> 1) & 1))) * g.Nz + (Z + ((b >> 2) & 1));
+        const int mk = mask[n];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if ((mk >> c) & 1) acc = fmaf(m[3 * b + c], u[c * g.nn + n], acc);
+    }
+    out[((size_t)rhs * npatch + p) * 24 + row] = acc;
+}
+
 AGeom mfma_geom(const Grid& Gd, int level) {
     AGeom G = make_apply_geom(Gd, level);
+    if (Gd.lv[level].kind == LK_WEIGHTS) G.xchunk = kWeightsXChunk;
     const int nty = (G.Ny + TN - 1) / TN, ntz = (G.Nz + TN - 1) / TN;
     G.ntz = ntz;
     G.nyzb = nty * ntz;
@@ -286,25 +375,20 @@ AGeom mfma_geom(const Grid& Gd, int level) {
     return G;
 }
 
-}  // namespace
-
-int mfma_blocks(const Grid& Gd, int level) { return 8 * mfma_geom(Gd, level).per; }
-
-void launch_apply_fine_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,
-                            hipStream_t s) {
+template <bool WTS>
+void launch_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E, const MfmaSrc& S,
+                 hipStream_t s) {
     const Level& lv = Gd.lv[level];
     const AGeom G = mfma_geom(Gd, level);
     if ((size_t)G.Nx * G.Ny * G.Nz * 3 >= (1ull << 30)) {
-        set_error("fine level: too many nodes for 32-bit buffer offsets");
+        set_error("level operator: too many nodes for 32-bit buffer offsets");
         throw ArgFailure();
     }
     dim3 grid(8 * G.per, 1, nrhs), block(kThreads);
     const float* uf = static_cast<const float*>(u);
     const uint8_t* m = lv.mask.as<uint8_t>();
-    const float* e = lv.e32.as<float>();
-    const float* ke = Gd.ke32.as<float>();
 #define LP_MFMA_CASE(K) \
-    case K: hipLaunchKernelGGL(k_apply_fine_mfma<K>, grid, block, 0, s, uf, vstride, m, G, e, ke, E); break;
+    case K: hipLaunchKernelGGL((k_apply_mfma<K, WTS>), grid, block, 0, s, uf, vstride, m, G, S, E); break;
     switch (E.kind) {
         LP_MFMA_CASE(EPI_STORE)
         LP_MFMA_CASE(EPI_STORE_DOT)
@@ -316,11 +400,51 @@ void launch_apply_fine_mfma(const Grid& Gd, int level, const void* u, size_t vst
         LP_MFMA_CASE(EPI_RESID)
         LP_MFMA_CASE(EPI_MASKED)
         default:
-            set_error("fine level: unknown epilogue");
+            set_error("level operator: unknown epilogue");
             throw ArgFailure();
     }
 #undef LP_MFMA_CASE
     LP_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+int mfma_blocks(const Grid& Gd, int level) { return 8 * mfma_geom(Gd, level).per; }
+
+bool weights_mfma_path(const Grid& G, int level, int eb) {
+    return G.opt.precision == LP_PREC_MIXED && eb == 4 && level >= 1 && G.lv[level].kind == LK_WEIGHTS;
+}
+
+void launch_apply_fine_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,
+                            hipStream_t s) {
+    MfmaSrc S{};
+    S.e = Gd.lv[level].e32.as<float>();
+    S.a = Gd.ke32.as<float>();
+    launch_mfma<false>(Gd, level, u, vstride, nrhs, E, S, s);
+}
+
+void launch_apply_weights_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int nrhs,
+                               const EpiArgs& E, hipStream_t s) {
+    const Level& lv = Gd.lv[level];
+    MfmaSrc S{};
+    S.e = Gd.lv[level - 1].e32.as<float>();
+    S.a = Gd.mq32.as<float>();
+    S.pidx = lv.patch_idx.as<int>();
+    S.npatch = lv.npatch;
+    if (lv.npatch > 0) {
+        const size_t need = (size_t)nrhs * lv.npatch * 24 * sizeof(float);
+        if (lv.patch_prod.bytes < need) {
+            set_error("weights level: patch product buffer smaller than the batch");
+            throw ArgFailure();
+        }
+        S.pprod = lv.patch_prod.as<float>();
+        const unsigned nb = (unsigned)((lv.npatch * 24 + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_patch_prod, dim3(nb, nrhs), dim3(kBlock), 0, s, static_cast<const float*>(u), vstride,
+                           lv.mask.as<uint8_t>(), lv.g, lv.patch_cell.as<int>(), lv.patch32.as<float>(), lv.npatch,
+                           lv.patch_prod.as<float>());
+        LP_HIP(hipGetLastError());
+    }
+    launch_mfma<true>(Gd, level, u, vstride, nrhs, E, S, s);
 }
 
 }  // namespace lp

@@ -476,10 +476,6 @@ int build_hierarchy(Grid& G) {
     long long n_active = (long long)read_counter(counter, s);
     L0.n_active = n_active;
     alloc_vectors(G, L0);
-    if (G.opt.precision == LP_PREC_MIXED && !G.gal_a.p) {
-        G.gal_a.alloc(3 * L0.g.nn * (size_t)G.max_rhs * 4);
-        G.gal_b.alloc(3 * L0.g.nn * (size_t)G.max_rhs * 4);
-    }
     int nl = 1;
     const bool need64 = G.opt.precision == LP_PREC_FP64;
 
@@ -504,8 +500,11 @@ int build_hierarchy(Grid& G) {
             const unsigned long long np = read_counter(counter, s);
             n_active = (long long)read_counter(counter2, s);
             C.npatch = (int)np;
-            DBuf list, ctr;
+            DBuf& list = C.patch_cell;
+            DBuf ctr;
             list.alloc(std::max<size_t>(1, np) * sizeof(int));
+            if (G.opt.precision == LP_PREC_MIXED)
+                C.patch_prod.alloc(std::max<size_t>(1, np) * 24 * sizeof(float) * (size_t)G.max_rhs);
             ctr.alloc(sizeof(unsigned int));
             LP_HIP(hipMemsetAsync(ctr.p, 0, sizeof(unsigned int), s));
             hipLaunchKernelGGL(k_patch_assign, dim3(nblk(C.g.ncell)), dim3(kBlock), 0, s, C.patch_idx.as<int>(),
@@ -522,7 +521,7 @@ int build_hierarchy(Grid& G) {
                                G.mq64.as<double>(), C.patch_idx.as<int>(), C.patch64.as<double>(), F.g, C.g,
                                dtmp.as<double>());
             LP_HIP(hipGetLastError());
-            LP_HIP(hipStreamSynchronize(s));   // list / ctr go out of scope
+            LP_HIP(hipStreamSynchronize(s));   // ctr goes out of scope
         } else {
             C.kind = LK_EXPLICIT;
             if (need64) C.k64.alloc(576 * C.g.ncell * 8);

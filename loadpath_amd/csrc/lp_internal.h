@@ -93,6 +93,8 @@ struct Level {
     DBuf e64, e32;              // fine: cell moduli
     DBuf patch_idx;             // weights: int per cell, -1 or slot
     DBuf patch64, patch32;      // weights: (npatch, 576)
+    DBuf patch_cell;            // weights: int per patch, its cell
+    DBuf patch_prod;            // weights, mixed path: FP32 (max_rhs, npatch, 24) patch products
     int npatch = 0;
     DBuf k64, k32;              // explicit: (576, ncell)
     double lmax = 0.0;
@@ -125,7 +127,6 @@ struct Grid {
     DBuf x, r, p, ap, z, b;
     DBuf rv, zv;                // V-cycle precision copies of r and z at level 0
     DBuf vscale;                // per-rhs V-cycle input scale (mixed path)
-    DBuf gal_a, gal_b;          // fine-grid FP32 scratch of the Galerkin level-1 apply (mixed path)
     // reductions
     DBuf partials;              // doubles
     size_t partials_cap = 0;
@@ -253,14 +254,10 @@ void launch_prolong_add(const void* xc, const uint8_t* cmask, const Geom& coarse
                         void* xf, const Geom& fine, int elem_bytes, int nrhs, size_t cstride,
                         size_t fstride, hipStream_t s);
 void launch_prolong_plain(const double* c, double* f, int cx, int cy, int cz, hipStream_t s);
-// xf = prolong(xc * cm) on the fine grid (overwrites xf)
-void launch_prolong_set(const void* xc, const uint8_t* cmask, const Geom& coarse, void* xf, const Geom& fine, int eb,
-                        int nrhs, size_t cstride, size_t fstride, hipStream_t s);
-// mixed path, first coarse level applied as P^T (m K0 m) P through the fine
-// MFMA operator: prolong, masked fine apply, restrict + finish + epilogue
-void launch_apply_galerkin1(const Grid& G, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,
-                            hipStream_t s);
-bool galerkin1_path(const Grid& G, int level, int eb);
+// mixed path, weights level: the FP16-MFMA operator (patch products first)
+void launch_apply_weights_mfma(const Grid& G, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,
+                               hipStream_t s);
+bool weights_mfma_path(const Grid& G, int level, int eb);
 void launch_restrict_plain(const double* f, double* c, int cx, int cy, int cz, hipStream_t s);
 
 // ---- hierarchy.hip -------------------------------------------------------

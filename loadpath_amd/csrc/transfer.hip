@@ -143,19 +143,6 @@ void launch_prolong_add(const void* xc, const uint8_t* cmask, const Geom& coarse
     LP_HIP(hipGetLastError());
 }
 
-void launch_prolong_set(const void* xc, const uint8_t* cmask, const Geom& coarse, void* xf, const Geom& fine, int eb,
-                        int nrhs, size_t cstride, size_t fstride, hipStream_t s) {
-    TG g = make_tg(fine, coarse);
-    dim3 grid((unsigned)((g.fnn + kBlock - 1) / kBlock), nrhs);
-    if (eb == 8)
-        hipLaunchKernelGGL((k_prolong<double, false>), grid, dim3(kBlock), 0, s, (const double*)xc, cmask,
-                           (const uint8_t*)nullptr, (double*)xf, g, cstride, fstride);
-    else
-        hipLaunchKernelGGL((k_prolong<float, false>), grid, dim3(kBlock), 0, s, (const float*)xc, cmask,
-                           (const uint8_t*)nullptr, (float*)xf, g, cstride, fstride);
-    LP_HIP(hipGetLastError());
-}
-
 void launch_prolong_plain(const double* c, double* f, int cx, int cy, int cz, hipStream_t s) {
     TG g;
     g.cx = cx; g.cy = cy; g.cz = cz;

@@ -57,10 +57,11 @@ def test_mfma_operator_cancels_rigid_offset():
     assert np.abs(got - want).max() <= TOL_FP16 * np.abs(want).max()
 
 
-def test_galerkin_level1_matches_oracle():
-    """Level 1 applied as P^T (m K0 m) P through the fine MFMA operator
-    equals the oracle's stored Galerkin level (FP64) to FP16 tolerance, on
-    a grid with supports and removed cells (patch cells present)."""
+def test_weights_level_matches_oracle():
+    """The first coarse level through the FP16-MFMA weights operator (sum of
+    w_q M_q, explicit patches for cells touching held dofs) equals the
+    oracle's level (FP64) to FP16 tolerance, on a grid with supports and
+    removed cells (patch cells present)."""
     mesh, fixed, f, e = cantilever_bar(16, 10, 8, seed=13)
     rng = np.random.default_rng(14)
     e = e.copy()
