@@ -185,12 +185,72 @@ __device__ __forceinline__ void epi_finish(const EpiArgs& E, int m, size_t o, si
     }
 }
 
+// the same epilogue with the kind read at run time and each kind's loads
+// inline (one switch, no staging struct: nothing spills to scratch)
 template <typename T>
 __device__ __forceinline__ void epilogue(const EpiArgs& E, int m, size_t o, size_t nn, size_t n, const T (&yv)[3],
                                          const T (&ur)[3], double& part) {
-    EpiIn<T> in;
-    epi_load<T>(E, o, nn, n, in);
-    epi_finish<T>(E, m, o, nn, yv, ur, in, part);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const size_t i = o + c * nn;
+        const T y_ = yv[c];
+        switch (E.kind) {
+            case EPI_STORE: static_cast<T*>(E.out)[i] = y_; break;
+            case EPI_STORE_DOT:
+                static_cast<T*>(E.out)[i] = y_;
+                part += (double)ur[c] * (double)y_;
+                break;
+            case EPI_CHEB: {
+                const T di = static_cast<const T*>(E.d)[i];
+                T* X_ = static_cast<T*>(E.x);
+                T* R_ = static_cast<T*>(E.r);
+                X_[i] = X_[i] + di;
+                const T ri = R_[i] - y_;
+                R_[i] = ri;
+                const T dv = static_cast<const T*>(E.dinv)[c * nn + n];
+                static_cast<T*>(E.d2)[i] = (T)(E.c1 * (double)di + E.c2 * (double)dv * (double)ri);
+                break;
+            }
+            case EPI_RESID_SCALED: {
+                const T ri = static_cast<const T*>(E.b)[i] - y_;
+                static_cast<T*>(E.r)[i] = ri;
+                const T dv = static_cast<const T*>(E.dinv)[c * nn + n];
+                static_cast<T*>(E.d2)[i] = (T)(E.c1 * (double)dv * (double)ri);
+                break;
+            }
+            case EPI_RESID_MASK: {
+                const bool mc = (m >> c) & 1;
+                static_cast<T*>(E.out)[i] = mc ? (T)(static_cast<const T*>(E.b)[i] - y_) : T(0);
+                break;
+            }
+            case EPI_JACOBI: {
+                const T dv = static_cast<const T*>(E.dinv)[c * nn + n];
+                const T bi = static_cast<const T*>(E.b)[i];
+                static_cast<T*>(E.out)[i] = (T)((double)ur[c] + E.c1 * (double)dv * (double)(bi - y_));
+                break;
+            }
+            case EPI_POWER: {
+                const bool mc = (m >> c) & 1;
+                const T dv = static_cast<const T*>(E.dinv)[c * nn + n];
+                const T av = mc ? (T)(dv * y_) : T(0);
+                static_cast<T*>(E.out)[i] = av;
+                part += (double)ur[c] * (double)av;
+                break;
+            }
+            case EPI_RESID: {
+                const T ri = static_cast<const T*>(E.b)[i] - y_;
+                static_cast<T*>(E.out)[i] = ri;
+                part += (double)ri * (double)ri;
+                break;
+            }
+            case EPI_MASKED: {
+                const bool mc = (m >> c) & 1;
+                static_cast<T*>(E.out)[i] = mc ? y_ : T(0);
+                break;
+            }
+            default: break;
+        }
+    }
 }
 
 template <typename T>
