@@ -125,6 +125,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--size", type=int, nargs=3, default=None, help="override the grid (testing)")
     ap.add_argument("-v", "--verbose", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device copy (3 times): the known-size kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the multi-load solve shards by load case with no exchange during the "
@@ -132,6 +135,15 @@ def main():
         sys.exit(2)
     log = (lambda m: print(m, file=sys.stderr, flush=True)) if a.verbose else None
     nx, ny, nz = a.size or (WORKLOAD["nx"], WORKLOAD["ny"], WORKLOAD["nz"])
+    if a.copy_calibration:
+        import torch
+
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     mixed = run_gpu("mixed", a.steps, a.warmup, nx, ny, nz, profile=True, log=log)
     fp64 = None if a.no_fp64 else run_gpu("fp64", a.steps, a.warmup, nx, ny, nz, log=log)

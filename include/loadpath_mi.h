@@ -198,7 +198,10 @@ int lp_profile_stats(lp_grid* g, long long* launches, double* total_ms, double* 
 
 /* Launch-shape tuning (no reference counterpart: the autotuner's knob).
  * param 0: fine-level nodes per thread along y (1 or 2); 1: fine-level
- * nodes per thread along x (x-chunk, 1..64). Returns the previous value. */
+ * nodes per thread along x (x-chunk, 1..64); 2: on the FP16-MFMA levels of
+ * the mixed path, fuse the Chebyshev vector passes into the operator
+ * launches (1, the default) or run them as separate passes (0). Returns the
+ * previous value. */
 int lp_tune(lp_grid* g, int param, int value);
 
 #ifdef __cplusplus

@@ -106,6 +106,12 @@ def load_library(path: os.PathLike | None = None):
     p = Path(path) if path is not None else LIB_PATH
     if not p.exists():
         raise ImportError(f"{p} is not built: run `python -m loadpath_amd.build` (hipcc, gfx950)")
+    # torch first: its HIP runtime is then the one in the process and the
+    # library binds to it. Loaded the other way round, two HIP runtimes share
+    # the process and the library's finds no device once torch has
+    # initialised its own.
+    import torch  # noqa: F401
+
     lib = C.CDLL(str(p))
     for name, (res, args) in _SIGS.items():
         f = getattr(lib, name)

@@ -53,6 +53,9 @@ static double apply_bytes(const Grid& G, int level, int eb, int nrhs, int epi) {
     int vecs = 2;
     switch (epi) {
         case EPI_CHEB: vecs = 7; break;
+        case EPI_CHEB_FIRST: vecs = 5; break;
+        case EPI_CHEB_ONLY: vecs = 3; break;
+        case EPI_CHEB_LAST: vecs = 5; break;
         case EPI_RESID_SCALED: vecs = 5; break;
         case EPI_RESID_MASK: vecs = 3; break;
         case EPI_JACOBI: vecs = 4; break;

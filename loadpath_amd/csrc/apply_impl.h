@@ -99,9 +99,13 @@ __device__ __forceinline__ void epi_load(const EpiArgs& E, size_t o, size_t nn, 
         const size_t i = o + c * nn;
         switch (kind) {
             case EPI_CHEB:
-                in.v[0][c] = static_cast<const T*>(E.d)[i];
+            case EPI_CHEB_LAST:
                 in.v[1][c] = static_cast<const T*>(E.x)[i];
                 in.v[2][c] = static_cast<const T*>(E.r)[i];
+                in.v[3][c] = static_cast<const T*>(E.dinv)[c * nn + n];
+                break;
+            case EPI_CHEB_FIRST:
+            case EPI_CHEB_ONLY:
                 in.v[3][c] = static_cast<const T*>(E.dinv)[c * nn + n];
                 break;
             case EPI_RESID_SCALED:
@@ -139,12 +143,34 @@ __device__ __forceinline__ void epi_finish(const EpiArgs& E, int m, size_t o, si
                 static_cast<T*>(E.out)[i] = y_;
                 part += (double)ur[c] * (double)y_;
                 break;
+            // the Chebyshev kinds take d = the operand itself (ur) and, for the
+            // first step, b = in.v[0]
             case EPI_CHEB: {
-                const T di = in.v[0][c];
+                const T di = ur[c];
                 static_cast<T*>(E.x)[i] = in.v[1][c] + di;
                 const T ri = in.v[2][c] - y_;
                 static_cast<T*>(E.r)[i] = ri;
                 static_cast<T*>(E.d2)[i] = (T)(c1 * (AR)di + c2 * (AR)in.v[3][c] * (AR)ri);
+                break;
+            }
+            case EPI_CHEB_FIRST: {
+                const T di = ur[c];
+                static_cast<T*>(E.x)[i] = di;
+                const T ri = in.v[0][c] - y_;
+                static_cast<T*>(E.r)[i] = ri;
+                static_cast<T*>(E.d2)[i] = (T)(c1 * (AR)di + c2 * (AR)in.v[3][c] * (AR)ri);
+                break;
+            }
+            case EPI_CHEB_ONLY: {
+                const T di = ur[c];
+                const T dn = (T)(c1 * (AR)di + c2 * (AR)in.v[3][c] * (AR)(in.v[0][c] - y_));
+                static_cast<T*>(E.x)[i] = di + dn;
+                break;
+            }
+            case EPI_CHEB_LAST: {
+                const T di = ur[c];
+                const T dn = (T)(c1 * (AR)di + c2 * (AR)in.v[3][c] * (AR)(in.v[2][c] - y_));
+                static_cast<T*>(E.x)[i] = in.v[1][c] + di + dn;
                 break;
             }
             case EPI_RESID_SCALED: {

@@ -87,6 +87,8 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
     double part = 0.0;
     const bool skip = (E.scal != nullptr && !E.scal[rhs].active) || logical >= G.nblk;
     constexpr bool need_raw_all = EK == EPI_STORE_DOT || EK == EPI_JACOBI || EK == EPI_POWER;
+    // first Chebyshev step from zero: u is b, the operand is d = (dinv b) s
+    constexpr bool STAGE_B = EK == EPI_CHEB_FIRST || EK == EPI_CHEB_ONLY;
     if (!skip) {
         const int ntz = G.ntz;
         const int xc = logical / G.nyzb;
@@ -103,6 +105,8 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
         const __amdgpu_buffer_rsrc_t rm = make_rsrc(mask, nnu);
         const size_t ne = WTS ? (size_t)G.fnx * G.fny * G.fnz : G.ncell;
         const __amdgpu_buffer_rsrc_t re = make_rsrc(S.e, (unsigned)ne * 4u);
+        const __amdgpu_buffer_rsrc_t rdv = make_rsrc(STAGE_B ? E.dinv : nullptr, STAGE_B ? 3u * nnu * 4u : 0u);
+        const float s_b = (float)E.c3;
         const int tid = (int)threadIdx.x;
         const int lane = tid & 63;
         const int wave = tid >> 6;
@@ -137,8 +141,11 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
                 const unsigned n = (unsigned)(x * nyz + y * Nz + z);
                 const int m = bload_u8(rm, ok ? n : kOOB);
 #pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    pre[k][c] = bload<float>(ru, ((m >> c) & 1) ? (c * nnu + n) * 4u : kOOB);
+                for (int c = 0; c < 3; ++c) {
+                    const unsigned off = ((m >> c) & 1) ? (c * nnu + n) * 4u : kOOB;
+                    pre[k][c] = bload<float>(ru, off);
+                    if (STAGE_B) pre[k][c] = (bload<float>(rdv, off) * pre[k][c]) * s_b;
+                }
             }
         };
         auto store = [&](int buf) {
@@ -322,7 +329,16 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const bool mc = (em >> c) & 1;
-                    ur[c] = (!mc || need_raw_all) ? eraw[c] : q[c * PSTR];
+                    if (STAGE_B) {
+                        // d at this node (the staged operand, or computed for a
+                        // masked-out dof, whose identity row returns it); b for
+                        // the residual
+                        const float b = eraw[c];
+                        ur[c] = mc ? q[c * PSTR] : (ein.v[3][c] * b) * s_b;
+                        ein.v[0][c] = b;
+                    } else {
+                        ur[c] = (!mc || need_raw_all) ? eraw[c] : q[c * PSTR];
+                    }
                     if (!mc) yv[c] = ur[c];
                 }
                 epi_finish<float, EK, float>(E, em, (size_t)rhs * vstride + n, G.nn, yv, ur, ein, part);
@@ -338,11 +354,14 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
 }
 
 // patch cells of the weights level: (patch, row) per thread, the explicit
-// 24x24 patch matrix times the cell's masked corner vector (FP32)
+// 24x24 patch matrix times the cell's masked corner vector (FP32); with
+// dinv set, the operand is (dinv u) s, as the fused first Chebyshev step
+// stages it
 __global__ void __launch_bounds__(kBlock) k_patch_prod(const float* __restrict__ u0, size_t vstride,
                                                        const uint8_t* __restrict__ mask, Geom g,
                                                        const int* __restrict__ cells, const float* __restrict__ pm,
-                                                       int npatch, float* __restrict__ out) {
+                                                       int npatch, const float* __restrict__ dinv, float s,
+                                                       float* __restrict__ out) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int rhs = blockIdx.y;
     if (t >= npatch * 24) return;
@@ -358,7 +377,10 @@ __global__ void __launch_bounds__(kBlock) k_patch_prod(const float* __restrict__
         const int mk = mask[n];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-            if ((mk >> c) & 1) acc = fmaf(m[3 * b + c], u[c * g.nn + n], acc);
+            if ((mk >> c) & 1) {
+                const float v = u[c * g.nn + n];
+                acc = fmaf(m[3 * b + c], dinv ? (dinv[c * g.nn + n] * v) * s : v, acc);
+            }
     }
     out[((size_t)rhs * npatch + p) * 24 + row] = acc;
 }
@@ -399,6 +421,9 @@ void launch_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int n
         LP_MFMA_CASE(EPI_POWER)
         LP_MFMA_CASE(EPI_RESID)
         LP_MFMA_CASE(EPI_MASKED)
+        LP_MFMA_CASE(EPI_CHEB_FIRST)
+        LP_MFMA_CASE(EPI_CHEB_ONLY)
+        LP_MFMA_CASE(EPI_CHEB_LAST)
         default:
             set_error("level operator: unknown epilogue");
             throw ArgFailure();
@@ -410,6 +435,11 @@ void launch_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int n
 }  // namespace
 
 int mfma_blocks(const Grid& Gd, int level) { return 8 * mfma_geom(Gd, level).per; }
+
+bool mfma_level(const Grid& G, int level, int eb) {
+    return G.opt.precision == LP_PREC_MIXED && eb == 4 &&
+           (G.lv[level].kind == LK_FINE || G.lv[level].kind == LK_WEIGHTS);
+}
 
 bool weights_mfma_path(const Grid& G, int level, int eb) {
     return G.opt.precision == LP_PREC_MIXED && eb == 4 && level >= 1 && G.lv[level].kind == LK_WEIGHTS;
@@ -439,8 +469,10 @@ void launch_apply_weights_mfma(const Grid& Gd, int level, const void* u, size_t 
         }
         S.pprod = lv.patch_prod.as<float>();
         const unsigned nb = (unsigned)((lv.npatch * 24 + kBlock - 1) / kBlock);
+        const bool staged = E.kind == EPI_CHEB_FIRST || E.kind == EPI_CHEB_ONLY;
         hipLaunchKernelGGL(k_patch_prod, dim3(nb, nrhs), dim3(kBlock), 0, s, static_cast<const float*>(u), vstride,
                            lv.mask.as<uint8_t>(), lv.g, lv.patch_cell.as<int>(), lv.patch32.as<float>(), lv.npatch,
+                           staged ? static_cast<const float*>(E.dinv) : nullptr, (float)E.c3,
                            lv.patch_prod.as<float>());
         LP_HIP(hipGetLastError());
     }

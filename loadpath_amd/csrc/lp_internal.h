@@ -141,6 +141,7 @@ struct Grid {
     double ke_orbit[32] = {};
     int tune_fine_ny = 1;       // fine-level nodes per thread along y (1 or 2)
     int tune_fine_xchunk = 16;  // fine-level nodes per thread along x
+    int tune_fused_cheb = 1;    // MFMA levels: Chebyshev vector passes fused into the operator
     int vbytes() const { return opt.precision == LP_PREC_FP64 ? 8 : 4; }
     // launch profiling of one level operator (HIP events on the grid's stream)
     struct Prof {
@@ -180,6 +181,12 @@ enum EpiKind {
     EPI_POWER = 6,        // out = dinv * y * m; partial += u . out
     EPI_RESID = 7,        // out = b - y; partial += out . out (double)
     EPI_MASKED = 8,       // out = m ? y : 0 (the masked operator m K m u, no identity rows)
+    // fused Chebyshev steps (FP16-MFMA level operators only). The first step
+    // from x = 0 stages its operand d = (dinv b) s from b (u = b); the last
+    // step also adds the next direction to x, so no separate update pass:
+    EPI_CHEB_FIRST = 9,   // d = (dinv b) s: x = d; r = b - y; d2 = c1 d + c2 dinv r
+    EPI_CHEB_ONLY = 10,   // d = (dinv b) s: x = d + c1 d + c2 dinv (b - y)
+    EPI_CHEB_LAST = 11,   // x += d + c1 d + c2 dinv (r - y)
 };
 
 struct EpiArgs {
@@ -192,6 +199,7 @@ struct EpiArgs {
     void* d2 = nullptr;
     const void* dinv = nullptr;
     double c1 = 0, c2 = 0;
+    double c3 = 0;                // operand scale s of EPI_CHEB_FIRST / EPI_CHEB_ONLY
     double* partials = nullptr;   // per (rhs, block)
     const CgScal* scal = nullptr; // skip inactive rhs when set
     const double* scale_dev = nullptr;  // optional per-rhs scale read on device
@@ -258,6 +266,8 @@ void launch_prolong_plain(const double* c, double* f, int cx, int cy, int cz, hi
 void launch_apply_weights_mfma(const Grid& G, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,
                                hipStream_t s);
 bool weights_mfma_path(const Grid& G, int level, int eb);
+// the level's operator at element size eb is an FP16-MFMA kernel (fused Chebyshev kinds allowed)
+bool mfma_level(const Grid& G, int level, int eb);
 void launch_restrict_plain(const double* f, double* c, int cx, int cy, int cz, hipStream_t s);
 
 // ---- hierarchy.hip -------------------------------------------------------

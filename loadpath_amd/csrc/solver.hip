@@ -204,6 +204,39 @@ static void smooth(Grid& G, int i, int nrhs, bool from_zero) {
     const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
     const double sigma = theta / delta;
     double rho = 1.0 / sigma;
+    if (G.tune_fused_cheb && mfma_level(G, i, eb) && G.opt.degree >= 2) {
+        // the same recurrence with its vector passes fused into the operator
+        // launches: the first step stages d = (dinv b) / theta itself, the
+        // last adds the next direction to x (no start or update passes)
+        if (!from_zero) {
+            EpiArgs E;
+            E.kind = EPI_RESID_SCALED;
+            E.b = L.vb.p;
+            E.r = L.vr.p;
+            E.d2 = L.vd.p;
+            E.dinv = dinv;
+            E.c1 = 1.0 / theta;
+            launch_apply(G, i, eb, L.vx.p, n, nrhs, E, s);
+        }
+        const int napply = G.opt.degree - 1;
+        for (int it = 0; it < napply; ++it) {
+            const bool first = from_zero && it == 0, last = it + 1 == napply;
+            const double rho_new = 1.0 / (2.0 * sigma - rho);
+            EpiArgs E;
+            E.kind = first ? (last ? EPI_CHEB_ONLY : EPI_CHEB_FIRST) : (last ? EPI_CHEB_LAST : EPI_CHEB);
+            E.x = L.vx.p;
+            E.r = L.vr.p;
+            E.d2 = L.vd2.p;
+            E.dinv = dinv;
+            E.c1 = rho_new * rho;
+            E.c2 = 2.0 * rho_new / delta;
+            E.c3 = 1.0 / theta;
+            launch_apply(G, i, eb, first ? L.vb.p : L.vd.p, n, nrhs, E, s);
+            if (!last) swap_buf(L.vd, L.vd2);
+            rho = rho_new;
+        }
+        return;
+    }
     if (from_zero) {
         launch_cheb_start(L.vb.p, dinv, 1.0 / theta, L.vx.p, L.vr.p, L.vd.p, eb, n, n, nrhs, s);
     } else {
@@ -769,6 +802,12 @@ int lp_tune(lp_grid* g, int param, int value) {
         if (value < 1 || value > 64) return LP_ERR_ARG;
         const int old = g->tune_fine_xchunk;
         g->tune_fine_xchunk = value;
+        return old;
+    }
+    if (param == 2) {
+        if (value != 0 && value != 1) return LP_ERR_ARG;
+        const int old = g->tune_fused_cheb;
+        g->tune_fused_cheb = value;
         return old;
     }
     return LP_ERR_ARG;

@@ -75,6 +75,43 @@ def test_weights_level_matches_oracle():
     assert np.abs(got - want).max() <= TOL_FP16 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("degree", [2, 3])
+def test_mixed_vcycle_tracks_fp32(degree):
+    """The mixed V-cycle (FP16-MFMA operators on the two finest levels, the
+    Chebyshev steps fused into them) against the FP32 V-cycle of the same
+    hierarchy: the same preconditioner up to FP16 rounding."""
+    from loadpath_amd.mg import GridElasticity
+
+    mesh, fixed, f, e = cantilever_bar(24, 12, 12, seed=5)
+    gm = GridElasticity(mesh, 0.3, fixed, precision="mixed", coarse_cells=100, degree=degree)
+    g32 = GridElasticity(mesh, 0.3, fixed, precision="fp32", coarse_cells=100, degree=degree)
+    gm.set_moduli(e)
+    g32.set_moduli(e)
+    rng = np.random.default_rng(6)
+    r = rng.standard_normal((2,) + gm.levels[0].node_shape) * gm.level_mask(0)
+    zm = gm.precondition(r)
+    z32 = g32.precondition(r)
+    assert np.linalg.norm(zm - z32) <= 1e-2 * np.linalg.norm(z32)
+
+
+def test_fused_chebyshev_matches_separate_passes():
+    """The fused Chebyshev steps (start, update and last-step passes folded
+    into the FP16-MFMA operator launches) give the V-cycle of the separate
+    passes up to FP32 rounding."""
+    from loadpath_amd import _lib
+    from loadpath_amd.mg import GridElasticity
+
+    mesh, fixed, f, e = cantilever_bar(24, 12, 12, seed=7)
+    for degree in (2, 3):
+        g = GridElasticity(mesh, 0.3, fixed, precision="mixed", coarse_cells=100, degree=degree)
+        g.set_moduli(e)
+        r = np.random.default_rng(8).standard_normal((2,) + g.levels[0].node_shape) * g.level_mask(0)
+        z_fused = g.precondition(r)
+        assert _lib.lib().lp_tune(g._h, 2, 0) == 1
+        z_sep = g.precondition(r)
+        assert np.linalg.norm(z_fused - z_sep) <= 1e-5 * np.linalg.norm(z_sep), degree
+
+
 def test_mixed_solve_matches_direct():
     from oracle.fem import assemble, hex8_ke
 
