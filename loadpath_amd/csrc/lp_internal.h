@@ -126,7 +126,6 @@ struct Grid {
     // CG vectors (FP64), max_rhs each
     DBuf x, r, p, ap, z, b;
     DBuf rv, zv;                // V-cycle precision copies of r and z at level 0
-    DBuf vscale;                // per-rhs V-cycle input scale (mixed path)
     // reductions
     DBuf partials;              // doubles
     size_t partials_cap = 0;
@@ -234,11 +233,9 @@ void launch_cast(const void* src, int src_bytes, void* dst, int dst_bytes, size_
                  int nrhs, hipStream_t s);
 void launch_fill(void* dst, int elem_bytes, size_t n, double v, hipStream_t s);
 // scale[rhs] = max |v_rhs| (1 if zero)
-void launch_absmax(const double* v, size_t n, size_t stride, int nrhs, double* partials, double* scale,
-                   hipStream_t s);
-// dst = src / scale (invert 0) or src * scale (invert 1), per rhs
-void launch_cast_scaled(const void* src, int sb, void* dst, int db, size_t n, size_t stride, int nrhs,
-                        const double* scale, int invert, hipStream_t s);
+// z = (double) v and the partial sums of r . z (skipping inactive rhs)
+void launch_cast_dot(const void* v, int vb, const double* r, double* z, size_t n, size_t stride, int nrhs,
+                     double* partials, const CgScal* scal, hipStream_t s);
 void launch_scale_by_dev(void* v, int elem_bytes, size_t n, size_t stride, int nrhs,
                          const double* num, size_t num_stride, int mode, hipStream_t s);
 void launch_cheb_start(const void* b, const void* dinv, double s_, void* x, void* r, void* d,
@@ -247,7 +244,8 @@ void launch_axpy_inplace(void* x, const void* d, int elem_bytes, size_t n, size_
                          hipStream_t st);
 void launch_jacobi_start(const void* b, const void* dinv, double w, void* x, int elem_bytes,
                          size_t n, size_t stride, int nrhs, hipStream_t st);
-void launch_cg_step(Grid& G, int nrhs, hipStream_t s);   // x += a p; r -= a ap; partial rr
+// x += a p; r -= a ap; rv = r in the V-cycle precision (vb bytes); partial rr
+void launch_cg_step(Grid& G, int nrhs, void* rv, int vb, hipStream_t s);
 void launch_xpby(Grid& G, int nrhs, hipStream_t s);      // p = z + beta p
 // finalize kernels on CgScal
 void launch_cg_finalize(Grid& G, int nrhs, int stage, int nb, hipStream_t s);

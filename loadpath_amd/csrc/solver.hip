@@ -306,17 +306,8 @@ static void precondition(Grid& G, const double* r, double* z, int nrhs) {
     const int eb = G.vbytes();
     Level& L = G.lv[0];
     const size_t n = 3 * L.g.nn;
-    if (G.opt.precision == LP_PREC_MIXED) {
-        // the V-cycle is linear: run it on r / max|r| so the FP16 operands of
-        // the fine-level contraction stay in range, and scale z back
-        G.vscale.alloc(G.max_rhs * sizeof(double));
-        ensure_partials(G, (size_t)nrhs * dot_blocks(n));
-        launch_absmax(r, n, n, nrhs, G.partials.as<double>(), G.vscale.as<double>(), G.stream);
-        launch_cast_scaled(r, 8, L.vb.p, eb, n, n, nrhs, G.vscale.as<double>(), 0, G.stream);
-        vcycle(G, 0, nrhs);
-        launch_cast_scaled(L.vx.p, eb, z, 8, n, n, nrhs, G.vscale.as<double>(), 1, G.stream);
-        return;
-    }
+    // (the FP16-MFMA operators of the mixed path scale each cell's operand
+    // themselves, so the V-cycle input needs no range scaling)
     launch_cast(r, 8, L.vb.p, eb, n, n, nrhs, G.stream);
     vcycle(G, 0, nrhs);
     launch_cast(L.vx.p, eb, z, 8, n, n, nrhs, G.stream);
@@ -356,10 +347,15 @@ void pcg(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
         launch_fill(G.x.p, 8, n * nrhs, 0.0, s);
         launch_cast(G.b.p, 8, G.r.p, 8, n, n, nrhs, s);
     }
-    precondition(G, G.r.as<double>(), G.z.as<double>(), nrhs);
-    launch_cast(G.z.p, 8, G.p.p, 8, n, n, nrhs, s);
-    launch_dot(G.r.p, G.z.p, 8, n, n, nrhs, P, S, s);
+    // z = M r with the cast back fused into the r . z sums; later iterations
+    // get the V-cycle input from the CG step itself
+    Level& L0 = G.lv[0];
+    const int eb = G.vbytes();
+    launch_cast(G.r.p, 8, L0.vb.p, eb, n, n, nrhs, s);
+    vcycle(G, 0, nrhs);
+    launch_cast_dot(L0.vx.p, eb, G.r.as<double>(), G.z.as<double>(), n, n, nrhs, P, S, s);
     launch_cg_finalize(G, nrhs, 4, nbd, s);
+    launch_cast(G.z.p, 8, G.p.p, 8, n, n, nrhs, s);
     std::vector<CgScal> h;
     for (int it = 1; it <= G.opt.maxiter; ++it) {
         EpiArgs E;
@@ -369,14 +365,14 @@ void pcg(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
         E.scal = S;
         launch_apply(G, 0, 8, G.p.p, n, nrhs, E, s);
         launch_cg_finalize(G, nrhs, 0, nba, s);
-        launch_cg_step(G, nrhs, s);
+        launch_cg_step(G, nrhs, L0.vb.p, eb, s);
         launch_cg_finalize(G, nrhs, 1, nbd, s);
         read_scal(G, nrhs, h);
         bool any = false;
         for (int k = 0; k < nrhs; ++k) any |= h[k].active != 0;
         if (!any) break;
-        precondition(G, G.r.as<double>(), G.z.as<double>(), nrhs);
-        launch_dot(G.r.p, G.z.p, 8, n, n, nrhs, P, S, s);
+        vcycle(G, 0, nrhs);
+        launch_cast_dot(L0.vx.p, eb, G.r.as<double>(), G.z.as<double>(), n, n, nrhs, P, S, s);
         launch_cg_finalize(G, nrhs, 2, nbd, s);
         launch_xpby(G, nrhs, s);
     }

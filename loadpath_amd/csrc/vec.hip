@@ -111,9 +111,10 @@ __global__ void k_jacobi_start(const T* __restrict__ b, const T* __restrict__ di
 }
 
 // x += alpha p; r -= alpha ap; partial |r|^2   (mgops._cg_step, mgops.py:346-354)
+template <typename V>
 __global__ void k_cg_step(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ p,
                           const double* __restrict__ ap, size_t n, size_t stride, const CgScal* scal,
-                          double* partials) {
+                          double* partials, V* __restrict__ rv) {
     __shared__ double sred[4];
     const int rhs = blockIdx.y;
     double acc = 0;
@@ -124,6 +125,7 @@ __global__ void k_cg_step(double* __restrict__ x, double* __restrict__ r, const 
             x[k] += a * p[k];
             const double ri = r[k] - a * ap[k];
             r[k] = ri;
+            rv[k] = (V)ri;
             acc += ri * ri;
         }
     }
@@ -240,44 +242,22 @@ __global__ void k_warm_norm(const double* __restrict__ b, const double* __restri
     if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = t;
 }
 
-// per-block max |v| (the V-cycle input scale of the mixed path)
-__global__ void k_absmax(const double* __restrict__ v, size_t n, size_t stride, double* partials) {
-    __shared__ double s[256];
+// z = (double) v (the V-cycle's result); partial r . z for active rhs
+template <typename S>
+__global__ void k_cast_dot(const S* __restrict__ v, const double* __restrict__ r, double* __restrict__ z, size_t n,
+                           size_t stride, double* partials, const CgScal* scal) {
+    __shared__ double sred[4];
     const int rhs = blockIdx.y;
-    double m = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        m = fmax(m, fabs(v[rhs * stride + i]));
-    s[threadIdx.x] = m;
-    __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + o]);
-        __syncthreads();
+    const bool on = !(scal && !scal[rhs].active);
+    double acc = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = rhs * stride + i;
+        const double zi = (double)v[k];
+        z[k] = zi;
+        if (on) acc += r[k] * zi;
     }
-    if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = s[0];
-}
-
-__global__ void k_absmax_final(const double* __restrict__ partials, int nb, double* scale) {
-    __shared__ double s[256];
-    const int rhs = blockIdx.x;
-    double m = 0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) m = fmax(m, partials[(size_t)rhs * nb + i]);
-    s[threadIdx.x] = m;
-    __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + o]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) scale[rhs] = s[0] > 0 ? s[0] : 1.0;
-}
-
-// dst = src / scale[rhs] (invert = 0) or src * scale[rhs] (invert = 1)
-template <typename S, typename D>
-__global__ void k_cast_scaled(const S* __restrict__ src, D* __restrict__ dst, size_t n, size_t stride,
-                              const double* __restrict__ scale, int invert) {
-    const int rhs = blockIdx.y;
-    const double f = invert ? scale[rhs] : 1.0 / scale[rhs];
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        dst[rhs * stride + i] = (D)((double)src[rhs * stride + i] * f);
+    const double t = block_sum(acc, sred);
+    if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = t;
 }
 
 inline int grid_for(size_t n) {
@@ -333,27 +313,15 @@ void launch_cast(const void* src, int sb, void* dst, int db, size_t n, size_t st
     LP_HIP(hipGetLastError());
 }
 
-void launch_absmax(const double* v, size_t n, size_t stride, int nrhs, double* partials, double* scale,
-                   hipStream_t s) {
-    const int nb = dot_blocks(n);
-    hipLaunchKernelGGL(k_absmax, dim3(nb, nrhs), dim3(256), 0, s, v, n, stride, partials);
-    hipLaunchKernelGGL(k_absmax_final, dim3(nrhs), dim3(256), 0, s, partials, nb, scale);
-    LP_HIP(hipGetLastError());
-}
-
-void launch_cast_scaled(const void* src, int sb, void* dst, int db, size_t n, size_t stride, int nrhs,
-                        const double* scale, int invert, hipStream_t s) {
-    dim3 grid(grid_for(n), nrhs);
-    if (sb == 8 && db == 4)
-        hipLaunchKernelGGL((k_cast_scaled<double, float>), grid, dim3(kBlock), 0, s, (const double*)src, (float*)dst, n,
-                           stride, scale, invert);
-    else if (sb == 4 && db == 8)
-        hipLaunchKernelGGL((k_cast_scaled<float, double>), grid, dim3(kBlock), 0, s, (const float*)src, (double*)dst, n,
-                           stride, scale, invert);
-    else {
-        set_error("cast_scaled: unsupported types");
-        throw ArgFailure();
-    }
+void launch_cast_dot(const void* v, int vb, const double* r, double* z, size_t n, size_t stride, int nrhs,
+                     double* partials, const CgScal* scal, hipStream_t s) {
+    dim3 grid(dot_blocks(n), nrhs);
+    if (vb == 4)
+        hipLaunchKernelGGL(k_cast_dot<float>, grid, dim3(kBlock), 0, s, (const float*)v, r, z, n, stride, partials,
+                           scal);
+    else
+        hipLaunchKernelGGL(k_cast_dot<double>, grid, dim3(kBlock), 0, s, (const double*)v, r, z, n, stride, partials,
+                           scal);
     LP_HIP(hipGetLastError());
 }
 
@@ -405,11 +373,17 @@ void launch_jacobi_start(const void* b, const void* dinv, double w, void* x, int
     LP_HIP(hipGetLastError());
 }
 
-void launch_cg_step(Grid& G, int nrhs, hipStream_t s) {
+void launch_cg_step(Grid& G, int nrhs, void* rv, int vb, hipStream_t s) {
     const size_t n = 3 * G.g0.nn;
     const int nb = dot_blocks(n);
-    hipLaunchKernelGGL(k_cg_step, dim3(nb, nrhs), dim3(kBlock), 0, s, G.x.as<double>(), G.r.as<double>(),
-                       G.p.as<double>(), G.ap.as<double>(), n, n, G.scal.as<CgScal>(), G.partials.as<double>());
+    if (vb == 4)
+        hipLaunchKernelGGL(k_cg_step<float>, dim3(nb, nrhs), dim3(kBlock), 0, s, G.x.as<double>(), G.r.as<double>(),
+                           G.p.as<double>(), G.ap.as<double>(), n, n, G.scal.as<CgScal>(), G.partials.as<double>(),
+                           static_cast<float*>(rv));
+    else
+        hipLaunchKernelGGL(k_cg_step<double>, dim3(nb, nrhs), dim3(kBlock), 0, s, G.x.as<double>(), G.r.as<double>(),
+                           G.p.as<double>(), G.ap.as<double>(), n, n, G.scal.as<CgScal>(), G.partials.as<double>(),
+                           static_cast<double*>(rv));
     LP_HIP(hipGetLastError());
 }
 
