@@ -30,6 +30,11 @@ for _v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
     os.environ[_v] = str(_CORES)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the fine-level FP16-MFMA operator (8 right-hand
+# sides, mean over the 52 launches of one mixed SIMP iteration): FETCH_SIZE x 2
+# + WRITE_SIZE x 1, the factors from a 1 GiB copy in the same run; from
+# profiles/counters_multiload_256x128x128.txt
+FINE_MFMA_TRAFFIC_BYTES = 2.0591e9
 WORKLOAD = dict(nx=256, ny=128, nz=128, cases=8)
 
 
@@ -156,8 +161,9 @@ def main():
     if prof and prof["launches"]:
         gbs = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9
         roof = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
-                    traffic=None, kernel="k_apply_fine_mfma (V-cycle fine-level operator, FP16 MFMA contraction, "
-                                         "FP32 smoother epilogue)",
+                    traffic=FINE_MFMA_TRAFFIC_BYTES if (nx, ny, nz) == (WORKLOAD["nx"], WORKLOAD["ny"], WORKLOAD["nz"])
+                    else None, kernel="k_apply_mfma<K, false> (V-cycle fine-level operator: FP16 MFMA contraction, "
+                                         "fused Chebyshev / residual epilogues in FP32)",
                     launches=prof["launches"], avg_us=round(prof["ms"] * 1e3 / prof["launches"], 2),
                     avg_bytes=round(prof["bytes"] / prof["launches"]))
     cpu = None if a.no_cpu else cpu_baseline(mixed["timed"], log)
