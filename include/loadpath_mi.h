@@ -171,6 +171,13 @@ int lp_simp_iterate(lp_simp* s, int n, lp_simp_stats* stats_out);
 /* what: 0 x, 1 xphys, 2 dc (filtered), 3 ce (summed over cases): (nx,ny,nz) doubles */
 int lp_simp_get(lp_simp* s, int what, double* out);
 int lp_simp_set_x(lp_simp* s, const double* x);
+/* The design update of lp_simp_iterate: 0 optimality criteria with the
+ * volume bisection (reference.py:240-251, the default); 1 MMA (mma.py) on
+ * 10 c / c_first with the volume constraint sum xphys / (vf n) - 1 <= 0, both
+ * gradients through the density filter (the normalization of loadpath's MMA
+ * loops on compliance-type objectives, thermal.py:1035-1043). Boxes without
+ * passive cells. */
+int lp_simp_set_optimizer(lp_simp* s, int optimizer);
 /* displacements of the last solve, (nrhs, 3, nodes) doubles */
 int lp_simp_displacements(lp_simp* s, double* out);
 
@@ -195,6 +202,23 @@ int lp_time_apply(lp_grid* g, int level, int dtype_bytes, int nrhs, int reps, do
  * launches were dropped past the event pool. */
 int lp_profile(lp_grid* g, int level, int dtype_bytes, int enable);
 int lp_profile_stats(lp_grid* g, long long* launches, double* total_ms, double* total_bytes);
+
+/* The method of moving asymptotes, mma.MMA (mma.py:31-214): n design
+ * variables, 1 <= m <= 8 constraints; xmin, xmax host arrays (n). update
+ * takes the design x, the objective's value and gradient (f0, df0 (n)), the
+ * constraint values f (m, host) and gradients df (m, n), and writes the new
+ * design; x, df0, df and xnew are device pointers when device_ptr is set.
+ * tighten pulls the asymptotes toward the last design (mma.py:52-58).
+ * get: what 0 the lower asymptotes (n), 1 the upper (n), 2 the subproblem's
+ * multipliers lambda (m), 3 the iteration count (1 value). */
+typedef struct lp_mma lp_mma;
+int lp_mma_create(lp_mma** out, long long n, int m, const double* xmin, const double* xmax, double move, double c,
+                  double asyinit, double asyincr, double asydecr);
+int lp_mma_destroy(lp_mma* mma);
+int lp_mma_tighten(lp_mma* mma, double factor);
+int lp_mma_update(lp_mma* mma, const double* x, double f0, const double* df0, const double* f, const double* df,
+                  double* xnew, int device_ptr);
+int lp_mma_get(lp_mma* mma, int what, double* out);
 
 /* Launch-shape tuning (no reference counterpart: the autotuner's knob).
  * param 0: fine-level nodes per thread along y (1 or 2); 1: fine-level

@@ -145,6 +145,22 @@ __global__ void k_filter(const double* __restrict__ v, const double* __restrict_
     }
 }
 
+// MMA inputs: df0 = s dc, dcon = t dv, and block partials of sum xphys
+__global__ void k_mma_inputs(const double* __restrict__ dc, const double* __restrict__ dv,
+                             const double* __restrict__ xphys, size_t n, double s, double t, double* df0,
+                             double* dcon, double* partials) {
+    __shared__ double sred[4];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double part = 0.0;
+    if (i < n) {
+        df0[i] = s * dc[i];
+        dcon[i] = t * dv[i];
+        part = xphys[i];
+    }
+    const double r = bsum(part, sred);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
 // xnew = max(0, max(x - move, min(1, min(x + move, x sqrt(-dc/dv/lmid)))))
 __global__ void k_oc(const double* __restrict__ x, const double* __restrict__ dc, const double* __restrict__ dv,
                      const uint8_t* __restrict__ passive, size_t n, double move, Bisect* bis, double* xnew) {
@@ -238,6 +254,14 @@ struct lp_simp {
     double target = 0;
     DBuf taps, x, xnew, xphys, dc, dv, ce, hs_inv, ones, passive, loads, bis, partials, scalar, chg;
     bool has_passive = false;
+    // optimizer: 0 optimality criteria with the volume bisection, 1 MMA
+    int optimizer = 0;
+    lp_mma* mma = nullptr;
+    double c_ref = 0.0;
+    DBuf df0, dcon;
+    ~lp_simp() {
+        if (mma) lp_mma_destroy(mma);
+    }
 };
 
 extern "C" {
@@ -378,6 +402,50 @@ int lp_simp_iterate(lp_simp* S, int niter, lp_simp_stats* stats) {
                                (const double*)nullptr, (const uint8_t*)nullptr, S->taps.as<Tap>(), S->ntaps, g0.nx, g0.ny,
                                g0.nz, S->dv.as<double>(), (double*)nullptr, (const Bisect*)nullptr);
             LP_HIP(hipGetLastError());
+            if (S->optimizer == 1) {
+                // MMA on f0 = 10 c / c_ref with the volume constraint
+                // sum xphys / (vf n) - 1 <= 0, both gradients through the
+                // filter (the normalization of loadpath's MMA loops on
+                // compliance-type objectives, thermal.py:1035-1043)
+                double c = 0.0;
+                LP_HIP(hipMemcpyAsync(&c, scal, 8, hipMemcpyDeviceToHost, s));
+                LP_HIP(hipStreamSynchronize(s));
+                if (S->c_ref == 0.0) S->c_ref = std::fabs(c) > 0 ? std::fabs(c) : 1.0;
+                hipLaunchKernelGGL(k_mma_inputs, dim3(nb), dim3(kBlock), 0, s, S->dc.as<double>(), S->dv.as<double>(),
+                                   S->xphys.as<double>(), n, 10.0 / S->c_ref, 1.0 / (S->volfrac * (double)n),
+                                   S->df0.as<double>(), S->dcon.as<double>(), S->partials.as<double>());
+                hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kBlock), 0, s, S->partials.as<double>(), (int)nb, scal + 1);
+                LP_HIP(hipGetLastError());
+                double vol = 0.0;
+                LP_HIP(hipMemcpyAsync(&vol, scal + 1, 8, hipMemcpyDeviceToHost, s));
+                LP_HIP(hipStreamSynchronize(s));
+                const double con = vol / (S->volfrac * (double)n) - 1.0;
+                const int rc = lp_mma_update(S->mma, S->x.as<double>(), 10.0 * c / S->c_ref, S->df0.as<double>(), &con,
+                                             S->dcon.as<double>(), S->xnew.as<double>(), 1);
+                if (rc != LP_OK) return rc;
+                hipLaunchKernelGGL(k_filter, dim3(nb), dim3(kBlock), 0, s, S->xnew.as<double>(), (const double*)nullptr,
+                                   S->hs_inv.as<double>(), (const uint8_t*)nullptr, S->taps.as<Tap>(), S->ntaps, g0.nx,
+                                   g0.ny, g0.nz, S->xphys.as<double>(), (double*)nullptr, (const Bisect*)nullptr);
+                LP_HIP(hipGetLastError());
+                LP_HIP(hipMemsetAsync(S->chg.p, 0, 8, s));
+                hipLaunchKernelGGL(k_change, dim3(std::min(nb, 1024u)), dim3(kBlock), 0, s, S->xnew.as<double>(),
+                                   S->x.as<double>(), n, S->chg.as<unsigned long long>());
+                LP_HIP(hipGetLastError());
+                std::swap(S->x.p, S->xnew.p);
+                double chg = 0.0;
+                LP_HIP(hipMemcpyAsync(&chg, S->chg.p, 8, hipMemcpyDeviceToHost, s));
+                LP_HIP(hipStreamSynchronize(s));
+                lp_simp_stats& st = stats[it];
+                st.compliance = c;
+                st.change = chg;
+                st.volume = vol / (double)n;
+                st.cg_iters = *std::max_element(iters.begin(), iters.end());
+                st.cg_iters_total = 0;
+                for (int k = 0; k < S->nrhs; ++k) st.cg_iters_total += iters[k];
+                st.oc_evals = 0;
+                st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                continue;
+            }
             // bisection on the device
             Bisect b0{0.0, 1e9, 0.0, S->target, 0, 0, 0, 0};
             LP_HIP(hipMemcpyAsync(S->bis.p, &b0, sizeof(b0), hipMemcpyHostToDevice, s));
@@ -417,6 +485,27 @@ int lp_simp_iterate(lp_simp* S, int niter, lp_simp_stats* stats) {
                 set_error("OC bisection did not close within 100 evaluations");
                 return LP_ERR_STATE;
             }
+        }
+        return LP_OK;
+    });
+}
+
+int lp_simp_set_optimizer(lp_simp* S, int optimizer) {
+    return guarded_s([&]() -> int {
+        if (!S || (optimizer != 0 && optimizer != 1)) return LP_ERR_ARG;
+        if (optimizer == 1 && S->has_passive) {
+            set_error("lp_simp_set_optimizer: MMA runs on boxes without passive cells");
+            return LP_ERR_ARG;
+        }
+        S->optimizer = optimizer;
+        if (optimizer == 1 && !S->mma) {
+            const size_t n = S->ncell;
+            std::vector<double> lo(n, 0.0), hi(n, 1.0);
+            const int rc = lp_mma_create(&S->mma, (long long)n, 1, lo.data(), hi.data(), S->move, 1000.0, 0.5, 1.2, 0.7);
+            if (rc != LP_OK) return rc;
+            S->df0.alloc(n * 8);
+            S->dcon.alloc(n * 8);
+            S->c_ref = 0.0;
         }
         return LP_OK;
     });

@@ -70,7 +70,7 @@ class GridSimp:
 
     def __init__(self, problem, *, precision="fp32", smoother="chebyshev", omega=0.6, degree=2, rtol=1e-8,
                  maxiter=500, coarse_cells=2000, max_levels=0, element=None, nu=0.3, penal=3.0, emin=1e-9,
-                 move=0.2, filter="top3d", warm_start=True):
+                 move=0.2, filter="top3d", warm_start=True, optimizer="oc"):
         from .mg import GridElasticity
         from .fem import box_mesh
 
@@ -97,6 +97,10 @@ class GridSimp:
                                              int(warm_start)), "lp_simp_create")
         self._h = h
         self.iterations = 0
+        if optimizer not in ("oc", "mma"):
+            raise ValueError(f"optimizer must be 'oc' or 'mma', not {optimizer!r}")
+        self.optimizer = optimizer
+        _lib.check(_lib.lib().lp_simp_set_optimizer(h, 0 if optimizer == "oc" else 1), "simp_set_optimizer")
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -116,7 +120,7 @@ class GridSimp:
         out = []
         for k in range(n):
             s = stats[k]
-            out.append(dict(compliance=s.compliance, change=s.change, cg_iters=s.cg_iters,
+            out.append(dict(compliance=s.compliance, change=s.change, volume=s.volume, cg_iters=s.cg_iters,
                             cg_iters_total=s.cg_iters_total, oc_evals=s.oc_evals, seconds=s.seconds))
         return out
 

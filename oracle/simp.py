@@ -232,6 +232,43 @@ def oc_bisect(x, dc, dv, h, hs, volfrac, move=0.2, passive=None, l1=0.0, l2=1e9,
     return xnew, xphys, evals
 
 
+def optimize_mma(p: Problem, iterations, solve=None, penal=3.0, move=0.2, element="im", filt=None):
+    """Compliance with the volume constraint under MMA: f0 = 10 c / c_first,
+    con = sum xphys / (vf n) - 1, both gradients through the filter (the
+    normalization of loadpath's MMA loops on compliance-type objectives,
+    thermal.py:1035-1043), xphys = H x / hs. Returns the history of
+    (compliance, volume, change) and the final (x, xphys)."""
+    from .mma import MMA
+
+    ev = Evaluator(p, element)
+    if solve is None:
+        solve = lambda e: [ev.solve(e, f) for f in ev.fs]
+    h, hs = filt if filt is not None else top3d_filter(p.dims, p.rmin)
+    n = len(hs)
+    x = np.full(n, p.volfrac)
+    xphys = x.copy()
+    mma = MMA(n, 1, move=move)
+    c_ref = None
+    hist = []
+    for _ in range(iterations):
+        e = simp_modulus(xphys, penal)
+        us = solve(e)
+        ce = sum(element_energy(u, ev.edof, ev.ke) for u in us)
+        c = float(np.sum(e * ce))
+        dc = h @ (compliance_sensitivity(xphys, ce, penal) / hs)
+        dv = h @ (np.ones(n) / hs)
+        if c_ref is None:
+            c_ref = abs(c) or 1.0
+        vol = float(xphys.sum())
+        con = vol / (p.volfrac * n) - 1.0
+        xnew = mma.update(x, 10.0 * c / c_ref, 10.0 * dc / c_ref, np.array([con]), (dv / (p.volfrac * n))[None])
+        xphys = (h @ xnew) / hs
+        change = float(np.abs(xnew - x).max())
+        x = xnew
+        hist.append((c, vol / n, change))
+    return hist, x, xphys
+
+
 def optimize_top3d(p: Problem, solve=None, maxit=200, iterations=None, penal=3.0, move=0.2,
                    element="im", on_iter=None, filt=None):
     """The top3d loop (benchmarks/reference.py:214-257) for the 3-D case.

@@ -56,3 +56,23 @@ def test_top3d_golden_cantilever(gpu):
     # metric never drops below 0.01 here): the field agrees on average
     assert np.abs(r["x"] - z["x"]).mean() < 3e-2
     assert ((r["x"] > 0.5) != (z["x"] > 0.5)).mean() < 0.03
+
+
+def test_mma_iterations_match_oracle(gpu):
+    """The device loop with the MMA update (lp_simp_set_optimizer) against
+    the oracle's MMA loop (direct solves, numpy MMA): the same compliance,
+    volume and design over the first iterations."""
+    from loadpath_amd.problems import cantilever_3d
+    from loadpath_amd.simp import GridSimp
+    from oracle import simp as os_
+
+    p = cantilever_3d(16, 4, 8)
+    s = GridSimp(p, element="plain", rtol=1e-12, maxiter=3000, coarse_cells=50, optimizer="mma")
+    hist_o, xo, xpo = os_.optimize_mma(_oracle_problem(p), iterations=4, element="plain")
+    hist = s.iterate(4)
+    for (co, vo, cho), st in zip(hist_o, hist):
+        assert st["compliance"] == pytest.approx(co, rel=1e-8)
+        assert st["volume"] == pytest.approx(vo, rel=1e-10)
+        assert st["change"] == pytest.approx(cho, rel=1e-6, abs=1e-9)
+    assert np.allclose(s.x.ravel(), xo, rtol=0, atol=1e-7)
+    assert np.allclose(s.xphys.ravel(), xpo, rtol=0, atol=1e-7)
