@@ -121,6 +121,39 @@ def cpu_baseline(gpu_timed, log=None):
                        f"by cells and by the {cg_total:.0f} CG iterations per SIMP iteration of the 8 cases")
 
 
+def run_studies_bench(log=None, max_iterations=40):
+    """The 32 uneven studies (problems.batched_studies, default_rng(5)):
+    all at once on their own streams (mixed path, graph-replayed V-cycles,
+    autotuned fine operator) against the same 32 run one after another on
+    the plain FP64 path (no graphs, default launch shape). Each study runs
+    to change <= 0.01 or max_iterations SIMP iterations."""
+    from loadpath_amd.batch import run_studies
+    from loadpath_amd.problems import batched_studies
+
+    ps = batched_studies()
+    run_studies(ps[:2], precision="mixed", max_iterations=1)   # first-use costs out of the timing
+    b = run_studies(ps, precision="mixed", max_iterations=max_iterations, concurrent=True)
+    if log:
+        log(f"  batch: {b['wall_s']:.2f} s, p50 {b['p50_s']:.2f} s, p99 {b['p99_s']:.2f} s")
+    q = run_studies(ps, precision="fp64", max_iterations=max_iterations, concurrent=False, graph=False,
+                    autotune=False)
+    if log:
+        log(f"  sequential fp64: {q['wall_s']:.2f} s")
+    return {
+        "metric": "studies/s, 32 uneven SIMP studies (nx 32-96, ny/nz 16-48) in one batch",
+        "value": round(b["studies_per_s"], 4), "unit": "studies/s", "n_gpus": 1, "higher_is_better": True,
+        "p50_s": round(b["p50_s"], 3), "p99_s": round(b["p99_s"], 3), "wall_s": round(b["wall_s"], 3),
+        "sequential_fp64_studies_per_s": round(q["studies_per_s"], 4),
+        "sequential_fp64_p50_s": round(q["p50_s"], 3), "sequential_fp64_p99_s": round(q["p99_s"], 3),
+        "speedup_vs_sequential_fp64": round(q["wall_s"] / b["wall_s"], 3),
+        "iterations": [s["iterations"] for s in b["studies"]],
+        "config": {"workload": "batched_studies(count=32, seed=5): cantilever / half-MBB alternating, point or "
+                               "4x4-node patch load in a seeded direction, vf U(0.1, 0.4), rmin 1.5 / 2.0, top3d "
+                               "filter + OC, stop at change <= 0.01 or %d iterations" % max_iterations},
+        "data": "synthetic",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +163,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--size", type=int, nargs=3, default=None, help="override the grid (testing)")
     ap.add_argument("-v", "--verbose", action="store_true")
+    ap.add_argument("--studies", action="store_true",
+                    help="instead: the 32 uneven batched studies (concurrent, mixed, graphs, autotuned) against "
+                         "the same studies one after another on the plain FP64 path")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (3 times): the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -139,6 +175,9 @@ def main():
               "solve; runs on more than one GPU are not measured by this script.", file=sys.stderr)
         sys.exit(2)
     log = (lambda m: print(m, file=sys.stderr, flush=True)) if a.verbose else None
+    if a.studies:
+        print(json.dumps(run_studies_bench(log)))
+        return
     nx, ny, nz = a.size or (WORKLOAD["nx"], WORKLOAD["ny"], WORKLOAD["nz"])
     if a.copy_calibration:
         import torch

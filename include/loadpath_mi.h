@@ -178,6 +178,11 @@ int lp_simp_set_x(lp_simp* s, const double* x);
  * loops on compliance-type objectives, thermal.py:1035-1043). Boxes without
  * passive cells. */
 int lp_simp_set_optimizer(lp_simp* s, int optimizer);
+
+/* Heaviside projection simp._project / _dproject (simp.py:1443-1450):
+ * out = (tanh(b e) + tanh(b (x - e))) / (tanh(b e) + tanh(b (1 - e))), dout
+ * its derivative in x; either output may be NULL. */
+int lp_project(long long n, const double* xt, double beta, double eta, double* out, double* dout, int device_ptr);
 /* displacements of the last solve, (nrhs, 3, nodes) doubles */
 int lp_simp_displacements(lp_simp* s, double* out);
 
@@ -220,12 +225,27 @@ int lp_mma_update(lp_mma* mma, const double* x, double f0, const double* df0, co
                   double* xnew, int device_ptr);
 int lp_mma_get(lp_mma* mma, int what, double* out);
 
+/* The autotuner (no reference counterpart): times the fine-level operator
+ * of the V-cycle (FP16 MFMA on the mixed path, the vector kernel otherwise)
+ * on nrhs right-hand sides for each x-chunk in {4, 8, 16, 32}, keeps the
+ * fastest and returns it with its time per apply. */
+int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best);
+
 /* Launch-shape tuning (no reference counterpart: the autotuner's knob).
  * param 0: fine-level nodes per thread along y (1 or 2); 1: fine-level
  * nodes per thread along x (x-chunk, 1..64); 2: on the FP16-MFMA levels of
  * the mixed path, fuse the Chebyshev vector passes into the operator
- * launches (1, the default) or run them as separate passes (0). Returns the
- * previous value. */
+ * launches (1, the default) or run them as separate passes (0); 3: replay
+ * the PCG's V-cycle from a captured hipGraph (1, the default with the
+ * Chebyshev smoother) or launch it kernel by kernel (0); 4: the CG's beta,
+ * flexible Polak-Ribiere (1), the reference's Fletcher-Reeves (0) or
+ * flexible on the mixed path only (-1, the default): the FP16-rounded
+ * V-cycle is not exactly symmetric; 5: on the mixed path, repeat a solve
+ * that breaks down or reaches the cap with FP32 operators on every level
+ * until the next hierarchy build (1, the default), do not (0), or force
+ * the repeat after one iteration (2, for tests). Returns the previous
+ * value. Param 6 returns the number of such repeats so far (value
+ * ignored). */
 int lp_tune(lp_grid* g, int param, int value);
 
 #ifdef __cplusplus

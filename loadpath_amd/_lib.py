@@ -86,12 +86,14 @@ _SIGS = {
     "lp_simp_get": (_I, [_P, _I, _P]),
     "lp_simp_set_x": (_I, [_P, _P]),
     "lp_simp_set_optimizer": (_I, [_P, _I]),
+    "lp_project": (_I, [C.c_longlong, _P, _D, _D, _P, _P, _I]),
     "lp_simp_displacements": (_I, [_P, _P]),
     "lp_filter_apply": (_I, [_I, _I, _I, _D, _I, _P, _P, _I]),
     "lp_last_error": (C.c_char_p, []),
     "lp_version": (_I, []),
     "lp_time_apply": (_I, [_P, _I, _I, _I, _I, _P]),
     "lp_tune": (_I, [_P, _I, _I]),
+    "lp_autotune": (_I, [_P, _I, _I, _P, _P]),
     "lp_profile": (_I, [_P, _I, _I, _I]),
     "lp_profile_stats": (_I, [_P, _P, _P, _P]),
     "lp_mma_create": (_I, [C.POINTER(_P), C.c_longlong, _I, _P, _P, _D, _D, _D, _D, _D]),

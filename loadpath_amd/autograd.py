@@ -18,7 +18,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-__all__ = ["compliance", "stress_pnorm", "Compliance", "StressPNormObjective"]
+__all__ = ["compliance", "stress_pnorm", "project", "Compliance", "StressPNormObjective", "Projection"]
 
 
 def _np(x: torch.Tensor) -> np.ndarray:
@@ -79,3 +79,34 @@ def compliance(x: torch.Tensor, grid, f, penal: float = 3.0, emin: float = 1e-9)
 
 def stress_pnorm(x: torch.Tensor, agg, f, penal: float = 3.0, emin: float = 1e-9, kappa: float = 1.0):
     return StressPNormObjective.apply(x, agg, np.asarray(f, float), float(penal), float(emin), float(kappa))
+
+
+class Projection(torch.autograd.Function):
+    """Heaviside projection (simp.py:1443-1450) with its analytic derivative,
+    on the GPU (lp_project): float64 tensors, CUDA (device pointers) or CPU."""
+
+    @staticmethod
+    def forward(ctx, xt, beta, eta):
+        import ctypes as C
+
+        from . import _lib
+
+        x = xt.detach().to(torch.float64).contiguous()
+        out = torch.empty_like(x)
+        dout = torch.empty_like(x)
+        dev = 1 if x.is_cuda else 0
+        if dev:
+            torch.cuda.synchronize()
+        _lib.check(_lib.lib().lp_project(x.numel(), C.c_void_p(x.data_ptr()), float(beta), float(eta),
+                                         C.c_void_p(out.data_ptr()), C.c_void_p(dout.data_ptr()), dev), "project")
+        ctx.save_for_backward(dout)
+        return out.to(xt.dtype)
+
+    @staticmethod
+    def backward(ctx, gout):
+        (dout,) = ctx.saved_tensors
+        return gout * dout.to(gout.dtype), None, None
+
+
+def project(xt: torch.Tensor, beta: float, eta: float = 0.5) -> torch.Tensor:
+    return Projection.apply(xt, float(beta), float(eta))

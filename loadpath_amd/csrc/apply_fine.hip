@@ -41,7 +41,7 @@ AGeom make_apply_geom(const Grid& Gd, int level) {
 
 int apply_blocks(const Grid& Gd, int level, int eb) {
     if (weights_mfma_path(Gd, level, eb)) return mfma_blocks(Gd, level);
-    if (Gd.lv[level].kind == LK_FINE && eb == 4 && Gd.opt.precision == LP_PREC_MIXED) return mfma_blocks(Gd, level);
+    if (Gd.lv[level].kind == LK_FINE && eb == 4 && Gd.fp16()) return mfma_blocks(Gd, level);
     const AGeom G = make_apply_geom(Gd, level);
     return Gd.lv[level].kind == LK_FINE ? 8 * G.per : G.nblk;
 }
@@ -82,7 +82,7 @@ int launch_apply(const Grid& G, int level, int eb, const void* u, size_t vstride
     }
     switch (G.lv[level].kind) {
         case LK_FINE:
-            if (eb == 4 && G.opt.precision == LP_PREC_MIXED)
+            if (eb == 4 && G.fp16())
                 launch_apply_fine_mfma(G, level, u, vstride, nrhs, E, s);
             else
                 launch_apply_fine(G, level, eb, u, vstride, nrhs, E, s);

@@ -196,6 +196,12 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
                 dmax = fmaxf(dmax, __shfl_xor(dmax, 32));
                 int ex;
                 (void)frexpf(dmax, &ex);                      // dmax = f 2^ex, f in [0.5, 1)
+                // clamp so the scale stays finite in FP32 (2^115 at most):
+                // an infinite scale would turn the exact zero of corner 0
+                // into NaN. Below the clamp a cell's largest difference
+                // still scales to at least 2^-14 (FP16's smallest normal)
+                // for every normal FP32 dmax (>= 2^-126)
+                ex = max(ex, -100);
                 const float sc = ldexpf(1.0f, 15 - ex);       // |d| sc < 2^15 < FP16 max
                 half4 B[3];
 #pragma unroll
@@ -437,12 +443,11 @@ void launch_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int n
 int mfma_blocks(const Grid& Gd, int level) { return 8 * mfma_geom(Gd, level).per; }
 
 bool mfma_level(const Grid& G, int level, int eb) {
-    return G.opt.precision == LP_PREC_MIXED && eb == 4 &&
-           (G.lv[level].kind == LK_FINE || G.lv[level].kind == LK_WEIGHTS);
+    return G.fp16() && eb == 4 && (G.lv[level].kind == LK_FINE || G.lv[level].kind == LK_WEIGHTS);
 }
 
 bool weights_mfma_path(const Grid& G, int level, int eb) {
-    return G.opt.precision == LP_PREC_MIXED && eb == 4 && level >= 1 && G.lv[level].kind == LK_WEIGHTS;
+    return G.fp16() && eb == 4 && level >= 1 && G.lv[level].kind == LK_WEIGHTS;
 }
 
 void launch_apply_fine_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,

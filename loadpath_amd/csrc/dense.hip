@@ -160,9 +160,13 @@ __global__ void k_matvec(const double* __restrict__ ainv, int n, const T* __rest
 
 }  // namespace
 
-bool dense_spd_inverse(double* a, int n, double* /*work*/, hipStream_t s) {
+bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s) {
     const int nblk = (n + NB - 1) / NB;
-    DBuf dinv, fail, panel, linv, t;
+    DBuf& dinv = w.dn_dinv;
+    DBuf& fail = w.dn_fail;
+    DBuf& panel = w.dn_panel;
+    DBuf& linv = w.dn_linv;
+    DBuf& t = w.dn_t;
     dinv.alloc((size_t)nblk * NB * NB * 8);
     fail.alloc(sizeof(int));
     panel.alloc((size_t)n * NB * 8);

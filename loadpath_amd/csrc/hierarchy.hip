@@ -451,7 +451,12 @@ static void alloc_vectors(Grid& G, Level& L) {
 
 int build_hierarchy(Grid& G) {
     hipStream_t s = G.stream;
-    DBuf counter, counter2, dtmp, dmaxb;
+    ++G.epoch;
+    G.fp16_off = false;
+    DBuf& counter = G.work.counter;
+    DBuf& counter2 = G.work.counter2;
+    DBuf& dtmp = G.work.dtmp;
+    DBuf& dmaxb = G.work.dmaxb;
     counter.alloc(sizeof(unsigned long long));
     counter2.alloc(sizeof(unsigned long long));
 
@@ -501,7 +506,7 @@ int build_hierarchy(Grid& G) {
             n_active = (long long)read_counter(counter2, s);
             C.npatch = (int)np;
             DBuf& list = C.patch_cell;
-            DBuf ctr;
+            DBuf& ctr = G.work.ctr;
             list.alloc(std::max<size_t>(1, np) * sizeof(int));
             if (G.opt.precision == LP_PREC_MIXED)
                 C.patch_prod.alloc(std::max<size_t>(1, np) * 24 * sizeof(float) * (size_t)G.max_rhs);
@@ -604,7 +609,7 @@ int build_hierarchy(Grid& G) {
         hipLaunchKernelGGL(k_mask_sym, dim3(nblk(ndof * ndof)), dim3(kBlock), 0, s, G.cinv.as<double>(),
                            C.mask.as<uint8_t>(), C.g.nn);
         LP_HIP(hipGetLastError());
-        if (!dense_spd_inverse(G.cinv.as<double>(), (int)ndof, nullptr, s)) return LP_SINGULAR;
+        if (!dense_spd_inverse(G.cinv.as<double>(), (int)ndof, G.work, s)) return LP_SINGULAR;
     }
     return LP_OK;
 }

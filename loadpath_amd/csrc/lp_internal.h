@@ -106,6 +106,16 @@ struct Level {
     bool lmax_warm = false;
 };
 
+// Scratch that the per-iteration paths (hierarchy rebuild, lambda_max,
+// coarsest factorization) reuse: it only grows, so a SIMP iteration makes
+// no device allocation (hipFree synchronizes the whole device, which would
+// serialize grids running concurrently on their own streams).
+struct Scratch {
+    DBuf counter, counter2, dtmp, dmaxb, ctr;
+    DBuf lmax_sc, lmax_av, lmax_tmp;
+    DBuf dn_dinv, dn_fail, dn_panel, dn_linv, dn_t;
+};
+
 struct Grid {
     lp_options opt{};
     Geom g0;
@@ -135,12 +145,31 @@ struct Grid {
     size_t host_pinned_bytes = 0;
     // staging for host pointers
     DBuf stage_a, stage_b;
+    Scratch work;
     int max_rhs = 1;
     bool ke_sym = false;        // Ke has the cube's symmetry (ke_pattern.h): orbit values used
     double ke_orbit[32] = {};
     int tune_fine_ny = 1;       // fine-level nodes per thread along y (1 or 2)
     int tune_fine_xchunk = 16;  // fine-level nodes per thread along x
     int tune_fused_cheb = 1;    // MFMA levels: Chebyshev vector passes fused into the operator
+    int tune_graph = 1;         // PCG replays the V-cycle from a captured hipGraph
+    int tune_flex = -1;         // flexible (Polak-Ribiere) beta: 1 on, 0 off, -1 on for the mixed path only
+    bool flex() const { return tune_flex < 0 ? opt.precision == LP_PREC_MIXED : tune_flex != 0; }
+    // mixed path: a solve the FP16 levels fail is repeated with FP32
+    // operators on every level (1), not repeated (0), or forced after one
+    // iteration (2, exercises the repeat); fp16_off holds for the rest of
+    // the current hierarchy, fallbacks counts the repeats
+    int tune_fallback = 1;
+    bool fp16_off = false;
+    int fallbacks = 0;
+    bool fp16() const { return opt.precision == LP_PREC_MIXED && !fp16_off; }
+    // the captured V-cycle: valid for one batch size and one epoch (every
+    // hierarchy build, lambda_max estimate or tuning change starts a new one,
+    // since the graph holds the level pointers and the Chebyshev coefficients)
+    hipGraphExec_t vgraph = nullptr;
+    int vgraph_nrhs = 0;
+    long long vgraph_epoch = -1;
+    long long epoch = 0;
     int vbytes() const { return opt.precision == LP_PREC_FP64 ? 8 : 4; }
     // launch profiling of one level operator (HIP events on the grid's stream)
     struct Prof {
@@ -154,6 +183,7 @@ struct Grid {
     mutable Prof prof;
     ~Grid() {
         for (hipEvent_t e : prof.ev) hipEventDestroy(e);
+        if (vgraph) hipGraphExecDestroy(vgraph);
     }
 };
 
@@ -235,7 +265,9 @@ void launch_fill(void* dst, int elem_bytes, size_t n, double v, hipStream_t s);
 // scale[rhs] = max |v_rhs| (1 if zero)
 // z = (double) v and the partial sums of r . z (skipping inactive rhs)
 void launch_cast_dot(const void* v, int vb, const double* r, double* z, size_t n, size_t stride, int nrhs,
-                     double* partials, const CgScal* scal, hipStream_t s);
+                     double* partials, const CgScal* scal, hipStream_t s, const double* ap = nullptr);
+// flexible beta from the two sums of launch_cast_dot(..., ap)
+void launch_cg_beta_flex(Grid& G, int nrhs, int nb, hipStream_t s);
 void launch_scale_by_dev(void* v, int elem_bytes, size_t n, size_t stride, int nrhs,
                          const double* num, size_t num_stride, int mode, hipStream_t s);
 void launch_cheb_start(const void* b, const void* dinv, double s_, void* x, void* r, void* d,
@@ -275,7 +307,7 @@ void level_cell_dofs(const Grid& G, int level, int64_t* out_dev);
 // ---- dense.hip -----------------------------------------------------------
 // Factor the SPD n x n matrix a (lower Cholesky, in place) and overwrite it
 // with its inverse. Returns false on a non-positive pivot.
-bool dense_spd_inverse(double* a, int n, double* work, hipStream_t s);
+bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s);
 void launch_dense_matvec(const double* ainv, int n, const void* b, void* x, int elem_bytes, int nrhs,
                          size_t stride, hipStream_t s);
 

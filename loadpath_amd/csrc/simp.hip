@@ -145,6 +145,17 @@ __global__ void k_filter(const double* __restrict__ v, const double* __restrict_
     }
 }
 
+// Heaviside projection and its derivative (simp.py:1443-1450)
+__global__ void k_project(const double* __restrict__ xt, size_t n, double beta, double eta, double* out,
+                          double* dout) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double den = tanh(beta * eta) + tanh(beta * (1.0 - eta));
+    const double t = tanh(beta * (xt[i] - eta));
+    if (out) out[i] = (tanh(beta * eta) + t) / den;
+    if (dout) dout[i] = beta * (1.0 - t * t) / den;
+}
+
 // MMA inputs: df0 = s dc, dcon = t dv, and block partials of sum xphys
 __global__ void k_mma_inputs(const double* __restrict__ dc, const double* __restrict__ dv,
                              const double* __restrict__ xphys, size_t n, double s, double t, double* df0,
@@ -486,6 +497,34 @@ int lp_simp_iterate(lp_simp* S, int niter, lp_simp_stats* stats) {
                 return LP_ERR_STATE;
             }
         }
+        return LP_OK;
+    });
+}
+
+int lp_project(long long n, const double* xt, double beta, double eta, double* out, double* dout, int device_ptr) {
+    return guarded_s([&]() -> int {
+        if (n < 0 || !xt || (!out && !dout) || !(beta > 0)) return LP_ERR_ARG;
+        if (n == 0) return LP_OK;
+        const size_t nb = (size_t)n * 8;
+        DBuf a, b, c;
+        const double* dx = xt;
+        double* o = out;
+        double* d = dout;
+        if (!device_ptr) {
+            a.alloc(nb);
+            LP_HIP(hipMemcpy(a.p, xt, nb, hipMemcpyHostToDevice));
+            dx = a.as<double>();
+            if (out) { b.alloc(nb); o = b.as<double>(); }
+            if (dout) { c.alloc(nb); d = c.as<double>(); }
+        }
+        hipLaunchKernelGGL(k_project, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, 0, dx, (size_t)n, beta,
+                           eta, o, d);
+        LP_HIP(hipGetLastError());
+        if (!device_ptr) {
+            if (out) LP_HIP(hipMemcpy(out, o, nb, hipMemcpyDeviceToHost));
+            if (dout) LP_HIP(hipMemcpy(dout, d, nb, hipMemcpyDeviceToHost));
+        }
+        LP_HIP(hipDeviceSynchronize());
         return LP_OK;
     });
 }

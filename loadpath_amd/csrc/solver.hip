@@ -113,7 +113,7 @@ static double estimate_lmax_level(Grid& G, int i, const double* start_host) {
     hipStream_t s = G.stream;
     L.lmax_vec.alloc(n * eb);
     if (start_host) {
-        DBuf tmp;
+        DBuf& tmp = G.work.lmax_tmp;
         tmp.alloc(n * 8);
         LP_HIP(hipMemcpyAsync(tmp.p, start_host, n * 8, hipMemcpyHostToDevice, s));
         if (eb == 8)
@@ -142,7 +142,8 @@ static double estimate_lmax_level(Grid& G, int i, const double* start_host) {
         LP_HIP(hipGetLastError());
     }
     const int its = G.opt.lmax_iters > 0 ? G.opt.lmax_iters : 12;
-    DBuf sc, av;
+    DBuf& sc = G.work.lmax_sc;
+    DBuf& av = G.work.lmax_av;
     sc.alloc(2 * sizeof(double) * its);
     av.alloc(n * eb);
     const void* dinv = eb == 8 ? L.dinv64.p : L.dinv32.p;
@@ -167,6 +168,7 @@ static double estimate_lmax_level(Grid& G, int i, const double* start_host) {
     LP_HIP(hipMemcpyAsync(h.data(), S, h.size() * 8, hipMemcpyDeviceToHost, s));
     LP_HIP(hipStreamSynchronize(s));
     L.lmax_warm = true;
+    ++G.epoch;
     double lam = 1.0;
     for (int it = 0; it < its; ++it) {
         if (h[2 * it] == 0.0) return 2.0;   // the reference returns 2.0 when the vector vanishes
@@ -302,6 +304,39 @@ void vcycle(Grid& G, int i, int nrhs) {
 }
 
 // z = M r for the FP64 CG vectors r -> z
+// one V-cycle on the level-0 vectors: replayed from a captured hipGraph
+// when that is valid (Chebyshev smoothing, whose buffer swaps cancel over a
+// V-cycle; no profiling events), captured again when the epoch changed
+static void vcycle_run(Grid& G, int nrhs) {
+    const bool use_graph = G.tune_graph && G.opt.smoother == LP_SMOOTH_CHEBYSHEV && !G.prof.on;
+    if (!use_graph) {
+        vcycle(G, 0, nrhs);
+        return;
+    }
+    if (!G.vgraph || G.vgraph_nrhs != nrhs || G.vgraph_epoch != G.epoch) {
+        if (G.vgraph) {
+            LP_HIP(hipGraphExecDestroy(G.vgraph));
+            G.vgraph = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        LP_HIP(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            vcycle(G, 0, nrhs);
+        } catch (...) {
+            hipStreamEndCapture(G.stream, &graph);
+            if (graph) hipGraphDestroy(graph);
+            throw;
+        }
+        LP_HIP(hipStreamEndCapture(G.stream, &graph));
+        const hipError_t e = hipGraphInstantiate(&G.vgraph, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        LP_HIP(e);
+        G.vgraph_nrhs = nrhs;
+        G.vgraph_epoch = G.epoch;
+    }
+    LP_HIP(hipGraphLaunch(G.vgraph, G.stream));
+}
+
 static void precondition(Grid& G, const double* r, double* z, int nrhs) {
     const int eb = G.vbytes();
     Level& L = G.lv[0];
@@ -321,14 +356,15 @@ static void read_scal(Grid& G, int nrhs, std::vector<CgScal>& out) {
 }
 
 // Batched PCG on G.b -> G.x (FP64). Returns per-rhs iterations and flags.
-void pcg(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
+static void pcg_run(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
     hipStream_t s = G.stream;
     const size_t n = 3 * G.g0.nn;
     const int nbd = dot_blocks(n);
     const int nba = apply_blocks(G, 0, 8);
-    ensure_partials(G, (size_t)nrhs * std::max(nbd, nba));
+    ensure_partials(G, (size_t)nrhs * std::max(2 * nbd, nba));
     double* P = G.partials.as<double>();
     CgScal* S = G.scal.as<CgScal>();
+    const bool flex = G.flex();
     launch_dot(G.b.p, G.b.p, 8, n, n, nrhs, P, nullptr, s);
     launch_cg_finalize(G, nrhs, 3, nbd, s);
     if (warm) {
@@ -352,7 +388,7 @@ void pcg(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
     Level& L0 = G.lv[0];
     const int eb = G.vbytes();
     launch_cast(G.r.p, 8, L0.vb.p, eb, n, n, nrhs, s);
-    vcycle(G, 0, nrhs);
+    vcycle_run(G, nrhs);
     launch_cast_dot(L0.vx.p, eb, G.r.as<double>(), G.z.as<double>(), n, n, nrhs, P, S, s);
     launch_cg_finalize(G, nrhs, 4, nbd, s);
     launch_cast(G.z.p, 8, G.p.p, 8, n, n, nrhs, s);
@@ -371,15 +407,46 @@ void pcg(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
         bool any = false;
         for (int k = 0; k < nrhs; ++k) any |= h[k].active != 0;
         if (!any) break;
-        vcycle(G, 0, nrhs);
-        launch_cast_dot(L0.vx.p, eb, G.r.as<double>(), G.z.as<double>(), n, n, nrhs, P, S, s);
-        launch_cg_finalize(G, nrhs, 2, nbd, s);
+        vcycle_run(G, nrhs);
+        if (flex) {
+            launch_cast_dot(L0.vx.p, eb, G.r.as<double>(), G.z.as<double>(), n, n, nrhs, P, S, s, G.ap.as<double>());
+            launch_cg_beta_flex(G, nrhs, nbd, s);
+        } else {
+            launch_cast_dot(L0.vx.p, eb, G.r.as<double>(), G.z.as<double>(), n, n, nrhs, P, S, s);
+            launch_cg_finalize(G, nrhs, 2, nbd, s);
+        }
         launch_xpby(G, nrhs, s);
     }
     read_scal(G, nrhs, h);
     for (int k = 0; k < nrhs; ++k) {
         iters[k] = h[k].iters + (h[k].failed ? 1 : 0);
         converged[k] = h[k].converged;
+    }
+}
+
+// PCG with the mixed path's safety net: when a right-hand side breaks down
+// or reaches the cap with the FP16 levels in the V-cycle, the solve is
+// repeated with FP32 operators on every level, warm from the iterates (the
+// warm start drops a non-finite or useless iterate to a cold start); the
+// switch holds until the hierarchy is rebuilt
+void pcg(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
+    const bool net = G.fp16() && G.tune_fallback != 0;
+    const int full = G.opt.maxiter;
+    if (net && G.tune_fallback == 2) G.opt.maxiter = 1;
+    pcg_run(G, nrhs, warm, iters, converged);
+    G.opt.maxiter = full;
+    if (!net) return;
+    bool ok = true;
+    for (int k = 0; k < nrhs; ++k) ok &= converged[k] != 0;
+    if (ok) return;
+    G.fp16_off = true;
+    ++G.epoch;   // a captured V-cycle holds the FP16 kernels
+    ++G.fallbacks;
+    std::vector<int> it2(nrhs), ok2(nrhs);
+    pcg_run(G, nrhs, true, it2.data(), ok2.data());
+    for (int k = 0; k < nrhs; ++k) {
+        iters[k] += it2[k];
+        converged[k] = ok2[k];
     }
 }
 
@@ -792,21 +859,70 @@ int lp_tune(lp_grid* g, int param, int value) {
         if (value != 1 && value != 2) return LP_ERR_ARG;
         const int old = g->tune_fine_ny;
         g->tune_fine_ny = value;
+        ++g->epoch;
         return old;
     }
     if (param == 1) {
         if (value < 1 || value > 64) return LP_ERR_ARG;
         const int old = g->tune_fine_xchunk;
         g->tune_fine_xchunk = value;
+        ++g->epoch;
         return old;
     }
     if (param == 2) {
         if (value != 0 && value != 1) return LP_ERR_ARG;
         const int old = g->tune_fused_cheb;
         g->tune_fused_cheb = value;
+        ++g->epoch;
         return old;
     }
+    if (param == 3) {
+        if (value != 0 && value != 1) return LP_ERR_ARG;
+        const int old = g->tune_graph;
+        g->tune_graph = value;
+        return old;
+    }
+    if (param == 4) {
+        if (value < -1 || value > 1) return LP_ERR_ARG;
+        const int old = g->tune_flex;
+        g->tune_flex = value;
+        return old;
+    }
+    if (param == 5) {
+        if (value < 0 || value > 2) return LP_ERR_ARG;
+        const int old = g->tune_fallback;
+        g->tune_fallback = value;
+        return old;
+    }
+    if (param == 6) return g->fallbacks;
     return LP_ERR_ARG;
+}
+
+int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best) {
+    return guarded([&]() -> int {
+        if (!g || !g->built || nrhs < 1 || nrhs > g->max_rhs || reps < 1) return LP_ERR_ARG;
+        Grid& G = *g;
+        const int eb = G.vbytes();
+        const int cand[] = {4, 8, 16, 32};
+        int best = G.tune_fine_xchunk;
+        double tbest = 1e300;
+        for (int c : cand) {
+            if (c > G.g0.Nx) continue;
+            G.tune_fine_xchunk = c;
+            double ms = 0.0;
+            const int rc = lp_time_apply(g, 0, eb, nrhs, reps, &ms);
+            if (rc != LP_OK) return rc;
+            if (ms < tbest) {
+                tbest = ms;
+                best = c;
+            }
+        }
+        G.tune_fine_xchunk = best;
+        ++G.epoch;
+        if (xchunk) *xchunk = best;
+        if (ms_best) *ms_best = tbest;
+        return LP_OK;
+    });
 }
 
 int lp_time_apply(lp_grid* g, int level, int dtype_bytes, int nrhs, int reps, double* ms_per_apply) {

@@ -242,22 +242,53 @@ __global__ void k_warm_norm(const double* __restrict__ b, const double* __restri
     if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = t;
 }
 
-// z = (double) v (the V-cycle's result); partial r . z for active rhs
+// z = (double) v (the V-cycle's result); partials of r . z for active rhs
+// and, when ap is set, of z . ap in the rows after the nrhs rows of r . z
+// (the flexible beta's second sum)
 template <typename S>
 __global__ void k_cast_dot(const S* __restrict__ v, const double* __restrict__ r, double* __restrict__ z, size_t n,
-                           size_t stride, double* partials, const CgScal* scal) {
+                           size_t stride, double* partials, const CgScal* scal, const double* __restrict__ ap) {
     __shared__ double sred[4];
     const int rhs = blockIdx.y;
     const bool on = !(scal && !scal[rhs].active);
-    double acc = 0;
+    double acc = 0, acc2 = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const size_t k = rhs * stride + i;
         const double zi = (double)v[k];
         z[k] = zi;
-        if (on) acc += r[k] * zi;
+        if (on) {
+            acc += r[k] * zi;
+            if (ap) acc2 += ap[k] * zi;
+        }
     }
     const double t = block_sum(acc, sred);
     if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = t;
+    if (ap) {
+        __syncthreads();
+        const double t2 = block_sum(acc2, sred);
+        if (threadIdx.x == 0) partials[((size_t)gridDim.y + rhs) * gridDim.x + blockIdx.x] = t2;
+    }
+}
+
+// flexible (Polak-Ribiere) beta for an inexact preconditioner:
+// beta = z1 . (r1 - r0) / (z0 . r0) = -alpha (z1 . A p0) / rz0, since
+// r1 - r0 = -alpha A p0; rz = z1 . r1
+__global__ void k_cg_beta_flex(const double* __restrict__ partials, int nb, int nrhs, CgScal* scal) {
+    __shared__ double sred[4];
+    const int rhs = blockIdx.x;
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+        a += partials[(size_t)rhs * nb + i];
+        b += partials[((size_t)nrhs + rhs) * nb + i];
+    }
+    const double rz1 = block_sum(a, sred);
+    __syncthreads();
+    const double zap = block_sum(b, sred);
+    if (threadIdx.x != 0) return;
+    CgScal& c = scal[rhs];
+    if (!c.active) return;
+    c.beta = -c.alpha * zap / c.rz;
+    c.rz = rz1;
 }
 
 inline int grid_for(size_t n) {
@@ -314,14 +345,20 @@ void launch_cast(const void* src, int sb, void* dst, int db, size_t n, size_t st
 }
 
 void launch_cast_dot(const void* v, int vb, const double* r, double* z, size_t n, size_t stride, int nrhs,
-                     double* partials, const CgScal* scal, hipStream_t s) {
+                     double* partials, const CgScal* scal, hipStream_t s, const double* ap) {
     dim3 grid(dot_blocks(n), nrhs);
     if (vb == 4)
         hipLaunchKernelGGL(k_cast_dot<float>, grid, dim3(kBlock), 0, s, (const float*)v, r, z, n, stride, partials,
-                           scal);
+                           scal, ap);
     else
         hipLaunchKernelGGL(k_cast_dot<double>, grid, dim3(kBlock), 0, s, (const double*)v, r, z, n, stride, partials,
-                           scal);
+                           scal, ap);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_cg_beta_flex(Grid& G, int nrhs, int nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_beta_flex, dim3(nrhs), dim3(kBlock), 0, s, G.partials.as<double>(), nb, nrhs,
+                       G.scal.as<CgScal>());
     LP_HIP(hipGetLastError());
 }
 

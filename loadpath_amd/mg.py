@@ -308,6 +308,23 @@ class GridElasticity:
                    "element_product")
         return out[0][tuple(self.elem_ijk.T)]
 
+    _TUNED = {}
+
+    def autotune(self, nrhs=None, reps=5) -> int:
+        """Pick the fine-level x-chunk of the V-cycle operator by timing
+        (lp_autotune); cached per (cells, precision, batch) for the process."""
+        nrhs = int(nrhs or self.max_rhs)
+        key = (self.cells, self.precision, nrhs)
+        lib = _lib.lib()
+        if key in GridElasticity._TUNED:
+            lib.lp_tune(self._h, 1, GridElasticity._TUNED[key])
+            return GridElasticity._TUNED[key]
+        xc = C.c_int()
+        ms = C.c_double()
+        _lib.check(lib.lp_autotune(self._h, nrhs, int(reps), C.byref(xc), C.byref(ms)), "autotune")
+        GridElasticity._TUNED[key] = xc.value
+        return xc.value
+
     def time_apply(self, level=0, dtype=np.float64, nrhs=1, reps=20) -> float:
         """Mean device time (ms) of one operator apply (HIP events)."""
         ms = C.c_double()

@@ -16,6 +16,12 @@ what they are:
   a unit total force with direction standard_normal(3)/|.| spread over an
   8 x 8-node patch of the top face, origin integers(...), default_rng(4);
   vf 0.15, rmin 2.5.
+- ``batched_studies``: 32 independent studies from default_rng(5): cells
+  nx in [32, 96], ny, nz in [16, 48], multiples of 8; vf U(0.1, 0.4); rmin
+  1.5 or 2.0; cantilevers (x = 0 clamped, load on the x = nx face) and half
+  MBB beams (ux = uy = 0 on x = 0, uz = 0 on the bottom edge at x = nx, load
+  on the top edge at x = 0) alternating; a unit force in a seeded direction
+  at one node or spread over a 4 x 4-node patch.
 """
 from __future__ import annotations
 
@@ -149,3 +155,29 @@ def multiload_256x128x128(nx=256, ny=128, nz=128, ncases=8, patch=8, seed=4):
         cases.append([(nd, d / len(nd))])
     return BoxProblem(f"multiload-{nx}x{ny}x{nz}", (nx, ny, nz), 0.15, 2.5,
                       [(nodes(range(nx + 1), range(ny + 1), 0), "xyz")], cases)
+
+
+def batched_studies(count=32, seed=5):
+    """The uneven studies of one batched run (module docstring)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        nx = 8 * int(rng.integers(4, 13))
+        ny = 8 * int(rng.integers(2, 7))
+        nz = 8 * int(rng.integers(2, 7))
+        vf = float(rng.uniform(0.1, 0.4))
+        rmin = float(rng.choice([1.5, 2.0]))
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        patch = bool(rng.integers(0, 2))
+        cy, cz = ny // 2, nz // 2
+        if k % 2 == 0:
+            fixed = [(nodes(0, range(ny + 1), range(nz + 1)), "xyz")]
+            nd = nodes(nx, range(cy - 2, cy + 2), range(cz - 2, cz + 2)) if patch else nodes(nx, cy, cz)
+            name = f"cantilever-{nx}x{ny}x{nz}"
+        else:
+            fixed = [(nodes(0, range(ny + 1), range(nz + 1)), "xy"), (nodes(nx, range(ny + 1), 0), "z")]
+            nd = nodes(0, range(cy - 2, cy + 2), nz) if patch else nodes(0, cy, nz)
+            name = f"mbb-{nx}x{ny}x{nz}"
+        out.append(BoxProblem(name, (nx, ny, nz), vf, rmin, fixed, [[(nd, d / len(nd))]]))
+    return out

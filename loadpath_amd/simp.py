@@ -57,6 +57,26 @@ class GridFilter:
         return self @ np.ones(self.n)
 
 
+def _project_call(xt, beta, eta, want_value, want_deriv):
+    x = np.ascontiguousarray(np.asarray(xt, float))
+    out = np.empty_like(x) if want_value else None
+    dout = np.empty_like(x) if want_deriv else None
+    _lib.check(_lib.lib().lp_project(x.size, _lib.ptr(x), float(beta), float(eta),
+                                     _lib.ptr(out) if want_value else None,
+                                     _lib.ptr(dout) if want_deriv else None, 0), "project")
+    return out, dout
+
+
+def project(xt, beta, eta):
+    """Heaviside projection simp._project (simp.py:1443-1446), on the GPU."""
+    return _project_call(xt, beta, eta, True, False)[0]
+
+
+def dproject(xt, beta, eta):
+    """Its derivative simp._dproject (simp.py:1448-1450), on the GPU."""
+    return _project_call(xt, beta, eta, False, True)[1]
+
+
 def density_filter(dims, radius: float, pad: bool = True):
     """(h, hs) as simp.density_filter (simp.py:1181-1189) on a full box."""
     h = GridFilter(dims, radius)

@@ -232,6 +232,18 @@ def oc_bisect(x, dc, dv, h, hs, volfrac, move=0.2, passive=None, l1=0.0, l2=1e9,
     return xnew, xphys, evals
 
 
+def project(xt, beta, eta):
+    """Heaviside projection (simp._project, simp.py:1443-1446)."""
+    den = np.tanh(beta * eta) + np.tanh(beta * (1 - eta))
+    return (np.tanh(beta * eta) + np.tanh(beta * (np.asarray(xt, float) - eta))) / den
+
+
+def dproject(xt, beta, eta):
+    """Its derivative (simp._dproject, simp.py:1448-1450)."""
+    den = np.tanh(beta * eta) + np.tanh(beta * (1 - eta))
+    return beta * (1 - np.tanh(beta * (np.asarray(xt, float) - eta)) ** 2) / den
+
+
 def optimize_mma(p: Problem, iterations, solve=None, penal=3.0, move=0.2, element="im", filt=None):
     """Compliance with the volume constraint under MMA: f0 = 10 c / c_first,
     con = sum xphys / (vf n) - 1, both gradients through the filter (the

@@ -57,6 +57,29 @@ def test_mfma_operator_cancels_rigid_offset():
     assert np.abs(got - want).max() <= TOL_FP16 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("level", [0, 1])
+def test_mfma_operator_tiny_magnitudes(level):
+    """Cells whose corner differences are below 2^-113 (FP32 vectors of the
+    V-cycle in void regions reach such values) must keep their FP16
+    accuracy: the per-cell power-of-two scale stays finite, so no NaN
+    comes from the exact zero of corner 0. Right-hand side 0 is tiny
+    everywhere, right-hand side 1 spans magnitudes 1 ... 1e-36 plane by
+    plane."""
+    mesh, fixed, f, e = cantilever_bar(16, 10, 8, seed=15)
+    g, o = _grids(mesh, fixed, e, coarse_cells=20)
+    lv = o.levels[level]
+    rng = np.random.default_rng(16)
+    u = rng.standard_normal((2,) + lv.node_shape)
+    u[0] *= 2e-36
+    nxp = lv.node_shape[1]
+    u[1] *= (10.0 ** -np.linspace(0, 36, nxp))[None, :, None, None]
+    got = g.apply(u.astype(np.float32), level=level, dtype=np.float32)
+    assert np.isfinite(got).all()
+    for k in range(2):
+        want = lv.apply(u[k].astype(np.float32).astype(np.float64))
+        assert np.abs(got[k] - want).max() <= TOL_FP16 * np.abs(want).max()
+
+
 def test_weights_level_matches_oracle():
     """The first coarse level through the FP16-MFMA weights operator (sum of
     w_q M_q, explicit patches for cells touching held dofs) equals the
@@ -131,6 +154,41 @@ def test_mixed_solve_matches_direct():
     g32.set_moduli(e)
     g32.solve_many([f])
     assert g.last_iters <= 1.3 * g32.last_iters + 2, (g.last_iters, g32.last_iters)
+
+
+def test_mixed_fallback_repeats_with_fp32():
+    """The mixed path's safety net: a solve the FP16 levels fail is repeated
+    with FP32 operators on every level, warm from the iterates. Forced here
+    after one iteration (lp_tune param 5 = 2): the answer is the direct
+    solve's, one repeat is counted, and the repeat needs no more iterations
+    than a plain FP32 solve plus the forced one."""
+    from oracle.fem import assemble, hex8_ke
+    from loadpath_amd import _lib
+    from loadpath_amd.mg import GridElasticity
+
+    mesh, fixed, f, e = cantilever_bar(16, 8, 8, seed=3)
+    k = assemble(mesh, hex8_ke(0.3), e)
+    free = ~fixed
+    ref = np.zeros(mesh.n_dof)
+    ref[free] = spla.spsolve(k[free][:, free].tocsc(), f[free])
+    g = GridElasticity(mesh, 0.3, fixed, precision="mixed", rtol=1e-10, coarse_cells=100, maxiter=2000)
+    g.set_moduli(e)
+    lib = _lib.lib()
+    assert lib.lp_tune(g._h, 5, 2) == 1
+    u = g.solve_many([f, 2.0 * f])
+    assert lib.lp_tune(g._h, 6, 0) == 1
+    assert np.linalg.norm(u[0] - ref) <= 1e-7 * np.linalg.norm(ref)
+    assert np.linalg.norm(u[1] - 2.0 * ref) <= 1e-7 * np.linalg.norm(2.0 * ref)
+    g32 = GridElasticity(mesh, 0.3, fixed, precision="fp32", rtol=1e-10, coarse_cells=100, maxiter=2000)
+    g32.set_moduli(e)
+    g32.solve_many([f])
+    assert g.last_iters <= g32.last_iters + 3, (g.last_iters, g32.last_iters)
+    # a new hierarchy uses the FP16 levels again; unforced, the healthy
+    # solve needs no repeat
+    assert lib.lp_tune(g._h, 5, 1) == 2
+    g.set_moduli(e)
+    g.solve_many([f])
+    assert lib.lp_tune(g._h, 6, 0) == 1
 
 
 def test_mixed_simp_tracks_fp64():

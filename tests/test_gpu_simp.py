@@ -76,3 +76,28 @@ def test_mma_iterations_match_oracle(gpu):
         assert st["change"] == pytest.approx(cho, rel=1e-6, abs=1e-9)
     assert np.allclose(s.x.ravel(), xo, rtol=0, atol=1e-7)
     assert np.allclose(s.xphys.ravel(), xpo, rtol=0, atol=1e-7)
+
+
+def test_projection_matches_oracle(gpu):
+    """lp_project against simp._project / _dproject restated (simp.py:1443-1450),
+    through numpy and through the torch.autograd binding on CUDA tensors."""
+    import torch
+
+    from loadpath_amd.autograd import project as tproject
+    from loadpath_amd.simp import dproject, project
+    from oracle.simp import dproject as odp
+    from oracle.simp import project as op
+
+    x = np.random.default_rng(11).uniform(0, 1, 20000)
+    for beta, eta in ((1.0, 0.5), (8.0, 0.3), (64.0, 0.55)):
+        assert np.allclose(project(x, beta, eta), op(x, beta, eta), rtol=1e-14, atol=1e-15)
+        # 1 - tanh^2 cancels far from eta: a 1-ulp tanh difference is an
+        # absolute error of a few ulp of the derivative's scale beta / den
+        d = odp(x, beta, eta)
+        assert np.abs(dproject(x, beta, eta) - d).max() <= 1e-13 * np.abs(d).max()
+    xt = torch.tensor(x, dtype=torch.float64, device="cuda", requires_grad=True)
+    y = tproject(xt, 8.0, 0.3)
+    (y * 2.0).sum().backward()
+    assert np.allclose(y.detach().cpu().numpy(), op(x, 8.0, 0.3), rtol=1e-14, atol=1e-15)
+    d = 2.0 * odp(x, 8.0, 0.3)
+    assert np.abs(xt.grad.cpu().numpy() - d).max() <= 1e-13 * np.abs(d).max()

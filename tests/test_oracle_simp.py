@@ -91,3 +91,22 @@ def test_compliance_sensitivity_finite_differences():
         xm[c] -= h
         fd = (ev.compliance(xp) - ev.compliance(xm)) / (2 * h)
         assert fd == pytest.approx(dc[c], rel=1e-5)
+
+
+def test_projection_closed_forms():
+    """simp._project / _dproject (simp.py:1443-1450): fixes 0 and 1 and eta's
+    value tanh(b e) / den; tends to the step at eta as beta grows; the
+    derivative equals central differences."""
+    from oracle.simp import dproject, project
+
+    for beta, eta in ((1.0, 0.5), (8.0, 0.3), (32.0, 0.6)):
+        assert project(0.0, beta, eta) == pytest.approx(0.0, abs=1e-15)
+        assert project(1.0, beta, eta) == pytest.approx(1.0, rel=1e-15)
+        den = np.tanh(beta * eta) + np.tanh(beta * (1 - eta))
+        assert project(eta, beta, eta) == pytest.approx(np.tanh(beta * eta) / den, rel=1e-15)
+        x = np.linspace(0.01, 0.99, 31)
+        h = 1e-6
+        fd = (project(x + h, beta, eta) - project(x - h, beta, eta)) / (2 * h)
+        assert np.allclose(dproject(x, beta, eta), fd, rtol=1e-6, atol=1e-9)
+    x = np.array([0.2, 0.45, 0.55, 0.8])
+    assert np.allclose(project(x, 512.0, 0.5), [0, 0, 1, 1], atol=1e-10)
