@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # sides, mean over the 52 launches of one mixed SIMP iteration): FETCH_SIZE x 2
 # + WRITE_SIZE x 1, the factors from a 1 GiB copy in the same run; from
 # profiles/counters_multiload_256x128x128.txt
-FINE_MFMA_TRAFFIC_BYTES = 2.0591e9
+FINE_MFMA_TRAFFIC_BYTES = 2.6226e9
 WORKLOAD = dict(nx=256, ny=128, nz=128, cases=8)
 
 
@@ -56,6 +56,12 @@ def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None):
         if log:
             log(f"  {precision} warmup {k}: {hist[-1]}")
     lib = _lib.lib()
+    if warmup > 0:
+        # launch shape of the V-cycle's fine-level operator, timed on this
+        # grid (set-up, outside the timed region)
+        xc = s.grid.autotune(s.nrhs)
+        if log:
+            log(f"  {precision} autotuned x-chunk: {xc}")
     eb = 8 if precision == "fp64" else 4
     if profile:
         _lib.check(lib.lp_profile(s.grid._h, 0, eb, 1), "profile")
@@ -201,8 +207,9 @@ def main():
         gbs = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9
         roof = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
                     traffic=FINE_MFMA_TRAFFIC_BYTES if (nx, ny, nz) == (WORKLOAD["nx"], WORKLOAD["ny"], WORKLOAD["nz"])
-                    else None, kernel="k_apply_mfma<K, false> (V-cycle fine-level operator: FP16 MFMA contraction, "
-                                         "fused Chebyshev / residual epilogues in FP32)",
+                    else None, kernel="k_apply_wave<K, false> (V-cycle fine-level operator: FP16 MFMA contraction, "
+                                         "node sums across lanes in registers, fused Chebyshev / residual "
+                                         "epilogues in FP32)",
                     launches=prof["launches"], avg_us=round(prof["ms"] * 1e3 / prof["launches"], 2),
                     avg_bytes=round(prof["bytes"] / prof["launches"]))
     cpu = None if a.no_cpu else cpu_baseline(mixed["timed"], log)

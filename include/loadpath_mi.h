@@ -227,7 +227,7 @@ int lp_mma_get(lp_mma* mma, int what, double* out);
 
 /* The autotuner (no reference counterpart): times the fine-level operator
  * of the V-cycle (FP16 MFMA on the mixed path, the vector kernel otherwise)
- * on nrhs right-hand sides for each x-chunk in {4, 8, 16, 32}, keeps the
+ * on nrhs right-hand sides for each x-chunk in {4, 8, 12, 16, 24, 32}, keeps the
  * fastest and returns it with its time per apply. */
 int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best);
 
@@ -245,7 +245,11 @@ int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best);
  * until the next hierarchy build (1, the default), do not (0), or force
  * the repeat after one iteration (2, for tests). Returns the previous
  * value. Param 6 returns the number of such repeats so far (value
- * ignored). */
+ * ignored). Param 7: which FP16-MFMA level operators run as the wave-tile
+ * kernel (node sums formed in registers across lanes) rather than the
+ * block-tile kernel (node sums through LDS): the fine level (1, the
+ * default; the faster choice on each level), both levels (2) or neither
+ * (0). */
 int lp_tune(lp_grid* g, int param, int value);
 
 #ifdef __cplusplus

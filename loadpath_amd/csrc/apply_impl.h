@@ -41,6 +41,7 @@ struct AGeom {
     int xchunk, nyzb, nxc, nblk, per;
     int ny_per;             // nodes per thread along y (fine kernel)
     int ntz;                // node tiles along z (fine MFMA kernel)
+    int nwt;                // wave tiles (wave-tile MFMA kernel)
 };
 
 // Ke as its KE_NV orbit values (see ke_pattern.h): kernel arguments, so

@@ -391,6 +391,339 @@ __global__ void __launch_bounds__(kBlock) k_patch_prod(const float* __restrict__
     out[((size_t)rhs * npatch + p) * 24 + row] = acc;
 }
 
+// ---------------------------------------------------------------------------
+// Wave-tile variant: the same contraction, with the node sums formed in
+// registers. Each wave owns a 15 x 15 node tile in (y, z) (16 x 16 cells)
+// and marches an x-chunk on its own (no block barriers after set-up). Lane
+// (hh, r, c) = (lane >> 5, (lane >> 4) & 1, lane & 15); MFMA tile t (0..7)
+// is cell rows Y0 - 1 + 2t + r, cell columns Z0 - 1 + c. The B half hh holds
+// the corners with bx = hh (its cell's 4 nodes in plane X + hh), the A
+// columns permuted to match, so after the contraction lane (hh, r, c) holds
+// its cell's 12 outputs for the corners with bx = hh. A node then sums:
+//  - in z, its cell (bz = 0 corners) and the left neighbour's (bz = 1):
+//    one DPP row_shr:1 add;
+//  - in y, the cell row below's by = 1 part: from the other 16-lane row of
+//    the tile or, for row 0, the previous tile's row 1 (v_permlane16_swap);
+//  - in x, the x-high part of the previous cell layer, kept in registers
+//    per tile and brought across the wave halves with v_permlane32_swap, so
+//    the lanes of half hh finish tile 2p + hh of each tile pair p.
+// The staged planes are private to the wave (two, alternating).
+constexpr int WN = 15;                  // nodes per wave tile side
+constexpr int WS = WN + 2;              // staged nodes per side
+constexpr int WPL = WS * WS;            // 289
+constexpr int WCS = 292;                // component stride of a staged plane
+constexpr int WPS = 3 * WCS;            // plane stride
+constexpr int kWaveTiles = kThreads / 64;
+constexpr int WSTAGE = (WPL + 63) / 64; // staging entries per lane
+
+__device__ __forceinline__ float dpp_shr1(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
+}
+
+// v_permlane16_swap: odd 16-lane rows of a <-> even rows of b
+__device__ __forceinline__ void swap16(float& a, float& b) {
+    auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+
+// v_permlane32_swap: lanes 32-63 of a <-> lanes 0-31 of b
+__device__ __forceinline__ void swap32(float& a, float& b) {
+    auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+
+template <int EK, bool WTS>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 4)))
+k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __restrict__ mask, AGeom G, MfmaSrc S,
+             EpiArgs E) {
+    __shared__ float stage[kWaveTiles * 2 * WPS];
+    __shared__ __align__(8) _Float16 alds[WTS ? 8 * 3 * 64 * 4 : 4];
+    using namespace apply_detail;
+    const int rhs = blockIdx.z;
+    const int L = blockIdx.x;
+    const int logical = (L & 7) * G.per + (L >> 3);
+    const int tid = (int)threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wt = logical * kWaveTiles + wave;
+    double part = 0.0;
+    const bool bskip = (E.scal != nullptr && !E.scal[rhs].active) || logical >= G.nblk;
+    constexpr bool need_raw_all = EK == EPI_STORE_DOT || EK == EPI_JACOBI || EK == EPI_POWER;
+    constexpr bool STAGE_B = EK == EPI_CHEB_FIRST || EK == EPI_CHEB_ONLY;
+    const int jc = lane & 31, hh = lane >> 5, r = (lane >> 4) & 1, c = lane & 15;
+    half4 A[3];
+    if (WTS) {
+        if (!bskip)
+            for (int k = tid; k < 8 * 3 * 64 * 4; k += kThreads) {
+                const int i = k & 3, l = (k >> 2) & 63, s = (k >> 8) % 3, q = k / 768;
+                const int row = l & 31, col = 8 * s + 4 * (l >> 5) + i;
+                alds[k] = row < 24 ? (_Float16)S.a[q * 576 + ke_row_of(row) * 24 + ke_row_of(col)] : (_Float16)0.0f;
+            }
+        __syncthreads();
+    } else {
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                A[s][i] = jc < 24 ? (_Float16)S.a[ke_row_of(jc) * 24 + ke_row_of(8 * s + 4 * hh + i)] : (_Float16)0.0f;
+    }
+    if (!bskip && wt < G.nwt) {
+        const int xc = wt / G.nyzb;
+        const int rem = wt - xc * G.nyzb;
+        const int ty = rem / G.ntz;
+        const int Y0 = ty * WN, Z0 = (rem - ty * G.ntz) * WN;
+        const int Ny = G.Ny, Nz = G.Nz, Nx = G.Nx;
+        const int nyz = Ny * Nz;
+        const int x0 = xc * G.xchunk;
+        const int x1 = min(x0 + G.xchunk, Nx);
+        const float* __restrict__ u = u0 + (size_t)rhs * vstride;
+        const unsigned nnu = (unsigned)G.nn;
+        const __amdgpu_buffer_rsrc_t ru = make_rsrc(u, 3u * nnu * 4u);
+        const __amdgpu_buffer_rsrc_t rm = make_rsrc(mask, nnu);
+        const size_t ne = WTS ? (size_t)G.fnx * G.fny * G.fnz : G.ncell;
+        const __amdgpu_buffer_rsrc_t re = make_rsrc(S.e, (unsigned)ne * 4u);
+        const __amdgpu_buffer_rsrc_t rdv = make_rsrc(STAGE_B ? E.dinv : nullptr, STAGE_B ? 3u * nnu * 4u : 0u);
+        const float s_b = (float)E.c3;
+        float* SB = stage + wave * 2 * WPS;
+
+        // staging: entries lane + 64 k of the 17 x 17 patch; goff = the
+        // node's (y, z) offset in a plane, or -1 outside the grid
+        int goff[WSTAGE];
+#pragma unroll
+        for (int k = 0; k < WSTAGE; ++k) {
+            const int e = lane + 64 * k;
+            const int sj = e / WS, sk = e - (e / WS) * WS;
+            const int y = Y0 - 1 + sj, z = Z0 - 1 + sk;
+            goff[k] = (e < WPL && y >= 0 && y < Ny && z >= 0 && z < Nz) ? y * Nz + z : -1;
+        }
+        float pre[WSTAGE][3];
+        auto fetch = [&](int x) {
+            const bool inx = x >= 0 && x < Nx;
+#pragma unroll
+            for (int k = 0; k < WSTAGE; ++k) {
+                const bool ok = inx && goff[k] >= 0;
+                const unsigned n = (unsigned)(x * nyz + goff[k]);
+                const int m = bload_u8(rm, ok ? n : kOOB);
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    const unsigned off = ((m >> cc) & 1) ? (cc * nnu + n) * 4u : kOOB;
+                    pre[k][cc] = bload<float>(ru, off);
+                    if (STAGE_B) pre[k][cc] = (bload<float>(rdv, off) * pre[k][cc]) * s_b;
+                }
+            }
+        };
+        auto store = [&](int buf) {
+            __builtin_amdgcn_wave_barrier();
+            float* P = SB + buf * WPS;
+#pragma unroll
+            for (int k = 0; k < WSTAGE; ++k) {
+                const int e = lane + 64 * k;
+                if (e < WPL) {
+                    P[e] = pre[k][0];
+                    P[WCS + e] = pre[k][1];
+                    P[2 * WCS + e] = pre[k][2];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        };
+
+        // one tile's contraction on cell layer X (planes in buffers bl = X,
+        // bh = X + 1): nv = the node sums of this layer's side hh, u0v = the
+        // staged values at the lane's corner 0 (node (X, cell row, cell col))
+        float ycar[3] = {0.0f, 0.0f, 0.0f};   // y carry: previous tile's by = 1 sums
+        auto tile = [&](int X, int t, int bl, int bh, float ecell, float (&nv)[3], float (&u0v)[3]) {
+            const float* PL = SB + bl * WPS;
+            const float* PO = SB + (hh ? bh : bl) * WPS;
+            const int s0 = (2 * t + r) * WS + c;
+            float d[12];
+            float dmax = 0.0f;
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) u0v[cc] = PL[cc * WCS + s0];
+#pragma unroll
+            for (int q = 0; q < 12; ++q) {
+                const int b4 = q / 3, cc = q % 3;
+                d[q] = PO[cc * WCS + s0 + (b4 & 1) * WS + (b4 >> 1)] - u0v[cc];
+                dmax = fmaxf(dmax, fabsf(d[q]));
+            }
+            {
+                // a cell's 24 operands sit in lanes l and l + 32: one scale
+                float lo = dmax, hi = dmax;
+                swap32(lo, hi);
+                dmax = fmaxf(lo, hi);
+            }
+            int ex;
+            (void)frexpf(dmax, &ex);
+            ex = max(ex, -100);       // a finite scale (see k_apply_mfma)
+            const float sc = ldexpf(1.0f, 15 - ex);
+            half4 B[3];
+#pragma unroll
+            for (int s = 0; s < 3; ++s)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) B[s][i] = (_Float16)(d[4 * s + i] * sc);
+            float y[12];
+            if (!WTS) {
+                floatx16 acc = {};
+#pragma unroll
+                for (int s = 0; s < 3; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x8f16(A[s], B[s], acc, 0, 0, 0);
+                const float f = ecell * ldexpf(1.0f, ex - 15);
+#pragma unroll
+                for (int q = 0; q < 12; ++q) y[q] = acc[q] * f;
+            } else {
+                const int cy = Y0 - 1 + 2 * t + r, cz = Z0 - 1 + c;
+                const bool cok = X >= 0 && X < G.nx && cy >= 0 && cy < G.ny && cz >= 0 && cz < G.nz;
+                const int cell = (X * G.ny + cy) * G.nz + cz;
+                const int pi = cok ? S.pidx[cell] : -1;
+#pragma unroll
+                for (int q = 0; q < 12; ++q) y[q] = 0.0f;
+#pragma unroll 2
+                for (int ch = 0; ch < 8; ++ch) {
+                    const int fx = 2 * X + (ch & 1), fy = 2 * cy + ((ch >> 1) & 1), fz = 2 * cz + ((ch >> 2) & 1);
+                    const bool wok = cok && pi < 0 && fx < G.fnx && fy < G.fny && fz < G.fnz;
+                    const float w = bload<float>(re, wok ? (unsigned)((fx * G.fny + fy) * G.fnz + fz) * 4u : kOOB);
+                    floatx16 acc = {};
+#pragma unroll
+                    for (int s = 0; s < 3; ++s) {
+                        const half4 a = *reinterpret_cast<const half4*>(alds + ((ch * 3 + s) * 64 + lane) * 4);
+                        acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a, B[s], acc, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 12; ++q) y[q] = fmaf(w, acc[q], y[q]);
+                }
+                const float f = ldexpf(1.0f, ex - 15);
+                if (pi >= 0) {
+                    const float* pp = S.pprod + ((size_t)rhs * S.npatch + pi) * 24;
+#pragma unroll
+                    for (int q = 0; q < 12; ++q) y[q] = pp[row_of_slot(hh, q)];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 12; ++q) y[q] *= f;
+                }
+            }
+            // slot q = 3 (by + 2 bz) + comp; z: own bz = 0 + left neighbour's bz = 1
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const float z0s = y[cc] + dpp_shr1(y[6 + cc]);
+                float z1s = y[3 + cc] + dpp_shr1(y[9 + cc]);
+                // y: row 1 takes row 0's by = 1 sums, row 0 the previous tile's row 1
+                float pv = ycar[cc];
+                swap16(pv, z1s);
+                nv[cc] = z0s + (r ? pv : z1s);
+                ycar[cc] = z1s;
+            }
+        };
+
+        // the epilogue's inputs of one node, fetched before the pair's tiles
+        // (the node's own operand values only where the epilogue needs them:
+        // masked-out dofs, the kinds that read it, b of the first Chebyshev
+        // step)
+        struct Pend {
+            int em;
+            float raw[3];
+            EpiIn<float> in;
+            bool own;
+            unsigned n;
+        };
+        auto epi_fetch = [&](int X, int p, Pend& P) {
+            const int t = 2 * p + hh;
+            const int yn = Y0 - 1 + 2 * t + r, zn = Z0 - 1 + c;
+            P.own = X < x1 && c >= 1 && (t > 0 || r > 0) && yn < Ny && zn < Nz;
+            P.n = (unsigned)(X * nyz + yn * Nz + zn);
+            P.em = 0;
+            if (!P.own) return;
+            P.em = bload_u8(rm, P.n);
+            if (need_raw_all || STAGE_B)
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) P.raw[cc] = bload<float>(ru, (cc * nnu + P.n) * 4u);
+            epi_load<float, EK>(E, (size_t)rhs * vstride + P.n, G.nn, P.n, P.in);
+        };
+        auto ecell_load = [&](int X, int t) -> float {
+            if (WTS) return 0.0f;
+            const int cy = Y0 - 1 + 2 * t + r, cz = Z0 - 1 + c;
+            const bool cok = X >= 0 && X < G.nx && cy >= 0 && cy < G.ny && cz >= 0 && cz < G.nz;
+            return bload<float>(re, cok ? (unsigned)((X * G.ny + cy) * G.nz + cz) * 4u : kOOB);
+        };
+
+        float car[8][3];   // x carry: the previous layer's node sums per tile
+        float u0a[3], u0b[3];
+        // planes x0 - 1 -> buffer 0, x0 -> 1; the layer before the chunk
+        // gives the x carry (nothing before the grid)
+        fetch(x0 - 1);
+        store(0);
+        fetch(x0);
+        store(1);
+        fetch(x0 + 1);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const float ec = ecell_load(x0 - 1, t);
+            if (x0 > 0) {
+                tile(x0 - 1, t, 0, 1, ec, car[t], u0a);
+            } else {
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) car[t][cc] = 0.0f;
+            }
+        }
+        store(0);
+        Pend cur;
+        float ea0 = ecell_load(x0, 0), ea1 = ecell_load(x0, 1);
+        for (int X = x0; X < x1; ++X) {
+            const int bl = (X - x0 + 1) & 1, bh = bl ^ 1;
+            const bool more = X + 1 < x1;   // plane X + 2 is needed by layer X + 1
+            if (more) fetch(X + 2);
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                epi_fetch(X, p, cur);
+                const float e0 = ea0, e1 = ea1;
+                if (p < 3) {
+                    ea0 = ecell_load(X, 2 * p + 2);
+                    ea1 = ecell_load(X, 2 * p + 3);
+                } else {
+                    ea0 = ecell_load(X + 1, 0);
+                    ea1 = ecell_load(X + 1, 1);
+                }
+                float na[3], nb[3];
+                tile(X, 2 * p, bl, bh, e0, na, u0a);
+                tile(X, 2 * p + 1, bl, bh, e1, nb, u0b);
+                // x: lanes of half hh finish tile 2p + hh
+                float yv[3], ur[3];
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    float ca = car[2 * p][cc], nbv = nb[cc];
+                    swap32(ca, nbv);
+                    const float r1 = hh ? ca : na[cc];
+                    const float r2 = hh ? car[2 * p + 1][cc] : nbv;
+                    yv[cc] = r1 + r2;
+                    car[2 * p][cc] = na[cc];
+                    car[2 * p + 1][cc] = nbv;
+                }
+                if (cur.own) {
+                    if (!(need_raw_all || STAGE_B) && cur.em != 7)
+#pragma unroll
+                        for (int cc = 0; cc < 3; ++cc) cur.raw[cc] = bload<float>(ru, (cc * nnu + cur.n) * 4u);
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) {
+                        const bool mc = (cur.em >> cc) & 1;
+                        const float st = hh ? u0b[cc] : u0a[cc];
+                        if (STAGE_B) {
+                            const float b = cur.raw[cc];
+                            ur[cc] = mc ? st : (cur.in.v[3][cc] * b) * s_b;
+                            cur.in.v[0][cc] = b;
+                        } else {
+                            ur[cc] = (!mc || need_raw_all) ? cur.raw[cc] : st;
+                        }
+                        if (!mc) yv[cc] = ur[cc];
+                    }
+                    epi_finish<float, EK, float>(E, cur.em, (size_t)rhs * vstride + cur.n, G.nn, yv, ur, cur.in,
+                                                 part);
+                }
+            }
+            // plane X's buffer takes plane X + 2 (loaded during this layer)
+            if (more) store(bl);
+        }
+    }
+    if (E.partials != nullptr) block_partial(part, E, rhs, L);
+}
+
 AGeom mfma_geom(const Grid& Gd, int level) {
     AGeom G = make_apply_geom(Gd, level);
     if (Gd.lv[level].kind == LK_WEIGHTS) G.xchunk = kWeightsXChunk;
@@ -438,9 +771,65 @@ void launch_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int n
     LP_HIP(hipGetLastError());
 }
 
+// wave-tile kernel: 15 x 15 node tiles per wave, kWaveTiles waves per block
+AGeom wave_geom(const Grid& Gd, int level) {
+    AGeom G = make_apply_geom(Gd, level);
+    if (Gd.lv[level].kind == LK_WEIGHTS) G.xchunk = kWeightsXChunk;
+    const int nty = (G.Ny + WN - 1) / WN, ntz = (G.Nz + WN - 1) / WN;
+    G.ntz = ntz;
+    G.nyzb = nty * ntz;
+    G.nxc = (G.Nx + G.xchunk - 1) / G.xchunk;
+    G.nwt = G.nyzb * G.nxc;
+    G.nblk = (G.nwt + kWaveTiles - 1) / kWaveTiles;
+    G.per = (G.nblk + 7) / 8;
+    return G;
+}
+
+template <bool WTS>
+void launch_wave(const Grid& Gd, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,
+                 const MfmaSrc& S, hipStream_t s) {
+    const Level& lv = Gd.lv[level];
+    const AGeom G = wave_geom(Gd, level);
+    if ((size_t)G.Nx * G.Ny * G.Nz * 3 >= (1ull << 30)) {
+        set_error("level operator: too many nodes for 32-bit buffer offsets");
+        throw ArgFailure();
+    }
+    dim3 grid(8 * G.per, 1, nrhs), block(kThreads);
+    const float* uf = static_cast<const float*>(u);
+    const uint8_t* m = lv.mask.as<uint8_t>();
+#define LP_WAVE_CASE(K) \
+    case K: hipLaunchKernelGGL((k_apply_wave<K, WTS>), grid, block, 0, s, uf, vstride, m, G, S, E); break;
+    switch (E.kind) {
+        LP_WAVE_CASE(EPI_STORE)
+        LP_WAVE_CASE(EPI_STORE_DOT)
+        LP_WAVE_CASE(EPI_CHEB)
+        LP_WAVE_CASE(EPI_RESID_SCALED)
+        LP_WAVE_CASE(EPI_RESID_MASK)
+        LP_WAVE_CASE(EPI_JACOBI)
+        LP_WAVE_CASE(EPI_POWER)
+        LP_WAVE_CASE(EPI_RESID)
+        LP_WAVE_CASE(EPI_MASKED)
+        LP_WAVE_CASE(EPI_CHEB_FIRST)
+        LP_WAVE_CASE(EPI_CHEB_ONLY)
+        LP_WAVE_CASE(EPI_CHEB_LAST)
+        default:
+            set_error("level operator: unknown epilogue");
+            throw ArgFailure();
+    }
+#undef LP_WAVE_CASE
+    LP_HIP(hipGetLastError());
+}
+
 }  // namespace
 
-int mfma_blocks(const Grid& Gd, int level) { return 8 * mfma_geom(Gd, level).per; }
+// the wave-tile kernel on the fine level (tune_wave 1) or on both levels (2)
+static bool use_wave(const Grid& Gd, int level) {
+    return Gd.tune_wave == 2 || (Gd.tune_wave == 1 && Gd.lv[level].kind == LK_FINE);
+}
+
+int mfma_blocks(const Grid& Gd, int level) {
+    return 8 * (use_wave(Gd, level) ? wave_geom(Gd, level).per : mfma_geom(Gd, level).per);
+}
 
 bool mfma_level(const Grid& G, int level, int eb) {
     return G.fp16() && eb == 4 && (G.lv[level].kind == LK_FINE || G.lv[level].kind == LK_WEIGHTS);
@@ -455,7 +844,10 @@ void launch_apply_fine_mfma(const Grid& Gd, int level, const void* u, size_t vst
     MfmaSrc S{};
     S.e = Gd.lv[level].e32.as<float>();
     S.a = Gd.ke32.as<float>();
-    launch_mfma<false>(Gd, level, u, vstride, nrhs, E, S, s);
+    if (use_wave(Gd, level))
+        launch_wave<false>(Gd, level, u, vstride, nrhs, E, S, s);
+    else
+        launch_mfma<false>(Gd, level, u, vstride, nrhs, E, S, s);
 }
 
 void launch_apply_weights_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int nrhs,
@@ -481,7 +873,10 @@ void launch_apply_weights_mfma(const Grid& Gd, int level, const void* u, size_t 
                            lv.patch_prod.as<float>());
         LP_HIP(hipGetLastError());
     }
-    launch_mfma<true>(Gd, level, u, vstride, nrhs, E, S, s);
+    if (use_wave(Gd, level))
+        launch_wave<true>(Gd, level, u, vstride, nrhs, E, S, s);
+    else
+        launch_mfma<true>(Gd, level, u, vstride, nrhs, E, S, s);
 }
 
 }  // namespace lp

@@ -153,6 +153,7 @@ struct Grid {
     int tune_fine_xchunk = 16;  // fine-level nodes per thread along x
     int tune_fused_cheb = 1;    // MFMA levels: Chebyshev vector passes fused into the operator
     int tune_graph = 1;         // PCG replays the V-cycle from a captured hipGraph
+    int tune_wave = 1;          // FP16-MFMA levels: wave-tile kernel on the fine level (1), on both (2), on none (0)
     int tune_flex = -1;         // flexible (Polak-Ribiere) beta: 1 on, 0 off, -1 on for the mixed path only
     bool flex() const { return tune_flex < 0 ? opt.precision == LP_PREC_MIXED : tune_flex != 0; }
     // mixed path: a solve the FP16 levels fail is repeated with FP32

@@ -895,6 +895,13 @@ int lp_tune(lp_grid* g, int param, int value) {
         return old;
     }
     if (param == 6) return g->fallbacks;
+    if (param == 7) {
+        if (value < 0 || value > 2) return LP_ERR_ARG;
+        const int old = g->tune_wave;
+        g->tune_wave = value;
+        ++g->epoch;
+        return old;
+    }
     return LP_ERR_ARG;
 }
 
@@ -903,7 +910,7 @@ int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best) {
         if (!g || !g->built || nrhs < 1 || nrhs > g->max_rhs || reps < 1) return LP_ERR_ARG;
         Grid& G = *g;
         const int eb = G.vbytes();
-        const int cand[] = {4, 8, 16, 32};
+        const int cand[] = {4, 8, 12, 16, 24, 32};
         int best = G.tune_fine_xchunk;
         double tbest = 1e300;
         for (int c : cand) {
