@@ -309,18 +309,19 @@ __device__ __forceinline__ int bload_u8(__amdgpu_buffer_rsrc_t r, unsigned off) 
 // ---------------------------------------------------------------------------
 // A block owns 256 consecutive nodes of the flattened (y, z) plane (one per
 // thread) and an x-chunk. Three x-planes of its range plus a halo of Nz+1
-// nodes on either side are staged in LDS (masked u, 3 components padded to
-// 4); the plane two layers ahead is loaded into registers while the current
+// nodes on either side are staged in LDS (masked u, component-major, so a
+// wave's reads of one component are consecutive words: no bank conflicts);
+// the plane two layers ahead is loaded into registers while the current
 // layer computes, then stored, with one barrier per layer. Each thread takes
 // its 4 cells of the layer one at a time (24 values live): the lo rows
 // finish the node below the layer, the hi rows start the node above.
 template <typename T>
 struct FineSmem {
-    static constexpr int kPad = 4;
+    static constexpr int kComps = 3;   // staged words per node and plane
 };
 
 template <typename T, bool SYM>
-__global__ void __launch_bounds__(256) k_apply_fine(const T* __restrict__ u0, size_t vstride,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))) k_apply_fine(const T* __restrict__ u0, size_t vstride,
                                                     const uint8_t* __restrict__ mask, AGeom G,
                                                     const T* __restrict__ ecell, const T* __restrict__ ke,
                                                     KeVals<T> kv, EpiArgs E) {
@@ -348,8 +349,7 @@ __global__ void __launch_bounds__(256) k_apply_fine(const T* __restrict__ u0, si
         const unsigned nnu = (unsigned)G.nn;
         const int halo = Nz + 1;
         const int W = (int)blockDim.x + 2 * halo;         // staged nodes per plane
-        constexpr int PAD = FineSmem<T>::kPad;
-        const int planeT = W * PAD;
+        const int planeT = W * FineSmem<T>::kComps;   // component c of node e at c * W + e
         const __amdgpu_buffer_rsrc_t ru = make_rsrc(u, 3u * nnu * (unsigned)sizeof(T));
         const __amdgpu_buffer_rsrc_t rm = make_rsrc(mask, nnu);
         const __amdgpu_buffer_rsrc_t re = make_rsrc(ecell, (unsigned)G.ncell * (unsigned)sizeof(T));
@@ -380,7 +380,7 @@ __global__ void __launch_bounds__(256) k_apply_fine(const T* __restrict__ u0, si
                 const int e = (int)threadIdx.x + k * (int)blockDim.x;
                 if (e < W) {
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) p[e * PAD + c] = pre[k][c];
+                    for (int c = 0; c < 3; ++c) p[c * W + e] = pre[k][c];
                 }
             }
         };
@@ -399,43 +399,53 @@ __global__ void __launch_bounds__(256) k_apply_fine(const T* __restrict__ u0, si
 #pragma unroll
                 for (int cz = 0; cz < 2; ++cz) {
                     // one cell's 24 values live at a time: keep the scheduler
-                    // from hoisting the next cell's LDS reads above this one
+                    // from hoisting the next cell's LDS reads above this one,
+                    // or sinking this cell's arithmetic below them
                     asm volatile("" ::: "memory");
                     const bool ok = inx && cyv[cy] && czv[cz];
                     const unsigned ci = (unsigned)((X * G.ny + (y - 1 + cy)) * G.nz + (z - 1 + cz));
                     const T ec = bload<T>(re, ok ? ci * (unsigned)sizeof(T) : kOOB);
-                    T ub[24];
-#pragma unroll
-                    for (int b = 0; b < 8; ++b) {
-                        const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
-                        const int idx = me + (cy + by - 1) * Nz + (cz + bz - 1);
-                        const T* q = (bx ? PH : PL) + idx * PAD;
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) ub[3 * b + c] = q[c];
-                    }
                     if (!SYM) asm volatile("" : "+s"(kea));
                     cptr<T> Kp = (cptr<T>)kea;
                     auto K = [&](int e) -> T {
                         if (SYM) return KE_SGN[e] > 0 ? kv.v[KE_IDX[e]] : (KE_SGN[e] < 0 ? -kv.v[KE_IDX[e]] : T(0));
                         return Kp[e];
                     };
+                    // corner by corner (3 operand values live), each row's
+                    // sum still in the order j = 0..23
+                    T tr[2][3] = {{0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
+                        const int idx = me + (cy + by - 1) * Nz + (cz + bz - 1);
+                        const T* q = (bx ? PH : PL) + idx;
+                        T u3[3];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) u3[c] = q[c * W];
+#pragma unroll
+                        for (int side = 0; side < 2; ++side) {
+                            if (side == 0 && !lo) continue;
+                            const int a = side | ((1 - cy) << 1) | ((1 - cz) << 2);
+#pragma unroll
+                            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                                for (int c = 0; c < 3; ++c) tr[side][r] += K((3 * a + r) * 24 + 3 * b + c) * u3[c];
+                        }
+                    }
 #pragma unroll
                     for (int side = 0; side < 2; ++side) {
                         if (side == 0 && !lo) continue;
-                        const int a = side | ((1 - cy) << 1) | ((1 - cz) << 2);
-                        T t0 = 0, t1 = 0, t2 = 0;
-#pragma unroll
-                        for (int j = 0; j < 24; ++j) {
-                            t0 += K((3 * a + 0) * 24 + j) * ub[j];
-                            t1 += K((3 * a + 1) * 24 + j) * ub[j];
-                            t2 += K((3 * a + 2) * 24 + j) * ub[j];
-                        }
+                        const T t0 = tr[side][0], t1 = tr[side][1], t2 = tr[side][2];
                         if (side == 0) {
                             acc0 += ec * t0; acc1 += ec * t1; acc2 += ec * t2;
                         } else {
                             nx0 += ec * t0; nx1 += ec * t1; nx2 += ec * t2;
                         }
                     }
+                    // the cell is finished here: without this tie the
+                    // compiler interleaves the 4 cells of the layer and runs
+                    // out of registers (one wave per SIMD)
+                    asm volatile("" : "+v"(acc0), "+v"(acc1), "+v"(acc2), "+v"(nx0), "+v"(nx1), "+v"(nx2));
                 }
         };
 
@@ -460,12 +470,12 @@ __global__ void __launch_bounds__(256) k_apply_fine(const T* __restrict__ u0, si
                 const size_t n = (size_t)X * nyz + t;
                 const int m = mask[n];
                 T yv[3] = {acc0, acc1, acc2};
-                const T* q = lds + bl * planeT + ((int)threadIdx.x + halo) * PAD;
+                const T* q = lds + bl * planeT + ((int)threadIdx.x + halo);
                 T ur[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const bool mc = (m >> c) & 1;
-                    ur[c] = (!mc || need_raw_all) ? u[c * (size_t)nnu + n] : q[c];
+                    ur[c] = (!mc || need_raw_all) ? u[c * (size_t)nnu + n] : q[c * W];
                     if (!mc) yv[c] = ur[c];
                 }
                 epilogue<T>(E, m, (size_t)rhs * vstride + n, G.nn, n, yv, ur, part);
@@ -598,7 +608,7 @@ void launch_kind(const Grid& Gd, int level, const T* u, size_t vstride, int nrhs
             set_error("fine level: z extent too large for the staged plane");
             throw ArgFailure();
         }
-        const size_t smem = (size_t)3 * W * FineSmem<T>::kPad * sizeof(T);
+        const size_t smem = (size_t)3 * W * FineSmem<T>::kComps * sizeof(T);
         if (Gd.ke_sym)
             hipLaunchKernelGGL((k_apply_fine<T, true>), grid, block, smem, s, u, vstride, m, G, e, ke, kv, E);
         else

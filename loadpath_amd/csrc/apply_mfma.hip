@@ -360,35 +360,46 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
 }
 
 // patch cells of the weights level: (patch, row) per thread, the explicit
-// 24x24 patch matrix times the cell's masked corner vector (FP32); with
-// dinv set, the operand is (dinv u) s, as the fused first Chebyshev step
-// stages it
-__global__ void __launch_bounds__(kBlock) k_patch_prod(const float* __restrict__ u0, size_t vstride,
+// 24x24 patch matrix times the cell's masked corner vector (FP32) for every
+// right-hand side of the batch (the matrix row, masks and indices are read
+// once); with dinv set, the operand is (dinv u) s, as the fused first
+// Chebyshev step stages it
+__global__ void __launch_bounds__(kBlock) k_patch_prod(const float* __restrict__ u0, size_t vstride, int nrhs,
                                                        const uint8_t* __restrict__ mask, Geom g,
                                                        const int* __restrict__ cells, const float* __restrict__ pm,
                                                        int npatch, const float* __restrict__ dinv, float s,
                                                        float* __restrict__ out) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int rhs = blockIdx.y;
     if (t >= npatch * 24) return;
     const int p = t / 24, row = t - 24 * (t / 24);
     const int cell = cells[p];
     const int Z = cell % g.nz, Y = (cell / g.nz) % g.ny, X = cell / (g.nz * g.ny);
-    const float* u = u0 + (size_t)rhs * vstride;
     const float* m = pm + (size_t)p * 576 + row * 24;
-    float acc = 0.0f;
+    // per operand: its matrix entry (0 for a masked-out dof), offset and scale
+    float w[24], f[24];
+    size_t off[24];
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
         const size_t n = ((size_t)(X + (b & 1)) * g.Ny + (Y + ((b >> 1) & 1))) * g.Nz + (Z + ((b >> 2) & 1));
         const int mk = mask[n];
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-            if ((mk >> c) & 1) {
-                const float v = u[c * g.nn + n];
-                acc = fmaf(m[3 * b + c], dinv ? (dinv[c * g.nn + n] * v) * s : v, acc);
-            }
+        for (int c = 0; c < 3; ++c) {
+            const bool on = (mk >> c) & 1;
+            w[3 * b + c] = on ? m[3 * b + c] : 0.0f;
+            off[3 * b + c] = c * g.nn + n;
+            f[3 * b + c] = (on && dinv) ? dinv[c * g.nn + n] : 1.0f;
+        }
     }
-    out[((size_t)rhs * npatch + p) * 24 + row] = acc;
+    for (int rhs = 0; rhs < nrhs; ++rhs) {
+        const float* u = u0 + (size_t)rhs * vstride;
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 24; ++k) {
+            const float v = w[k] != 0.0f ? u[off[k]] : 0.0f;
+            acc = fmaf(w[k], dinv ? (f[k] * v) * s : v, acc);
+        }
+        out[((size_t)rhs * npatch + p) * 24 + row] = acc;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -867,7 +878,7 @@ void launch_apply_weights_mfma(const Grid& Gd, int level, const void* u, size_t 
         S.pprod = lv.patch_prod.as<float>();
         const unsigned nb = (unsigned)((lv.npatch * 24 + kBlock - 1) / kBlock);
         const bool staged = E.kind == EPI_CHEB_FIRST || E.kind == EPI_CHEB_ONLY;
-        hipLaunchKernelGGL(k_patch_prod, dim3(nb, nrhs), dim3(kBlock), 0, s, static_cast<const float*>(u), vstride,
+        hipLaunchKernelGGL(k_patch_prod, dim3(nb), dim3(kBlock), 0, s, static_cast<const float*>(u), vstride, nrhs,
                            lv.mask.as<uint8_t>(), lv.g, lv.patch_cell.as<int>(), lv.patch32.as<float>(), lv.npatch,
                            staged ? static_cast<const float*>(E.dinv) : nullptr, (float)E.c3,
                            lv.patch_prod.as<float>());
