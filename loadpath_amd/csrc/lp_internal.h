@@ -134,8 +134,7 @@ struct Grid {
     DBuf cinv;                  // ncoarse^2 doubles
     DBuf cwork;                 // factor workspace
     // CG vectors (FP64), max_rhs each
-    DBuf x, r, p, ap, z, b;
-    DBuf rv, zv;                // V-cycle precision copies of r and z at level 0
+    DBuf x, r, p, ap, b;        // FP64 CG state (z = M r is never stored in FP64)
     // reductions
     DBuf partials;              // doubles
     size_t partials_cap = 0;
@@ -192,6 +191,7 @@ struct Grid {
 struct CgScal {
     double rz, pap, alpha, beta, rr, bnorm, tol, tmp;
     int active, converged, failed, iters;
+    int xpend;   // alpha of this iteration not yet added to x (x += alpha p)
 };
 
 // ---- element.cpp -------------------------------------------------------
@@ -278,8 +278,10 @@ void launch_axpy_inplace(void* x, const void* d, int elem_bytes, size_t n, size_
 void launch_jacobi_start(const void* b, const void* dinv, double w, void* x, int elem_bytes,
                          size_t n, size_t stride, int nrhs, hipStream_t st);
 // x += a p; r -= a ap; rv = r in the V-cycle precision (vb bytes); partial rr
+// r -= alpha Ap with |r|^2 partials and rv = r in the V-cycle precision
 void launch_cg_step(Grid& G, int nrhs, void* rv, int vb, hipStream_t s);
-void launch_xpby(Grid& G, int nrhs, hipStream_t s);      // p = z + beta p
+// x += alpha p (pending rhs), then p = z + beta p (active rhs) unless xonly
+void launch_cg_update(Grid& G, int nrhs, const void* z, int zb, bool xonly, hipStream_t s);
 // finalize kernels on CgScal
 void launch_cg_finalize(Grid& G, int nrhs, int stage, int nb, hipStream_t s);
 // warm start: which 0 = partials of |b - s Ax|^2 (Ax in G.ap), 1 = apply the decision

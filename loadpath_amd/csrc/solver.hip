@@ -383,16 +383,18 @@ static void pcg_run(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
         launch_fill(G.x.p, 8, n * nrhs, 0.0, s);
         launch_cast(G.b.p, 8, G.r.p, 8, n, n, nrhs, s);
     }
-    // z = M r with the cast back fused into the r . z sums; later iterations
-    // get the V-cycle input from the CG step itself
+    // z = M r: the V-cycle's result is only summed against r (and Ap for
+    // the flexible beta) and folded into p, never stored in FP64; later
+    // iterations get the V-cycle input from the CG step itself
     Level& L0 = G.lv[0];
     const int eb = G.vbytes();
     launch_cast(G.r.p, 8, L0.vb.p, eb, n, n, nrhs, s);
     vcycle_run(G, nrhs);
-    launch_cast_dot(L0.vx.p, eb, G.r.as<double>(), G.z.as<double>(), n, n, nrhs, P, S, s);
+    launch_cast_dot(L0.vx.p, eb, G.r.as<double>(), nullptr, n, n, nrhs, P, S, s);
     launch_cg_finalize(G, nrhs, 4, nbd, s);
-    launch_cast(G.z.p, 8, G.p.p, 8, n, n, nrhs, s);
+    launch_cast(L0.vx.p, eb, G.p.p, 8, n, n, nrhs, s);
     std::vector<CgScal> h;
+    bool pending = false;   // alpha of the last iteration not yet added to x
     for (int it = 1; it <= G.opt.maxiter; ++it) {
         EpiArgs E;
         E.kind = EPI_STORE_DOT;
@@ -403,20 +405,24 @@ static void pcg_run(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
         launch_cg_finalize(G, nrhs, 0, nba, s);
         launch_cg_step(G, nrhs, L0.vb.p, eb, s);
         launch_cg_finalize(G, nrhs, 1, nbd, s);
+        pending = true;
         read_scal(G, nrhs, h);
         bool any = false;
         for (int k = 0; k < nrhs; ++k) any |= h[k].active != 0;
         if (!any) break;
         vcycle_run(G, nrhs);
         if (flex) {
-            launch_cast_dot(L0.vx.p, eb, G.r.as<double>(), G.z.as<double>(), n, n, nrhs, P, S, s, G.ap.as<double>());
+            launch_cast_dot(L0.vx.p, eb, G.r.as<double>(), nullptr, n, n, nrhs, P, S, s, G.ap.as<double>());
             launch_cg_beta_flex(G, nrhs, nbd, s);
         } else {
-            launch_cast_dot(L0.vx.p, eb, G.r.as<double>(), G.z.as<double>(), n, n, nrhs, P, S, s);
+            launch_cast_dot(L0.vx.p, eb, G.r.as<double>(), nullptr, n, n, nrhs, P, S, s);
             launch_cg_finalize(G, nrhs, 2, nbd, s);
         }
-        launch_xpby(G, nrhs, s);
+        // x += alpha p with this iteration's p, then p = z + beta p
+        launch_cg_update(G, nrhs, L0.vx.p, eb, false, s);
+        pending = false;
     }
+    if (pending) launch_cg_update(G, nrhs, L0.vx.p, eb, true, s);
     read_scal(G, nrhs, h);
     for (int k = 0; k < nrhs; ++k) {
         iters[k] = h[k].iters + (h[k].failed ? 1 : 0);
@@ -578,7 +584,6 @@ int lp_grid_create(lp_grid** out, int nx, int ny, int nz, const uint8_t* fixed_g
             G->r.alloc(vb);
             G->p.alloc(vb);
             G->ap.alloc(vb);
-            G->z.alloc(vb);
             G->b.alloc(vb);
             G->scal.alloc(G->max_rhs * sizeof(CgScal));
             LP_HIP(hipMemset(G->scal.p, 0, G->max_rhs * sizeof(CgScal)));

@@ -112,9 +112,8 @@ __global__ void k_jacobi_start(const T* __restrict__ b, const T* __restrict__ di
 
 // x += alpha p; r -= alpha ap; partial |r|^2   (mgops._cg_step, mgops.py:346-354)
 template <typename V>
-__global__ void k_cg_step(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ p,
-                          const double* __restrict__ ap, size_t n, size_t stride, const CgScal* scal,
-                          double* partials, V* __restrict__ rv) {
+__global__ void k_cg_step(double* __restrict__ r, const double* __restrict__ ap, size_t n, size_t stride,
+                          const CgScal* scal, double* partials, V* __restrict__ rv) {
     __shared__ double sred[4];
     const int rhs = blockIdx.y;
     double acc = 0;
@@ -122,7 +121,6 @@ __global__ void k_cg_step(double* __restrict__ x, double* __restrict__ r, const 
         const double a = scal[rhs].alpha;
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
             const size_t k = rhs * stride + i;
-            x[k] += a * p[k];
             const double ri = r[k] - a * ap[k];
             r[k] = ri;
             rv[k] = (V)ri;
@@ -133,15 +131,23 @@ __global__ void k_cg_step(double* __restrict__ x, double* __restrict__ r, const 
     if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = s;
 }
 
-// p = z + beta p   (mgops._xpby, mgops.py:357-360)
-__global__ void k_xpby(const double* __restrict__ z, double* __restrict__ p, size_t n, size_t stride,
-                       const CgScal* scal) {
+// x += alpha p with the direction of this iteration (rhs whose alpha is
+// pending), then p = z + beta p (mgops._xpby, mgops.py:357-360) for the
+// active ones, z = the V-cycle's result read in its own precision (never
+// stored in FP64); xonly: the x update alone (the last iteration)
+template <typename S>
+__global__ void k_cg_update(double* __restrict__ x, double* __restrict__ p, const S* __restrict__ z, size_t n,
+                            size_t stride, const CgScal* scal, int xonly) {
     const int rhs = blockIdx.y;
-    if (!scal[rhs].active) return;
-    const double beta = scal[rhs].beta;
+    const bool upd = scal[rhs].xpend != 0;
+    const bool dir = !xonly && scal[rhs].active;
+    if (!upd && !dir) return;
+    const double a = scal[rhs].alpha, beta = scal[rhs].beta;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const size_t k = rhs * stride + i;
-        p[k] = z[k] + beta * p[k];
+        const double pk = p[k];
+        if (upd) x[k] += a * pk;
+        if (dir) p[k] = (double)z[k] + beta * pk;
     }
 }
 
@@ -157,6 +163,7 @@ __global__ void k_cg_finalize(const double* __restrict__ partials, int nb, CgSca
     CgScal& c = scal[rhs];
     switch (stage) {
         case 0:  // pap = p.Ap -> alpha
+            c.xpend = 0;
             if (!c.active) break;
             c.pap = s;
             if (!(s > 0)) {
@@ -164,6 +171,7 @@ __global__ void k_cg_finalize(const double* __restrict__ partials, int nb, CgSca
                 c.active = 0;
             } else {
                 c.alpha = c.rz / s;
+                c.xpend = 1;
             }
             break;
         case 1:  // rr -> convergence
@@ -190,6 +198,7 @@ __global__ void k_cg_finalize(const double* __restrict__ partials, int nb, CgSca
             c.rz = 0;
             c.alpha = 0;
             c.beta = 0;
+            c.xpend = 0;
             break;
         case 4:  // rz = r.z
             if (!c.active) break;
@@ -255,7 +264,7 @@ __global__ void k_cast_dot(const S* __restrict__ v, const double* __restrict__ r
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const size_t k = rhs * stride + i;
         const double zi = (double)v[k];
-        z[k] = zi;
+        if (z) z[k] = zi;
         if (on) {
             acc += r[k] * zi;
             if (ap) acc2 += ap[k] * zi;
@@ -414,20 +423,22 @@ void launch_cg_step(Grid& G, int nrhs, void* rv, int vb, hipStream_t s) {
     const size_t n = 3 * G.g0.nn;
     const int nb = dot_blocks(n);
     if (vb == 4)
-        hipLaunchKernelGGL(k_cg_step<float>, dim3(nb, nrhs), dim3(kBlock), 0, s, G.x.as<double>(), G.r.as<double>(),
-                           G.p.as<double>(), G.ap.as<double>(), n, n, G.scal.as<CgScal>(), G.partials.as<double>(),
-                           static_cast<float*>(rv));
+        hipLaunchKernelGGL(k_cg_step<float>, dim3(nb, nrhs), dim3(kBlock), 0, s, G.r.as<double>(), G.ap.as<double>(),
+                           n, n, G.scal.as<CgScal>(), G.partials.as<double>(), static_cast<float*>(rv));
     else
-        hipLaunchKernelGGL(k_cg_step<double>, dim3(nb, nrhs), dim3(kBlock), 0, s, G.x.as<double>(), G.r.as<double>(),
-                           G.p.as<double>(), G.ap.as<double>(), n, n, G.scal.as<CgScal>(), G.partials.as<double>(),
-                           static_cast<double*>(rv));
+        hipLaunchKernelGGL(k_cg_step<double>, dim3(nb, nrhs), dim3(kBlock), 0, s, G.r.as<double>(), G.ap.as<double>(),
+                           n, n, G.scal.as<CgScal>(), G.partials.as<double>(), static_cast<double*>(rv));
     LP_HIP(hipGetLastError());
 }
 
-void launch_xpby(Grid& G, int nrhs, hipStream_t s) {
+void launch_cg_update(Grid& G, int nrhs, const void* z, int zb, bool xonly, hipStream_t s) {
     const size_t n = 3 * G.g0.nn;
-    hipLaunchKernelGGL(k_xpby, dim3(grid_for(n), nrhs), dim3(kBlock), 0, s, G.z.as<double>(), G.p.as<double>(), n,
-                       n, G.scal.as<CgScal>());
+    if (zb == 4)
+        hipLaunchKernelGGL(k_cg_update<float>, dim3(grid_for(n), nrhs), dim3(kBlock), 0, s, G.x.as<double>(),
+                           G.p.as<double>(), static_cast<const float*>(z), n, n, G.scal.as<CgScal>(), xonly ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_cg_update<double>, dim3(grid_for(n), nrhs), dim3(kBlock), 0, s, G.x.as<double>(),
+                           G.p.as<double>(), static_cast<const double*>(z), n, n, G.scal.as<CgScal>(), xonly ? 1 : 0);
     LP_HIP(hipGetLastError());
 }
 
