@@ -42,9 +42,11 @@ __global__ void __launch_bounds__(kBlock) k_apply_explicit_batched(const float* 
     bool has = false;
     int X = 0, Y = 0, Z = 0, a = 0;
     if (valid) {
-        z = (int)(nid % G.Nz);
-        y = (int)((nid / G.Nz) % G.Ny);
-        x = (int)(nid / ((size_t)G.Nz * G.Ny));
+        const unsigned id = (unsigned)nid, nyz = (unsigned)G.Ny * (unsigned)G.Nz;
+        x = (int)(id / nyz);
+        const unsigned rem = id - (unsigned)x * nyz;
+        y = (int)(rem / (unsigned)G.Nz);
+        z = (int)(rem - (unsigned)y * (unsigned)G.Nz);
         const int cx = c8 & 1, cy = (c8 >> 1) & 1, cz = (c8 >> 2) & 1;
         X = x - 1 + cx;
         Y = y - 1 + cy;

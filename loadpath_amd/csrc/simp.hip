@@ -85,7 +85,9 @@ __global__ void k_sens(const double* __restrict__ u, const double* __restrict__ 
     const size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double part = 0;
     if (cell < g.ncell) {
-        const int Z = (int)(cell % g.nz), Y = (int)((cell / g.nz) % g.ny), X = (int)(cell / ((size_t)g.nz * g.ny));
+        const unsigned ci = (unsigned)cell, nyz = (unsigned)g.ny * (unsigned)g.nz;
+        const int X = (int)(ci / nyz), Y = (int)((ci - (unsigned)X * nyz) / (unsigned)g.nz);
+        const int Z = (int)(ci - (unsigned)X * nyz - (unsigned)Y * (unsigned)g.nz);
         double ce = 0;
         for (int r = 0; r < nrhs; ++r) {
             const double* uu = u + r * vstride;
@@ -125,7 +127,9 @@ __global__ void k_filter(const double* __restrict__ v, const double* __restrict_
     const size_t n = (size_t)nx * ny * nz;
     double part = 0;
     if (cell < n) {
-        const int z = (int)(cell % nz), y = (int)((cell / nz) % ny), x = (int)(cell / ((size_t)nz * ny));
+        const unsigned ci = (unsigned)cell, nyz = (unsigned)ny * (unsigned)nz;
+        const int x = (int)(ci / nyz), y = (int)((ci - (unsigned)x * nyz) / (unsigned)nz);
+        const int z = (int)(ci - (unsigned)x * nyz - (unsigned)y * (unsigned)nz);
         double acc = 0;
         for (int t = 0; t < ntaps; ++t) {
             const Tap tp = taps[t];
@@ -173,14 +177,9 @@ __global__ void k_mma_inputs(const double* __restrict__ dc, const double* __rest
 }
 
 // xnew = max(0, max(x - move, min(1, min(x + move, x sqrt(-dc/dv/lmid)))))
-__global__ void k_oc(const double* __restrict__ x, const double* __restrict__ dc, const double* __restrict__ dv,
-                     const uint8_t* __restrict__ passive, size_t n, double move, Bisect* bis, double* xnew) {
-    if (bis->done) return;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0 && threadIdx.x == 0) {
-    }
-    if (i >= n) return;
-    const double lmid = 0.5 * (bis->l2 + bis->l1);
+__device__ __forceinline__ double oc_value(const double* __restrict__ x, const double* __restrict__ dc,
+                                          const double* __restrict__ dv, const uint8_t* __restrict__ passive,
+                                          size_t i, double move, double lmid) {
     const double xi = x[i];
     double v = xi * sqrt(-dc[i] / dv[i] / lmid);
     v = fmin(xi + move, v);
@@ -188,7 +187,39 @@ __global__ void k_oc(const double* __restrict__ x, const double* __restrict__ dc
     v = fmax(xi - move, v);
     v = fmax(0.0, v);
     if (passive && passive[i]) v = 0.0;
-    xnew[i] = v;
+    return v;
+}
+
+// the design of the bisection's last evaluated multiplier
+__global__ void k_oc(const double* __restrict__ x, const double* __restrict__ dc, const double* __restrict__ dv,
+                     const uint8_t* __restrict__ passive, size_t n, double move, const Bisect* bis, double* xnew) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    xnew[i] = oc_value(x, dc, dv, passive, i, move, bis->lmid);
+}
+
+// one bisection evaluation at lmid = (l1 + l2) / 2: block partials of the
+// physical volume sum xphys = sum_c xnew_c W_c, W = H (1/hs) over the
+// non-passive cells (H is symmetric), so no filter pass is needed per
+// evaluation; nothing is stored
+__global__ void k_oc_volume(const double* __restrict__ x, const double* __restrict__ dc,
+                            const double* __restrict__ dv, const uint8_t* __restrict__ passive,
+                            const double* __restrict__ w, size_t n, double move, const Bisect* bis,
+                            double* partials) {
+    __shared__ double sred[4];
+    if (bis->done) return;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double lmid = 0.5 * (bis->l2 + bis->l1);
+    const double part = i < n ? oc_value(x, dc, dv, passive, i, move, lmid) * w[i] : 0.0;
+    const double r = bsum(part, sred);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// t = 1/hs on the non-passive cells, 0 on the passive ones
+__global__ void k_mask_passive(const double* __restrict__ hs_inv, const uint8_t* __restrict__ passive, size_t n,
+                               double* t) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) t[i] = passive[i] ? 0.0 : hs_inv[i];
 }
 
 __global__ void k_bisect_finalize(const double* __restrict__ partials, int nb, Bisect* bis, double tol) {
@@ -264,6 +295,8 @@ struct lp_simp {
     int ntaps = 0;
     double target = 0;
     DBuf taps, x, xnew, xphys, dc, dv, ce, hs_inv, ones, passive, loads, bis, partials, scalar, chg;
+    DBuf dv0;    // H (1/hs): the filtered volume sensitivity (constant)
+    DBuf wvol;   // H (1/hs on the non-passive cells): weights of the volume sum
     bool has_passive = false;
     // optimizer: 0 optimality criteria with the volume bisection, 1 MMA
     int optimizer = 0;
@@ -335,6 +368,25 @@ int lp_simp_create(lp_simp** out, lp_grid* g, int nrhs, const double* loads, con
                 double full = 0;
                 for (const Tap& t : kept) full += t.w;
                 launch_fill(S->hs_inv.p, 8, n, 1.0 / full, s);
+            }
+            // constant filtered fields: dv = H (1/hs) and the volume weights
+            S->dv0.alloc(n * 8);
+            S->wvol.alloc(n * 8);
+            hipLaunchKernelGGL(k_filter, dim3(nblk(n)), dim3(kBlock), 0, s, S->ones.as<double>(), S->hs_inv.as<double>(),
+                               (const double*)nullptr, (const uint8_t*)nullptr, S->taps.as<Tap>(), S->ntaps, g0.nx, g0.ny,
+                               g0.nz, S->dv0.as<double>(), (double*)nullptr, (const Bisect*)nullptr);
+            LP_HIP(hipGetLastError());
+            if (S->has_passive) {
+                hipLaunchKernelGGL(k_mask_passive, dim3(nblk(n)), dim3(kBlock), 0, s, S->hs_inv.as<double>(),
+                                   S->passive.as<uint8_t>(), n, S->xnew.as<double>());
+                LP_HIP(hipGetLastError());
+                hipLaunchKernelGGL(k_filter, dim3(nblk(n)), dim3(kBlock), 0, s, S->xnew.as<double>(),
+                                   (const double*)nullptr, (const double*)nullptr, (const uint8_t*)nullptr,
+                                   S->taps.as<Tap>(), S->ntaps, g0.nx, g0.ny, g0.nz, S->wvol.as<double>(),
+                                   (double*)nullptr, (const Bisect*)nullptr);
+                LP_HIP(hipGetLastError());
+            } else {
+                LP_HIP(hipMemcpyAsync(S->wvol.p, S->dv0.p, n * 8, hipMemcpyDeviceToDevice, s));
             }
             const size_t vn = 3 * g0.nn;
             S->loads.alloc(vn * nrhs * 8);
@@ -409,10 +461,7 @@ int lp_simp_iterate(lp_simp* S, int niter, lp_simp_stats* stats) {
                                g0.nz, S->xnew.as<double>(), (double*)nullptr, (const Bisect*)nullptr);
             LP_HIP(hipGetLastError());
             std::swap(S->dc.p, S->xnew.p);
-            hipLaunchKernelGGL(k_filter, dim3(nb), dim3(kBlock), 0, s, S->ones.as<double>(), S->hs_inv.as<double>(),
-                               (const double*)nullptr, (const uint8_t*)nullptr, S->taps.as<Tap>(), S->ntaps, g0.nx, g0.ny,
-                               g0.nz, S->dv.as<double>(), (double*)nullptr, (const Bisect*)nullptr);
-            LP_HIP(hipGetLastError());
+            LP_HIP(hipMemcpyAsync(S->dv.p, S->dv0.p, n * 8, hipMemcpyDeviceToDevice, s));
             if (S->optimizer == 1) {
                 // MMA on f0 = 10 c / c_ref with the volume constraint
                 // sum xphys / (vf n) - 1 <= 0, both gradients through the
@@ -461,15 +510,29 @@ int lp_simp_iterate(lp_simp* S, int niter, lp_simp_stats* stats) {
             Bisect b0{0.0, 1e9, 0.0, S->target, 0, 0, 0, 0};
             LP_HIP(hipMemcpyAsync(S->bis.p, &b0, sizeof(b0), hipMemcpyHostToDevice, s));
             const uint8_t* pas = S->has_passive ? S->passive.as<uint8_t>() : nullptr;
-            for (int e = 0; e < 100; ++e) {
-                hipLaunchKernelGGL(k_oc, dim3(nb), dim3(kBlock), 0, s, S->x.as<double>(), S->dc.as<double>(),
-                                   S->dv.as<double>(), pas, n, S->move, S->bis.as<Bisect>(), S->xnew.as<double>());
-                hipLaunchKernelGGL(k_filter, dim3(nb), dim3(kBlock), 0, s, S->xnew.as<double>(), (const double*)nullptr,
-                                   S->hs_inv.as<double>(), pas, S->taps.as<Tap>(), S->ntaps, g0.nx, g0.ny, g0.nz,
-                                   S->xphys.as<double>(), S->partials.as<double>(), S->bis.as<Bisect>());
-                hipLaunchKernelGGL(k_bisect_finalize, dim3(1), dim3(kBlock), 0, s, S->partials.as<double>(), (int)nb,
-                                   S->bis.as<Bisect>(), 1e-3);
+            // evaluations queued in rounds (they exit once the bracket has
+            // closed; the bracket 1e9 at relative 1e-3 takes ~50)
+            Bisect bq;
+            for (int e = 0; e < 100;) {
+                const int round = e == 0 ? 56 : 16;
+                for (int k = 0; k < round && e < 100; ++k, ++e) {
+                    hipLaunchKernelGGL(k_oc_volume, dim3(nb), dim3(kBlock), 0, s, S->x.as<double>(),
+                                       S->dc.as<double>(), S->dv.as<double>(), pas, S->wvol.as<double>(), n, S->move,
+                                       S->bis.as<Bisect>(), S->partials.as<double>());
+                    hipLaunchKernelGGL(k_bisect_finalize, dim3(1), dim3(kBlock), 0, s, S->partials.as<double>(),
+                                       (int)nb, S->bis.as<Bisect>(), 1e-3);
+                }
+                LP_HIP(hipGetLastError());
+                LP_HIP(hipMemcpyAsync(&bq, S->bis.p, sizeof(bq), hipMemcpyDeviceToHost, s));
+                LP_HIP(hipStreamSynchronize(s));
+                if (bq.done) break;
             }
+            // the design and its physical densities at the last evaluated multiplier
+            hipLaunchKernelGGL(k_oc, dim3(nb), dim3(kBlock), 0, s, S->x.as<double>(), S->dc.as<double>(),
+                               S->dv.as<double>(), pas, n, S->move, S->bis.as<Bisect>(), S->xnew.as<double>());
+            hipLaunchKernelGGL(k_filter, dim3(nb), dim3(kBlock), 0, s, S->xnew.as<double>(), (const double*)nullptr,
+                               S->hs_inv.as<double>(), pas, S->taps.as<Tap>(), S->ntaps, g0.nx, g0.ny, g0.nz,
+                               S->xphys.as<double>(), (double*)nullptr, (const Bisect*)nullptr);
             LP_HIP(hipGetLastError());
             // change, volume, x = xnew
             LP_HIP(hipMemsetAsync(S->chg.p, 0, 8, s));

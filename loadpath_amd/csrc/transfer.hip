@@ -24,35 +24,38 @@ struct TG {
 template <typename T>
 __global__ void k_restrict(const T* __restrict__ rf, const uint8_t* __restrict__ cmask, T* __restrict__ rc, TG g,
                            size_t fstride, size_t cstride) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     const int rhs = blockIdx.y;
-    if (i >= g.cnn) return;
-    const int Z = (int)(i % g.cz);
-    const int Y = (int)((i / g.cz) % g.cy);
-    const int X = (int)(i / ((size_t)g.cz * g.cy));
+    if (i >= (unsigned)g.cnn) return;
+    const unsigned cyz = (unsigned)g.cy * (unsigned)g.cz;
+    const int X = (int)(i / cyz);
+    const unsigned rem = i - (unsigned)X * cyz;
+    const int Y = (int)(rem / (unsigned)g.cz);
+    const int Z = (int)(rem - (unsigned)Y * (unsigned)g.cz);
     const int m = cmask ? (int)cmask[i] : 7;
     const T* f = rf + rhs * fstride;
+    // the 27 fine neighbours' offsets and weights, shared by the components
+    unsigned off[27];
+    T w[27];
+#pragma unroll
+    for (int a = -1; a <= 1; ++a)
+#pragma unroll
+        for (int b = -1; b <= 1; ++b)
+#pragma unroll
+            for (int e = -1; e <= 1; ++e) {
+                const int k = (a + 1) * 9 + (b + 1) * 3 + (e + 1);
+                const int x = 2 * X + a, y = 2 * Y + b, z = 2 * Z + e;
+                const bool in = x >= 0 && x < g.fx && y >= 0 && y < g.fy && z >= 0 && z < g.fz;
+                off[k] = in ? ((unsigned)x * (unsigned)g.fy + (unsigned)y) * (unsigned)g.fz + (unsigned)z : 0u;
+                w[k] = in ? (a != 0 ? T(0.5) : T(1)) * (b != 0 ? T(0.5) : T(1)) * (e != 0 ? T(0.5) : T(1)) : T(0);
+            }
     for (int c = 0; c < 3; ++c) {
         T acc = T(0);
         const T* fc = f + c * g.fnn;
+        // the same order as a loop over a, b, e skipping the outside ones
+        // (adding an exact zero for those changes nothing)
 #pragma unroll
-        for (int a = -1; a <= 1; ++a) {
-            const int x = 2 * X + a;
-            if (x < 0 || x >= g.fx) continue;
-            const T wa = a != 0 ? T(0.5) : T(1);
-#pragma unroll
-            for (int b = -1; b <= 1; ++b) {
-                const int y = 2 * Y + b;
-                if (y < 0 || y >= g.fy) continue;
-                const T wb = wa * (b != 0 ? T(0.5) : T(1));
-#pragma unroll
-                for (int e = -1; e <= 1; ++e) {
-                    const int z = 2 * Z + e;
-                    if (z < 0 || z >= g.fz) continue;
-                    acc += fc[((size_t)x * g.fy + y) * g.fz + z] * (wb * (e != 0 ? T(0.5) : T(1)));
-                }
-            }
-        }
+        for (int k = 0; k < 27; ++k) acc += fc[off[k]] * w[k];
         rc[rhs * cstride + c * g.cnn + i] = ((m >> c) & 1) ? acc : T(0);
     }
 }
@@ -61,41 +64,46 @@ template <typename T, bool ADD>
 __global__ void k_prolong(const T* __restrict__ xc, const uint8_t* __restrict__ cmask,
                           const uint8_t* __restrict__ fmask, T* __restrict__ xf, TG g, size_t cstride,
                           size_t fstride) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
     const int rhs = blockIdx.y;
-    if (i >= g.fnn) return;
-    const int z = (int)(i % g.fz);
-    const int y = (int)((i / g.fz) % g.fy);
-    const int x = (int)(i / ((size_t)g.fz * g.fy));
-    const int x0 = x >> 1, y0 = y >> 1, z0 = z >> 1;
-    const int x1 = x0 + (x & 1), y1 = y0 + (y & 1), z1 = z0 + (z & 1);
-    const T wx = (x & 1) ? T(0.5) : T(1), wy = (y & 1) ? T(0.5) : T(1), wz = (z & 1) ? T(0.5) : T(1);
+    if (i >= (unsigned)g.fnn) return;
+    const unsigned fyz = (unsigned)g.fy * (unsigned)g.fz;
+    const unsigned x = i / fyz;
+    const unsigned rem = i - x * fyz;
+    const unsigned y = rem / (unsigned)g.fz;
+    const unsigned z = rem - y * (unsigned)g.fz;
+    const unsigned xo = x & 1, yo = y & 1, zo = z & 1;
+    const unsigned x0 = x >> 1, y0 = y >> 1, z0 = z >> 1;
+    const unsigned x1 = x0 + xo, y1 = y0 + yo, z1 = z0 + zo;
+    const T wx = xo ? T(0.5) : T(1), wy = yo ? T(0.5) : T(1), wz = zo ? T(0.5) : T(1);
     const int fm = fmask ? (int)fmask[i] : 7;
+    // the 8 parent slots (a repeated parent for an even coordinate is
+    // loaded but added as an exact zero): offsets and masks, once
+    unsigned k8[8];
+    int m8[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const unsigned a = (q & 1) ? x1 : x0, b = (q & 2) ? y1 : y0, e = (q & 4) ? z1 : z0;
+        k8[q] = (a * (unsigned)g.cy + b) * (unsigned)g.cz + e;
+        m8[q] = cmask ? (int)cmask[k8[q]] : 7;
+    }
     const T* c0 = xc + rhs * cstride;
-    auto cidx = [&](int a, int b, int e) { return ((size_t)a * g.cy + b) * g.cz + e; };
     for (int c = 0; c < 3; ++c) {
         const T* cc = c0 + c * g.cnn;
-        auto val = [&](int a, int b, int e) {
-            const size_t k = cidx(a, b, e);
-            const T v = cc[k];
-            if (cmask) return ((cmask[k] >> c) & 1) ? v : T(0);
-            return v;
-        };
-        T acc = val(x0, y0, z0);
-        if (z & 1) acc = acc + val(x0, y0, z1);
-        if (y & 1) {
-            acc = acc + val(x0, y1, z0);
-            if (z & 1) acc = acc + val(x0, y1, z1);
-        }
-        if (x & 1) {
-            T bb = val(x1, y0, z0);
-            if (z & 1) bb = bb + val(x1, y0, z1);
-            if (y & 1) {
-                bb = bb + val(x1, y1, z0);
-                if (z & 1) bb = bb + val(x1, y1, z1);
-            }
-            acc = acc + bb;
-        }
+        T v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = ((m8[q] >> c) & 1) ? cc[k8[q]] : T(0);
+        // the summation order of the reference: (x0: z0, z1, then y1 pair),
+        // then the x1 group likewise
+        T acc = v[0];
+        acc = acc + (zo ? v[4] : T(0));
+        acc = acc + (yo ? v[2] : T(0));
+        acc = acc + ((yo & zo) ? v[6] : T(0));
+        T bb = v[1];
+        bb = bb + (zo ? v[5] : T(0));
+        bb = bb + (yo ? v[3] : T(0));
+        bb = bb + ((yo & zo) ? v[7] : T(0));
+        acc = acc + (xo ? bb : T(0));
         const T tmp = acc * (wx * wy * wz);
         const size_t k = rhs * fstride + c * g.fnn + i;
         if (ADD) {
