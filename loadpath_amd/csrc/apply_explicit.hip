@@ -14,100 +14,110 @@ namespace lp {
 
 namespace {
 
-// block sum of part into partials[rhs * gridDim.x + blockIdx.x]; the trailing
-// barrier lets the caller loop over right-hand sides with one smem slot
-__device__ __forceinline__ void block_partial_loop(double part, const EpiArgs& E, int rhs) {
-    __shared__ double sred[kBlock / 64];
-    part = apply_detail::wave_sum(part);
-    if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = part;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0;
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sred[i];
-        E.partials[(size_t)rhs * gridDim.x + blockIdx.x] = s;
-    }
-    __syncthreads();
-}
+// block = kXN nodes x their 8 cells: thread (c8, node) with c8 = t / kXN, so
+// the kXN lanes of one cell slot read consecutive cells' entries (128-byte
+// runs). The 8 cells' row sums meet in LDS and are added in the order of a
+// butterfly over c8 (((0+1)+(2+3))+((4+5)+(6+7))); then thread (rhs, node)
+// runs the epilogue.
+constexpr int kXN = kBlock / 8;     // nodes per block
+constexpr int kXR = kBlock / kXN;   // right-hand sides per pass (= 8)
 
-__global__ void __launch_bounds__(kBlock) k_apply_explicit_batched(const float* __restrict__ u0, size_t vstride,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 4))) k_apply_explicit_batched(const float* __restrict__ u0, size_t vstride,
                                                                    const uint8_t* __restrict__ mask, AGeom G,
                                                                    const float* __restrict__ k, int nrhs, EpiArgs E) {
     using namespace apply_detail;
-    const int c8 = threadIdx.x & 7;
-    const size_t nid = (size_t)blockIdx.x * (blockDim.x >> 3) + (threadIdx.x >> 3);
-    const size_t nn = G.nn;
+    __shared__ float red[kXR][8][kXN][3];
+    const int t = (int)threadIdx.x;
+    const int c8 = t / kXN, ln = t - c8 * kXN;
+    const unsigned nn = (unsigned)G.nn;
+    const unsigned nid = blockIdx.x * kXN + ln;
     const bool valid = nid < nn;
-    const bool need_raw_all = E.kind == EPI_STORE_DOT || E.kind == EPI_JACOBI || E.kind == EPI_POWER;
-    int x = 0, y = 0, z = 0;
-    bool has = false;
-    int X = 0, Y = 0, Z = 0, a = 0;
-    if (valid) {
-        const unsigned id = (unsigned)nid, nyz = (unsigned)G.Ny * (unsigned)G.Nz;
-        x = (int)(id / nyz);
-        const unsigned rem = id - (unsigned)x * nyz;
-        y = (int)(rem / (unsigned)G.Nz);
-        z = (int)(rem - (unsigned)y * (unsigned)G.Nz);
-        const int cx = c8 & 1, cy = (c8 >> 1) & 1, cz = (c8 >> 2) & 1;
-        X = x - 1 + cx;
-        Y = y - 1 + cy;
-        Z = z - 1 + cz;
-        has = X >= 0 && Y >= 0 && Z >= 0 && X < G.nx && Y < G.ny && Z < G.nz;
-        a = (1 - cx) | ((1 - cy) << 1) | ((1 - cz) << 2);
-    }
-    // the node's 3 rows of its cell's matrix, (row, column) for column j of
-    // the cell's corner vector
-    float kr[72];
+    const unsigned nyz = (unsigned)G.Ny * (unsigned)G.Nz;
+    const int x = valid ? (int)(nid / nyz) : 0;
+    const unsigned rem = valid ? nid - (unsigned)x * nyz : 0;
+    const int y = (int)(rem / (unsigned)G.Nz);
+    const int z = (int)(rem - (unsigned)y * (unsigned)G.Nz);
+    const int cx = c8 & 1, cy = (c8 >> 1) & 1, cz = (c8 >> 2) & 1;
+    const int X = x - 1 + cx, Y = y - 1 + cy, Z = z - 1 + cz;
+    const bool has = valid && X >= 0 && Y >= 0 && Z >= 0 && X < G.nx && Y < G.ny && Z < G.nz;
+    const int a = (1 - cx) | ((1 - cy) << 1) | ((1 - cz) << 2);
+    const size_t nc = G.ncell;
     const size_t ci = has ? ((size_t)X * G.ny + Y) * G.nz + Z : 0;
+    // the node's 3 rows of its cell's matrix
+    float kr[72];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+    for (int e = 0; e < 72; ++e) kr[e] = has ? k[(size_t)(3 * a * 24 + e) * nc + ci] : 0.0f;
+    unsigned off[8];
+    int mk[8];
 #pragma unroll
-        for (int j = 0; j < 24; ++j) kr[r * 24 + j] = has ? k[(size_t)((3 * a + r) * 24 + j) * G.ncell + ci] : 0.0f;
-    for (int rhs = 0; rhs < nrhs; ++rhs) {
-        double part = 0.0;
-        const bool skip = E.scal != nullptr && !E.scal[rhs].active;
-        float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
-        const float* __restrict__ u = u0 + (size_t)rhs * vstride;
-        if (!skip && has) {
+    for (int b = 0; b < 8; ++b) {
+        off[b] = has ? ((unsigned)(X + (b & 1)) * (unsigned)G.Ny + (unsigned)(Y + ((b >> 1) & 1))) * (unsigned)G.Nz +
+                           (unsigned)(Z + ((b >> 2) & 1))
+                     : 0u;
+        mk[b] = has ? (int)mask[off[b]] : 0;
+    }
+    const bool need_raw_all = E.kind == EPI_STORE_DOT || E.kind == EPI_JACOBI || E.kind == EPI_POWER;
+    for (int r0 = 0; r0 < nrhs; r0 += kXR) {
+        const int nr = min(kXR, nrhs - r0);
+        for (int q = 0; q < nr; ++q) {
+            const int rhs = r0 + q;
+            const bool skip = E.scal != nullptr && !E.scal[rhs].active;
+            float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+            if (!skip && has) {
+                const float* __restrict__ u = u0 + (size_t)rhs * vstride;
 #pragma unroll
-            for (int b = 0; b < 8; ++b) {
-                const size_t n = ((size_t)(X + (b & 1)) * G.Ny + (Y + ((b >> 1) & 1))) * G.Nz + (Z + ((b >> 2) & 1));
-                const int m = mask[n];
+                for (int b = 0; b < 8; ++b)
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const float v = ((m >> q) & 1) ? u[q * nn + n] : 0.0f;
-                    t0 = fmaf(kr[3 * b + q], v, t0);
-                    t1 = fmaf(kr[24 + 3 * b + q], v, t1);
-                    t2 = fmaf(kr[48 + 3 * b + q], v, t2);
-                }
+                    for (int c = 0; c < 3; ++c) {
+                        const float v = ((mk[b] >> c) & 1) ? u[(size_t)c * nn + off[b]] : 0.0f;
+                        t0 = fmaf(kr[3 * b + c], v, t0);
+                        t1 = fmaf(kr[24 + 3 * b + c], v, t1);
+                        t2 = fmaf(kr[48 + 3 * b + c], v, t2);
+                    }
             }
+            red[q][c8][ln][0] = t0;
+            red[q][c8][ln][1] = t1;
+            red[q][c8][ln][2] = t2;
         }
+        __syncthreads();
+        // thread (q, node): the node's sum over its 8 cells, then the epilogue
+        const int q = t / kXN;
+        const int rhs = r0 + q;
+        double part = 0.0;
+        const bool live = q < nr && valid && !(E.scal != nullptr && !E.scal[rhs].active);
+        if (live) {
+            float yv[3];
 #pragma unroll
-        for (int o = 1; o < 8; o <<= 1) {
-            t0 += __shfl_xor(t0, o, 64);
-            t1 += __shfl_xor(t1, o, 64);
-            t2 += __shfl_xor(t2, o, 64);
-        }
-        if (!skip && valid && c8 == 0) {
-            const size_t n = nid;
-            const int m = mask[n];
-            float yv[3] = {t0, t1, t2};
+            for (int c = 0; c < 3; ++c) {
+                const float s01 = red[q][0][ln][c] + red[q][1][ln][c], s23 = red[q][2][ln][c] + red[q][3][ln][c];
+                const float s45 = red[q][4][ln][c] + red[q][5][ln][c], s67 = red[q][6][ln][c] + red[q][7][ln][c];
+                yv[c] = (s01 + s23) + (s45 + s67);
+            }
+            const float* __restrict__ u = u0 + (size_t)rhs * vstride;
+            const int m = mask[nid];
             float ur[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const bool mc = (m >> c) & 1;
-                ur[c] = (!mc || need_raw_all) ? u[c * nn + n] : 0.0f;
+                ur[c] = (!mc || need_raw_all) ? u[(size_t)c * nn + nid] : 0.0f;
                 if (!mc) yv[c] = ur[c];
             }
-            epilogue<float>(E, m, (size_t)rhs * vstride + n, nn, n, yv, ur, part);
+            epilogue<float>(E, m, (size_t)rhs * vstride + nid, nn, nid, yv, ur, part);
         }
-        if (E.partials != nullptr) block_partial_loop(part, E, rhs);
+        if (E.partials != nullptr) {
+            // the kXN threads of one rhs are one half of a wave
+#pragma unroll
+            for (int o = kXN / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+            if (ln == 0 && q < nr) E.partials[(size_t)rhs * gridDim.x + blockIdx.x] = part;
+        }
+        __syncthreads();
     }
 }
 
 }  // namespace
 
 int explicit_blocks(const Grid& G, int level) {
-    return (int)((G.lv[level].g.nn + (kBlock / 8) - 1) / (kBlock / 8));
+    return (int)((G.lv[level].g.nn + kXN - 1) / kXN);
 }
 
 void launch_apply_explicit(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs, const EpiArgs& E,

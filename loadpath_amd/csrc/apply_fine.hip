@@ -42,6 +42,7 @@ AGeom make_apply_geom(const Grid& Gd, int level) {
 int apply_blocks(const Grid& Gd, int level, int eb) {
     if (weights_mfma_path(Gd, level, eb)) return mfma_blocks(Gd, level);
     if (Gd.lv[level].kind == LK_FINE && eb == 4 && Gd.fp16()) return mfma_blocks(Gd, level);
+    if (Gd.lv[level].kind == LK_EXPLICIT && eb == 4) return explicit_blocks(Gd, level);
     const AGeom G = make_apply_geom(Gd, level);
     return Gd.lv[level].kind == LK_FINE ? 8 * G.per : G.nblk;
 }

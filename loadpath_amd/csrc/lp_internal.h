@@ -242,6 +242,8 @@ int launch_apply(const Grid& G, int level, int elem_bytes, const void* u, size_t
 // blocks (= reduction partials per rhs) of one operator launch at element size eb
 int apply_blocks(const Grid& G, int level, int eb);
 int mfma_blocks(const Grid& G, int level);
+// blocks (= partials per rhs) of an FP32 explicit-level launch
+int explicit_blocks(const Grid& G, int level);
 void launch_apply_fine(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs,
                        const EpiArgs& E, hipStream_t s);
 void launch_apply_weights(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs,
