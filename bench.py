@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # sides, mean over the 52 launches of one mixed SIMP iteration): FETCH_SIZE x 2
 # + WRITE_SIZE x 1, the factors from a 1 GiB copy in the same run; from
 # profiles/counters_multiload_256x128x128.txt
-FINE_MFMA_TRAFFIC_BYTES = 2.6226e9
+FINE_MFMA_TRAFFIC_BYTES = 2.6285e9
 WORKLOAD = dict(nx=256, ny=128, nz=128, cases=8)
 
 
@@ -173,7 +173,7 @@ def main():
                     help="instead: the 32 uneven batched studies (concurrent, mixed, graphs, autotuned) against "
                          "the same studies one after another on the plain FP64 path")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="first run a plain 1 GiB device copy (3 times): the known-size kernel that scales the "
+                    help="first run a plain 1 GiB elementwise copy (3 times): the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -188,10 +188,12 @@ def main():
     if a.copy_calibration:
         import torch
 
+        # an elementwise kernel (grid over the whole array, so it runs on
+        # every XCD like the kernels it calibrates): 1 GiB read, 1 GiB written
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
 
