@@ -585,23 +585,36 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
                 const bool cok = X >= 0 && X < G.nx && cy >= 0 && cy < G.ny && cz >= 0 && cz < G.nz;
                 const int cell = (X * G.ny + cy) * G.nz + cz;
                 const int pi = cok ? S.pidx[cell] : -1;
-#pragma unroll
-                for (int q = 0; q < 12; ++q) y[q] = 0.0f;
-#pragma unroll 2
-                for (int ch = 0; ch < 8; ++ch) {
+                // the children's weights ride in the FP16 operand: B_q = B
+                // (w_q / w_max), so the 8 contractions accumulate in one
+                // MFMA chain (a child 2^-24 below the cell's largest drops
+                // out: far below the operand rounding)
+                auto wload = [&](int ch) -> float {
                     const int fx = 2 * X + (ch & 1), fy = 2 * cy + ((ch >> 1) & 1), fz = 2 * cz + ((ch >> 2) & 1);
                     const bool wok = cok && pi < 0 && fx < G.fnx && fy < G.fny && fz < G.fnz;
-                    const float w = bload<float>(re, wok ? (unsigned)((fx * G.fny + fy) * G.fnz + fz) * 4u : kOOB);
-                    floatx16 acc = {};
+                    return bload<float>(re, wok ? (unsigned)((fx * G.fny + fy) * G.fnz + fz) * 4u : kOOB);
+                };
+                float wmax = 0.0f;
+#pragma unroll
+                for (int ch = 0; ch < 8; ++ch) wmax = fmaxf(wmax, wload(ch));
+                const float rw = wmax > 0.0f ? 1.0f / wmax : 0.0f;
+                // two accumulation chains (even and odd children) so the
+                // MFMA latency of one overlaps the other
+                floatx16 acc0 = {}, acc1 = {};
+#pragma unroll 1
+                for (int ch = 0; ch < 8; ch += 2) {
+                    const _Float16 r0 = (_Float16)(wload(ch) * rw), r1 = (_Float16)(wload(ch + 1) * rw);
 #pragma unroll
                     for (int s = 0; s < 3; ++s) {
-                        const half4 a = *reinterpret_cast<const half4*>(alds + ((ch * 3 + s) * 64 + lane) * 4);
-                        acc = __builtin_amdgcn_mfma_f32_32x32x8f16(a, B[s], acc, 0, 0, 0);
+                        const half4 a0 = *reinterpret_cast<const half4*>(alds + ((ch * 3 + s) * 64 + lane) * 4);
+                        const half4 a1 = *reinterpret_cast<const half4*>(alds + (((ch + 1) * 3 + s) * 64 + lane) * 4);
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x8f16(a0, B[s] * r0, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x8f16(a1, B[s] * r1, acc1, 0, 0, 0);
                     }
-#pragma unroll
-                    for (int q = 0; q < 12; ++q) y[q] = fmaf(w, acc[q], y[q]);
                 }
-                const float f = ldexpf(1.0f, ex - 15);
+#pragma unroll
+                for (int q = 0; q < 12; ++q) y[q] = acc0[q] + acc1[q];
+                const float f = wmax * ldexpf(1.0f, ex - 15);
                 if (pi >= 0) {
                     const float* pp = S.pprod + ((size_t)rhs * S.npatch + pi) * 24;
 #pragma unroll
