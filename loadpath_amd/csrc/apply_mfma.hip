@@ -509,6 +509,9 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
             goff[k] = (e < WPL && y >= 0 && y < Ny && z >= 0 && z < Nz) ? y * Nz + z : -1;
         }
         float pre[WSTAGE][3];
+        // dinv of the staged-b kinds, multiplied in when the plane is written
+        // to LDS (multiplying at the load would wait for each load in turn)
+        float pdv[STAGE_B ? WSTAGE : 1][3];
         auto fetch = [&](int x) {
             const bool inx = x >= 0 && x < Nx;
 #pragma unroll
@@ -520,7 +523,7 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
                 for (int cc = 0; cc < 3; ++cc) {
                     const unsigned off = ((m >> cc) & 1) ? (cc * nnu + n) * 4u : kOOB;
                     pre[k][cc] = bload<float>(ru, off);
-                    if (STAGE_B) pre[k][cc] = (bload<float>(rdv, off) * pre[k][cc]) * s_b;
+                    if (STAGE_B) pdv[STAGE_B ? k : 0][cc] = bload<float>(rdv, off);
                 }
             }
         };
@@ -531,9 +534,9 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
             for (int k = 0; k < WSTAGE; ++k) {
                 const int e = lane + 64 * k;
                 if (e < WPL) {
-                    P[e] = pre[k][0];
-                    P[WCS + e] = pre[k][1];
-                    P[2 * WCS + e] = pre[k][2];
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc)
+                        P[cc * WCS + e] = STAGE_B ? (pdv[STAGE_B ? k : 0][cc] * pre[k][cc]) * s_b : pre[k][cc];
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -765,7 +768,7 @@ void launch_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int n
                  hipStream_t s) {
     const Level& lv = Gd.lv[level];
     const AGeom G = mfma_geom(Gd, level);
-    if ((size_t)G.Nx * G.Ny * G.Nz * 3 >= (1ull << 30)) {
+    if ((size_t)G.Nx * G.Ny * G.Nz * 3 * sizeof(float) > (1ull << 31)) {
         set_error("level operator: too many nodes for 32-bit buffer offsets");
         throw ArgFailure();
     }
@@ -814,7 +817,7 @@ void launch_wave(const Grid& Gd, int level, const void* u, size_t vstride, int n
                  const MfmaSrc& S, hipStream_t s) {
     const Level& lv = Gd.lv[level];
     const AGeom G = wave_geom(Gd, level);
-    if ((size_t)G.Nx * G.Ny * G.Nz * 3 >= (1ull << 30)) {
+    if ((size_t)G.Nx * G.Ny * G.Nz * 3 * sizeof(float) > (1ull << 31)) {
         set_error("level operator: too many nodes for 32-bit buffer offsets");
         throw ArgFailure();
     }

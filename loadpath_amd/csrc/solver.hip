@@ -511,6 +511,12 @@ int lp_grid_create(lp_grid** out, int nx, int ny, int nz, const uint8_t* fixed_g
             set_error("lp_grid_create: cell counts must be even (pad as mg.py:438-441)");
             return LP_ERR_ARG;
         }
+        // the level kernels address one right-hand side's FP64 node vector
+        // with 32-bit byte offsets and use offsets from 2^31 up as "outside"
+        if ((size_t)(nx + 1) * (ny + 1) * (nz + 1) * 3 * sizeof(double) >= (1ull << 31)) {
+            set_error("lp_grid_create: grid too large (at most 89,478,485 nodes)");
+            return LP_ERR_ARG;
+        }
         lp_grid* G = new lp_grid();
         try {
             G->opt = *opt;
