@@ -332,7 +332,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
     const int logical = (L & 7) * G.per + (L >> 3);
     double part = 0.0;
     const bool skip = (E.scal != nullptr && !E.scal[rhs].active) || logical >= G.nblk;
-    const bool need_raw_all = E.kind == EPI_STORE_DOT || E.kind == EPI_JACOBI || E.kind == EPI_POWER;
     if (!skip) {
         const int xc = logical / G.nyzb;
         const int yb = logical - xc * G.nyzb;
@@ -357,20 +356,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
         const bool czv[2] = {active && z > 0, active && z < G.nz};
         constexpr int MAXE = 4;                            // staged entries per thread (W <= 4 * 256)
         T pre[MAXE][3];
+        int pm[MAXE];
 
-        // load plane x's staged range into registers (masked), zero outside
+        // load plane x's staged range into registers, zero outside the grid;
+        // the loads do not depend on the mask (applied at the store), so they
+        // are all in flight together
         auto fetch = [&](int x) {
             const bool inx = x >= 0 && x < Nx;
 #pragma unroll
             for (int k = 0; k < MAXE; ++k) {
                 const int e = (int)threadIdx.x + k * (int)blockDim.x;
                 const int f = tbase - halo + e;
-                const bool ok = inx && e < W && f >= 0 && f < nyz;
+                // outside: bit 31 set, past the buffer (an arithmetic form: a
+                // select of the offset becomes branches around the loads)
+                const unsigned oob = (unsigned)!(inx && e < W && f >= 0 && f < nyz) << 31;
                 const unsigned n = (unsigned)(x * nyz + f);
-                const int m = bload_u8(rm, ok ? n : kOOB);
+                pm[k] = bload_u8(rm, n | oob);
 #pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    pre[k][c] = bload<T>(ru, ((m >> c) & 1) ? (unsigned)((c * nnu + n) * sizeof(T)) : kOOB);
+                for (int c = 0; c < 3; ++c) pre[k][c] = bload<T>(ru, (unsigned)((c * nnu + n) * sizeof(T)) | oob);
             }
         };
         auto store = [&](int buf) {
@@ -380,15 +383,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
                 const int e = (int)threadIdx.x + k * (int)blockDim.x;
                 if (e < W) {
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) p[c * W + e] = pre[k][c];
+                    for (int c = 0; c < 3; ++c) p[c * W + e] = ((pm[k] >> c) & 1) ? pre[k][c] : T(0);
                 }
             }
         };
 
+        // the 4 cells' moduli of layer X (cell (cy, cz) at 2 cy + cz), loaded
+        // a layer ahead: loads complete in order, so a load issued after the
+        // next plane's would wait for the whole plane
+        auto ec_load = [&](int X, T (&o)[4]) {
+            const bool inx = X >= 0 && X < G.nx;
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cz = 0; cz < 2; ++cz) {
+                    const bool ok = inx && cyv[cy] && czv[cz];
+                    const unsigned ci = (unsigned)((X * G.ny + (y - 1 + cy)) * G.nz + (z - 1 + cz));
+                    o[2 * cy + cz] = bload<T>(re, ok ? ci * (unsigned)sizeof(T) : kOOB);
+                }
+        };
         T acc0 = 0, acc1 = 0, acc2 = 0, nx0 = 0, nx1 = 0, nx2 = 0;
         unsigned long long kea = (unsigned long long)ke;
         // cell layer X between staged planes in buffers bl (X) and bh (X+1)
-        auto layer = [&](int X, int bl, int bh, bool lo) {
+        auto layer = [&](int X, int bl, int bh, bool lo, const T (&ecv)[4]) {
             nx0 = 0; nx1 = 0; nx2 = 0;
             const bool inx = X >= 0 && X < G.nx;
             const T* PL = lds + bl * planeT;
@@ -402,9 +419,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
                     // from hoisting the next cell's LDS reads above this one,
                     // or sinking this cell's arithmetic below them
                     asm volatile("" ::: "memory");
-                    const bool ok = inx && cyv[cy] && czv[cz];
-                    const unsigned ci = (unsigned)((X * G.ny + (y - 1 + cy)) * G.nz + (z - 1 + cz));
-                    const T ec = bload<T>(re, ok ? ci * (unsigned)sizeof(T) : kOOB);
+                    const T ec = ecv[2 * cy + cz];
                     if (!SYM) asm volatile("" : "+s"(kea));
                     cptr<T> Kp = (cptr<T>)kea;
                     auto K = [&](int e) -> T {
@@ -450,6 +465,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
         };
 
         // prologue: planes x0-1, x0, x0+1 -> buffers 0, 1, 2
+        T ecc[4], ecn[4];
+        ec_load(x0 - 1, ecc);
+        ec_load(x0, ecn);
         fetch(x0 - 1);
         store(0);
         fetch(x0);
@@ -457,27 +475,43 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
         fetch(x0 + 1);
         store(2);
         __syncthreads();
-        layer(x0 - 1, 0, 1, false);
+        layer(x0 - 1, 0, 1, false, ecc);
         acc0 = nx0; acc1 = nx1; acc2 = nx2;
         int bl = 1;
         for (int X = x0; X < x1; ++X) {
             const int bh = bl == 2 ? 0 : bl + 1;
             const int bn = bh == 2 ? 0 : bh + 1;          // buffer of plane X-1, refilled with X+2
             const bool more = X + 1 < x1;
-            if (more) fetch(X + 2);
-            layer(X, bl, bh, true);
+            // in order of use: this node's mask, the next layer's moduli, then
+            // plane X + 2 (past the chunk: a plane outside the grid, no traffic)
+            const unsigned n = (unsigned)(X * nyz + t);
+            const int m = bload_u8(rm, active ? n : kOOB);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) ecc[k] = ecn[k];
+            ec_load(X + 1, ecn);
+            fetch(more ? X + 2 : -2);
+            layer(X, bl, bh, true, ecc);
             if (active) {
-                const size_t n = (size_t)X * nyz + t;
-                const int m = mask[n];
                 T yv[3] = {acc0, acc1, acc2};
                 const T* q = lds + bl * planeT + ((int)threadIdx.x + halo);
+                // the staged value is the operand itself where the dof is
+                // free; masked-out dofs keep theirs (boundary nodes only:
+                // loaded, and waited for, inside the branch)
                 T ur[3];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const bool mc = (m >> c) & 1;
-                    ur[c] = (!mc || need_raw_all) ? u[c * (size_t)nnu + n] : q[c * W];
-                    if (!mc) yv[c] = ur[c];
+                for (int c = 0; c < 3; ++c) ur[c] = q[c * W];
+                if (m != 7) {
+                    T raw[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) raw[c] = bload<T>(ru, (unsigned)((c * nnu + n) * sizeof(T)));
+                    asm volatile("" : "+v"(raw[0]), "+v"(raw[1]), "+v"(raw[2]));
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        if (!((m >> c) & 1)) ur[c] = raw[c];
                 }
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    if (!((m >> c) & 1)) yv[c] = ur[c];
                 epilogue<T>(E, m, (size_t)rhs * vstride + n, G.nn, n, yv, ur, part);
             }
             acc0 = nx0; acc1 = nx1; acc2 = nx2;

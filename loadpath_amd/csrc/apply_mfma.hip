@@ -130,6 +130,9 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
 
         // staging: entries tid and tid + 256 of the 17 x 17 patch
         float pre[2][3];
+        // dinv of the staged-b kinds, multiplied in at the LDS store
+        // (multiplying at the load would wait for each load in turn)
+        float pdv[STAGE_B ? 2 : 1][3];
         auto fetch = [&](int x) {
             const bool inx = x >= 0 && x < Nx;
 #pragma unroll
@@ -144,7 +147,7 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
                 for (int c = 0; c < 3; ++c) {
                     const unsigned off = ((m >> c) & 1) ? (c * nnu + n) * 4u : kOOB;
                     pre[k][c] = bload<float>(ru, off);
-                    if (STAGE_B) pre[k][c] = (bload<float>(rdv, off) * pre[k][c]) * s_b;
+                    if (STAGE_B) pdv[STAGE_B ? k : 0][c] = bload<float>(rdv, off);
                 }
             }
         };
@@ -154,9 +157,9 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
             for (int k = 0; k < 2; ++k) {
                 const int e = tid + k * kThreads;
                 if (e < NSTAGE) {
-                    p[e] = pre[k][0];
-                    p[PSTR + e] = pre[k][1];
-                    p[2 * PSTR + e] = pre[k][2];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        p[c * PSTR + e] = STAGE_B ? (pdv[STAGE_B ? k : 0][c] * pre[k][c]) * s_b : pre[k][c];
                 }
             }
         };
