@@ -1,0 +1,295 @@
+"""Oracle: loadpath's production SIMP loop on a box (FP64, numpy).
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+Restates simp._optimize_run (simp.py:1640-2449) for a box domain with a
+preserved-solid mask and the plain compliance objective (no mirror,
+extrusion, overhang filter, heat, gravity, buckling, blockage or member
+limits; those branches are inactive there):
+- blueprint xp = P(H x / hs, beta, eta) and analysed (eroded) design
+  xe = P(H x / hs, beta, eta + erode), preserved cells 1
+                                                   simp.py:1775-1812
+- E = E_MIN + xe^3 (1 - E_MIN), E_MIN = 1e-6       simp.py:42, 55, 1861
+- c = sum_i w_i c_i, dc_xp = -sum_i w_i 3 xe^2 (1 - E_MIN) ce_i
+                                                   simp.py:1897-1904
+- dc = H^T (dc_xp dP(xt, eta_e) / hs), dv = H^T (w dP(xt, eta) / hs),
+  dP zero on preserved cells; dc <= 0, dv >= 1e-12
+                                                   simp.py:1994-2001, 2077, 2190-2204
+- the measured step: along the last step dx the Lagrangian's slope was
+  a = g_old . dx and is b = g_new . dx; the limit scales by
+  clip(a / (a - b), 1/2, 6/5), at most opts.move  simp.py:2228-2245, 1238-1239
+- the OC step x clip(sqrt(-dc / dv / lambda)) with the volume multiplier
+  found by outward bracketing in log lambda from the last one, then
+  Illinois false position to |V - target| <= 1e-13 target
+                                                   simp.py:2246-2381, 1235
+- projection continuation: beta doubles when change < 2 tol on a free step
+  or after beta_every iterations at that beta, the limit halves; at
+  beta_max it stops on change < tol (free step) or, the scheduled stop,
+  after beta_every iterations with the volume bracketed
+                                                   simp.py:2413-2449, 1436-1440
+
+Pinned by tests/test_oracle_loop.py:
+- the chain rule (dc, dv through filter, projection and erosion) against
+  central finite differences of the eroded compliance and the blueprint
+  volume on a tiny grid;
+- the volume root against a dense scan plus bisection in log lambda (V is
+  monotone in lambda), and |V - target| <= 1e-13 target at the root;
+- the measured step on a quadratic, where a / (a - b) is the exact line
+  minimum;
+- the loop's invariants on a small cantilever: the volume on target to
+  1e-10 once bracketed, the limit never above opts.move, beta doubling on
+  the schedule, compliance falling at the sharpest stage.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from .simp import Evaluator, GridFilter, dproject, element_energy, project
+
+E_MIN = 1e-6          # simp.py:42
+PENALTY = 3.0         # simp.py:55
+VOLUME_RTOL = 1e-13   # simp.py:1235
+STEP_SHRINK = 0.5     # simp.py:1238
+STEP_GROW = 1.2       # simp.py:1239
+
+
+@dataclass
+class LoopOptions:
+    """The SimpOptions fields the plain box loop reads (simp.py:899-1000)."""
+    volume_fraction: float
+    filter_radius: float
+    max_iterations: int = 120
+    beta_max: float = 32.0
+    beta_every: int = 10
+    beta_schedule: tuple | None = None
+    move: float = 0.2
+    adaptive_move: bool = True
+    tol: float = 0.01
+    stall_rtol: float = 2e-3
+    stall_window: int = 6
+    eta: float = 0.5
+    erode: float = 0.15
+    beta_start: float = 1.0
+    pad: bool = True
+
+
+def beta_hold(opts: LoopOptions, beta: float) -> int:
+    """Most iterations at projection sharpness beta (simp.py:1436-1440)."""
+    s = opts.beta_schedule
+    k = int(round(np.log2(max(beta, 1.0))))
+    return s[k] if s is not None and k < len(s) else opts.beta_every
+
+
+def step_fraction(a_s: float, b_s: float) -> float:
+    """Where along the last step the parabola through the two slopes has
+    its minimum: a / (a - b), infinite when the slope did not rise
+    (simp.py:2237-2241). Only meaningful for a < 0."""
+    return a_s / (a_s - b_s) if b_s > a_s else np.inf
+
+
+def volume_root(stepped, target: float, lam_last):
+    """The multiplier at which the stepped design's volume meets the target
+    (simp.py:2303-2373): bracket outward in log lambda from the last one,
+    reach x4 per step, then Illinois false position. stepped(lambda) ->
+    (x_new, V - target). Returns (x_new, lambda, f, f1, f2, evaluations)."""
+    n_eval = [0]
+
+    def ev(lm):
+        n_eval[0] += 1
+        return stepped(lm)
+
+    l1, l2 = 1e-12, 1e12
+    s_lo, s_hi = np.log(l1), np.log(l2)
+    sm = min(max(np.log(lam_last), s_lo), s_hi) if lam_last else 0.0
+    x_new, f = ev(float(np.exp(sm)))
+    lm = float(np.exp(sm))
+    s1 = s2 = sm
+    f1 = f2 = None
+    if f > 0:
+        f1 = f
+    else:
+        f2 = f
+    reach = 0.25
+    while (f1 is None or f2 is None) and f != 0.0:
+        if f1 is None:
+            if s1 <= s_lo:
+                break
+            s1 = max(s1 - reach, s_lo)
+            sm = s1
+        else:
+            if s2 >= s_hi:
+                break
+            s2 = min(s2 + reach, s_hi)
+            sm = s2
+        reach *= 4.0
+        lm = float(np.exp(sm))
+        x_new, f = ev(lm)
+        if f > 0:
+            f1, s1 = f, sm
+        else:
+            f2, s2 = f, sm
+    if f1 is not None and f2 is not None:
+        side = 0
+        for _ in range(60):
+            if abs(f) <= VOLUME_RTOL * target or f2 == f1 or s2 - s1 <= 4e-16 * max(abs(s1), abs(s2), 1.0):
+                break
+            sm = (s1 * f2 - s2 * f1) / (f2 - f1)
+            if not s1 < sm < s2:
+                sm = 0.5 * (s1 + s2)
+            lm = float(np.exp(sm))
+            x_new, f = ev(lm)
+            if f > 0:
+                s1, f1 = sm, f
+                if side == 1:
+                    f2 *= 0.5
+                side = 1
+            else:
+                s2, f2 = sm, f
+                if side == -1:
+                    f1 *= 0.5
+                side = -1
+    return x_new, lm, f, f1, f2, n_eval[0]
+
+
+class Loop:
+    """The loop's per-iteration pieces on one problem (filter, physical and
+    eroded designs, sensitivities), shared by the loop and the tests."""
+
+    def __init__(self, p, opts: LoopOptions, preserve=None, weights=None, solve=None, element="im"):
+        self.p, self.opts = p, opts
+        self.ev = Evaluator(p, element)
+        n = self.ev.mesh.n_elem
+        self.n = n
+        self.pres = np.zeros(n, bool) if preserve is None else np.asarray(preserve, bool).ravel()
+        self.design = ~self.pres
+        self.wdesign = self.design.astype(float)
+        self.target = opts.volume_fraction * float(self.wdesign.sum())
+        self.filt = GridFilter(p.dims, opts.filter_radius, pad=opts.pad)
+        self.hs = self.filt.hs
+        self.weights = np.ones(len(self.ev.fs)) if weights is None else np.asarray(weights, float)
+        self.solve = solve if solve is not None else (lambda e: [self.ev.solve(e, f) for f in self.ev.fs])
+        self.eta_e = opts.eta + opts.erode
+
+    def physical(self, x, beta):
+        xt = (self.filt @ x) / self.hs
+        xp = project(xt, beta, self.opts.eta)
+        xp[self.pres] = 1.0
+        return xt, xp
+
+    def eroded(self, xt, beta):
+        xe = project(xt, beta, self.eta_e)
+        xe[self.pres] = 1.0
+        return xe
+
+    def analyse(self, x, beta):
+        """compliance, dc, dv, xp, cases' displacements (simp.py:1860-2204)."""
+        xt, xp = self.physical(x, beta)
+        xe = self.eroded(xt, beta)
+        e = E_MIN + xe ** PENALTY * (1 - E_MIN)
+        us = self.solve(e)
+        comp = 0.0
+        dc_xp = np.zeros(self.n)
+        for w, u, f in zip(self.weights, us, self.ev.fs):
+            ce = element_energy(u, self.ev.edof, self.ev.ke)
+            comp += w * float(f @ u)
+            dc_xp -= w * PENALTY * xe ** (PENALTY - 1) * (1 - E_MIN) * ce
+        dproj_e = dproject(xt, beta, self.eta_e)
+        dproj_e[self.pres] = 0.0
+        dproj = dproject(xt, beta, self.opts.eta)
+        dproj[self.pres] = 0.0
+        dc = self.filt.T @ (dc_xp * dproj_e / self.hs)
+        dv = self.filt.T @ (self.wdesign * dproj / self.hs)
+        return comp, dc, dv, xp, us
+
+
+def optimize_loop(p, opts: LoopOptions, preserve=None, weights=None, solve=None, element="im", on_iter=None):
+    """simp._optimize_run's loop (simp.py:1840-2449) on a box. Returns
+    dict(x, xphys, history, converged, iterations)."""
+    t0 = time.time()
+    L = Loop(p, opts, preserve, weights, solve, element)
+    x = np.full(L.n, opts.volume_fraction)
+    x[L.pres] = 1.0
+    beta = float(opts.beta_start)
+    since_beta = 0
+    measured = opts.adaptive_move
+    scheduled = measured
+    step_prev = None
+    move_k = opts.move
+    lam_last = None
+    c0 = None
+    history = []
+    converged = False
+    it = 0
+    for it in range(1, opts.max_iterations + 1):
+        sec = None
+        comp, dc, dv, xp, _ = L.analyse(x, beta)
+        if c0 is None:
+            c0 = comp
+        dc = np.minimum(dc, 0.0)
+        dv = np.maximum(dv, 1e-12)
+        if measured:
+            if step_prev is not None:
+                dxp, gp, lam_p = step_prev
+                a_s = float(gp @ dxp)
+                b_s = float((dc + lam_p * dv) @ dxp)
+                if a_s < 0:
+                    t_s = step_fraction(a_s, b_s)
+                    sec = t_s
+                    move_k = min(move_k * min(max(t_s, STEP_SHRINK), STEP_GROW), opts.move)
+            move = move_k
+        else:
+            move = opts.move if beta <= 4 else opts.move / 2
+        lo_x = np.maximum(0.0, x - move)
+        hi_x = np.minimum(1.0, x + move)
+        ratio = np.sqrt(-dc / dv)
+
+        def stepped(lm_):
+            xn = np.clip(x * ratio / np.sqrt(lm_), lo_x, hi_x)
+            xn[L.pres] = 1.0
+            v = float(L.physical(xn, beta)[1] @ L.wdesign)
+            return xn, v - L.target
+
+        x_new, lm, _, f1, f2, n_eval = volume_root(stepped, L.target, lam_last)
+        lam_last = lm
+        vol_held = f1 is not None and f2 is not None
+        change = float(np.abs(x_new - x)[L.design].max())
+        if measured:
+            step_prev = (x_new - x, dc + lm * dv, lm)
+        x = x_new
+        since_beta += 1
+        rec = {"iteration": it, "compliance": comp, "relative": comp / c0 if c0 else 1.0,
+               "volume": float(xp[L.design].mean()), "beta": beta, "change": change,
+               "seconds": time.time() - t0, "volumeSolves": n_eval, "lambda": lm, "volHeld": vol_held}
+        if opts.adaptive_move:
+            rec["move"] = move
+            if sec is not None:
+                rec["stepFit"] = float(min(sec, 99.0))
+        history.append(rec)
+        if on_iter is not None:
+            on_iter(rec)
+        free_step = not measured or change < 0.999 * move
+        if beta < opts.beta_max and ((change < opts.tol * 2 and free_step) or since_beta >= beta_hold(opts, beta)):
+            beta = min(beta * 2, opts.beta_max)
+            since_beta = 0
+            step_prev = None
+            move_k = move_k / 2
+        elif beta >= opts.beta_max and it >= 3:
+            if change < opts.tol and free_step:
+                converged = True
+                break
+            w = opts.stall_window
+            if scheduled and vol_held:
+                if since_beta >= beta_hold(opts, beta):
+                    converged = True
+                    break
+            elif since_beta >= w and len(history) > w:
+                old = history[-1 - w]["compliance"]
+                if abs(comp - old) <= opts.stall_rtol * abs(old):
+                    converged = True
+                    break
+    xt, xp = L.physical(x, beta)
+    return dict(x=x.reshape(p.dims), xphys=xp.reshape(p.dims), history=history, converged=converged,
+                iterations=it, beta=beta)
