@@ -153,11 +153,21 @@ typedef struct lp_simp lp_simp;
 typedef struct lp_simp_stats {
     double compliance;    /* sum_i sum_c E_c ce_c of the iteration's design */
     double change;        /* max |xnew - x| */
-    double volume;        /* mean xphys after the update */
+    double volume;        /* mean xphys after the update (production loop:
+                             the blueprint's mean over design cells before it) */
     int cg_iters;         /* max PCG iterations over the load cases */
     int cg_iters_total;   /* PCG iterations summed over the load cases */
-    int oc_evals;         /* bisection evaluations */
+    int oc_evals;         /* bisection evaluations (production loop: volume
+                             evaluations of the multiplier root) */
     double seconds;
+    /* production loop only (lp_simp_set_loop), else 0 */
+    double beta;          /* projection sharpness of this iteration */
+    double move;          /* the step limit it used */
+    double step_fit;      /* where along the last step the measured minimum
+                             sat, min(., 99); -1 when not measured */
+    double lambda;        /* the volume multiplier */
+    int vol_held;         /* the root was bracketed */
+    int converged;        /* the loop's stop test passed after this iteration */
 } lp_simp_stats;
 
 /* loads: (nrhs, 3, nodes) FP64 node-grid load vectors (masked inside).
@@ -178,6 +188,40 @@ int lp_simp_set_x(lp_simp* s, const double* x);
  * loops on compliance-type objectives, thermal.py:1035-1043). Boxes without
  * passive cells. */
 int lp_simp_set_optimizer(lp_simp* s, int optimizer);
+
+/* loadpath's production loop, simp._optimize_run (simp.py:1640-2449) for a
+ * box with the plain compliance objective: blueprint xp = P(H x / hs, beta,
+ * eta), analysed (eroded) design xe = P(., beta, eta + erode), preserved
+ * cells 1, E = emin + xe^penal (1 - emin); dc = H (dc_xp dP_e / hs),
+ * dv = H (w dP / hs) (simp.py:1994-2204); the measured step
+ * (simp.py:2228-2245); the OC step with the volume multiplier bracketed in
+ * log lambda from the last one and landed by Illinois false position to
+ * 1e-13 of the target (simp.py:2303-2373); beta doubling on the schedule
+ * and the stop tests (simp.py:2413-2449). Fields as SimpOptions
+ * (simp.py:899-1000); schedule: iterations per doubling by level
+ * (beta_schedule) or NULL. */
+typedef struct lp_loop_opts {
+    double beta_start;    /* 1 */
+    double beta_max;      /* 32 */
+    int beta_every;       /* 10 */
+    double eta;           /* 0.5 */
+    double erode;         /* 0.15 */
+    double move;          /* 0.2 */
+    int adaptive_move;    /* 1: the measured step; 0: move, move / 2 past beta 4 */
+    double tol;           /* 0.01 */
+    double stall_rtol;    /* 2e-3 */
+    int stall_window;     /* 6 */
+    double emin;          /* 1e-6 (simp.py E_MIN) */
+    double penal;         /* 3 */
+    const int* schedule;  /* NULL or schedule_len entries */
+    int schedule_len;
+} lp_loop_opts;
+/* Switch s to the production loop and restart it: x = volfrac, preserved
+ * cells (preserve[c] != 0, NULL: none) 1, passive cells 0. case_weights:
+ * nrhs weights of the load cases' compliances, NULL: all 1. Each
+ * lp_simp_iterate call then runs loop iterations; stats.converged marks
+ * the iteration after which the loop stops. */
+int lp_simp_set_loop(lp_simp* s, const lp_loop_opts* o, const uint8_t* preserve, const double* case_weights);
 
 /* Heaviside projection simp._project / _dproject (simp.py:1443-1450):
  * out = (tanh(b e) + tanh(b (x - e))) / (tanh(b e) + tanh(b (1 - e))), dout

@@ -50,6 +50,32 @@ class SimpStats(C.Structure):
         ("cg_iters_total", C.c_int),
         ("oc_evals", C.c_int),
         ("seconds", C.c_double),
+        ("beta", C.c_double),
+        ("move", C.c_double),
+        ("step_fit", C.c_double),
+        ("lam", C.c_double),
+        ("vol_held", C.c_int),
+        ("converged", C.c_int),
+    ]
+
+
+class LoopOpts(C.Structure):
+    """lp_loop_opts (include/loadpath_mi.h)."""
+    _fields_ = [
+        ("beta_start", C.c_double),
+        ("beta_max", C.c_double),
+        ("beta_every", C.c_int),
+        ("eta", C.c_double),
+        ("erode", C.c_double),
+        ("move", C.c_double),
+        ("adaptive_move", C.c_int),
+        ("tol", C.c_double),
+        ("stall_rtol", C.c_double),
+        ("stall_window", C.c_int),
+        ("emin", C.c_double),
+        ("penal", C.c_double),
+        ("schedule", C.c_void_p),
+        ("schedule_len", C.c_int),
     ]
 
 
@@ -86,6 +112,7 @@ _SIGS = {
     "lp_simp_get": (_I, [_P, _I, _P]),
     "lp_simp_set_x": (_I, [_P, _P]),
     "lp_simp_set_optimizer": (_I, [_P, _I]),
+    "lp_simp_set_loop": (_I, [_P, C.POINTER(LoopOpts), _P, _P]),
     "lp_project": (_I, [C.c_longlong, _P, _D, _D, _P, _P, _I]),
     "lp_simp_displacements": (_I, [_P, _P]),
     "lp_filter_apply": (_I, [_I, _I, _I, _D, _I, _P, _P, _I]),

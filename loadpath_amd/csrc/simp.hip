@@ -80,7 +80,7 @@ __global__ void k_mask_loads(const double* __restrict__ f, const uint8_t* __rest
 // ce = sum_rhs u_c^T Ke u_c; partial of sum E ce; dc = -p (1-emin) x^(p-1) ce
 __global__ void k_sens(const double* __restrict__ u, const double* __restrict__ ke, Geom g, int nrhs, size_t vstride,
                        const double* __restrict__ xphys, const double* __restrict__ e, double penal, double emin,
-                       double* ce_out, double* dc, double* partials) {
+                       const double* __restrict__ wcase, double* ce_out, double* dc, double* partials) {
     __shared__ double sred[4];
     const size_t cell = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double part = 0;
@@ -106,7 +106,7 @@ __global__ void k_sens(const double* __restrict__ u, const double* __restrict__ 
                 for (int j = 0; j < 24; ++j) t += ke[i * 24 + j] * ue[j];
                 acc += ue[i] * t;
             }
-            ce += acc;
+            ce += wcase ? wcase[r] * acc : acc;
         }
         ce_out[cell] = ce;
         const double xp = xphys[cell];
@@ -237,6 +237,130 @@ __global__ void k_bisect_finalize(const double* __restrict__ partials, int nb, B
     if (!((bis->l2 - bis->l1) / (bis->l1 + bis->l2) > tol)) bis->done = 1;
 }
 
+// ---- the production loop (simp._optimize_run) ----------------------------
+
+__device__ __forceinline__ double proj_value(double xt, double beta, double eta) {
+    const double den = tanh(beta * eta) + tanh(beta * (1.0 - eta));
+    return (tanh(beta * eta) + tanh(beta * (xt - eta))) / den;
+}
+__device__ __forceinline__ double proj_deriv(double xt, double beta, double eta) {
+    const double den = tanh(beta * eta) + tanh(beta * (1.0 - eta));
+    const double t = tanh(beta * (xt - eta));
+    return beta * (1.0 - t * t) / den;
+}
+
+// blueprint xp = P(xt, eta) and analysed design xe = P(xt, eta_e), both 1 on
+// preserved cells and 0 on passive ones (simp.py:1775-1812); the moduli
+// E = emin + xe^p (1 - emin) (simp.py:1861); partials of sum xp over the
+// design (the recorded volume)
+__global__ void k_loop_physical(const double* __restrict__ xt, const uint8_t* __restrict__ pres,
+                                const uint8_t* __restrict__ passive, const double* __restrict__ wdes, size_t n,
+                                double beta, double eta, double eta_e, double penal, double emin, double* xp,
+                                double* xe, double* e, double* partials) {
+    __shared__ double sred[4];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double part = 0.0;
+    if (i < n) {
+        double p = proj_value(xt[i], beta, eta), q = proj_value(xt[i], beta, eta_e);
+        if (pres[i]) p = q = 1.0;
+        if (passive && passive[i]) p = q = 0.0;
+        xp[i] = p;
+        xe[i] = q;
+        e[i] = emin + pow(q, penal) * (1.0 - emin);
+        part = p * wdes[i];
+    }
+    const double r = bsum(part, sred);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// the chain rule's cell factors (simp.py:1994-2001, 2077):
+// g = dc_xp P'(xt, eta_e) / hs, gv = w P'(xt, eta) / hs, 0 on preserved
+// and passive cells
+__global__ void k_loop_chain(const double* __restrict__ dcx, const double* __restrict__ xt,
+                             const uint8_t* __restrict__ pres, const uint8_t* __restrict__ passive,
+                             const double* __restrict__ wdes, const double* __restrict__ hs_inv, size_t n, double beta,
+                             double eta, double eta_e, double* g, double* gv) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool off = pres[i] || (passive && passive[i]);
+    const double dpe = off ? 0.0 : proj_deriv(xt[i], beta, eta_e);
+    const double dp = off ? 0.0 : proj_deriv(xt[i], beta, eta);
+    g[i] = dcx[i] * dpe * hs_inv[i];
+    gv[i] = wdes[i] * dp * hs_inv[i];
+}
+
+// dc <= 0, dv >= 1e-12 in place (simp.py:2191, 2204), ratio = sqrt(-dc / dv);
+// with a previous step, partials of a = gp . dx and b = (dc + lp dv) . dx
+// (simp.py:2237-2238) in rows 0 and 1
+__global__ void k_loop_clamp(double* __restrict__ dc, double* __restrict__ dv, double* __restrict__ ratio, size_t n,
+                             const double* __restrict__ dxp, const double* __restrict__ gp, double lam_p,
+                             double* partials) {
+    __shared__ double sred[4];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double pa = 0.0, pb = 0.0;
+    if (i < n) {
+        const double c = fmin(dc[i], 0.0), v = fmax(dv[i], 1e-12);
+        dc[i] = c;
+        dv[i] = v;
+        ratio[i] = sqrt(-c / v);
+        if (dxp) {
+            pa = gp[i] * dxp[i];
+            pb = (c + lam_p * v) * dxp[i];
+        }
+    }
+    if (!dxp) return;
+    const double ra = bsum(pa, sred);
+    const double rb = bsum(pb, sred);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = ra;
+        partials[gridDim.x + blockIdx.x] = rb;
+    }
+}
+
+// one step at multiplier lm: xn = clip(x ratio / sqrt(lm), x -/+ move in
+// [0, 1]), preserved 1, passive 0 (simp.py:2288-2302)
+__global__ void k_loop_step(const double* __restrict__ x, const double* __restrict__ ratio,
+                            const uint8_t* __restrict__ pres, const uint8_t* __restrict__ passive, size_t n,
+                            double move, double sqrt_lm, double* xn) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi = x[i];
+    const double lo = fmax(0.0, xi - move), hi = fmin(1.0, xi + move);
+    double v = fmin(fmax(xi * ratio[i] / sqrt_lm, lo), hi);
+    if (pres[i]) v = 1.0;
+    if (passive && passive[i]) v = 0.0;
+    xn[i] = v;
+}
+
+// partials of the stepped blueprint's volume sum_design P(xt, eta)
+// (simp.py:2299-2300; preserved cells carry no design weight)
+__global__ void k_loop_volume(const double* __restrict__ xt, const uint8_t* __restrict__ pres,
+                              const uint8_t* __restrict__ passive, const double* __restrict__ wdes, size_t n,
+                              double beta, double eta, double* partials) {
+    __shared__ double sred[4];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double part = 0.0;
+    if (i < n) {
+        double p = proj_value(xt[i], beta, eta);
+        if (pres[i]) p = 1.0;
+        if (passive && passive[i]) p = 0.0;
+        part = p * wdes[i];
+    }
+    const double r = bsum(part, sred);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// the step just taken, for the next iteration's measurement: dx = xn - x,
+// gp = dc + lm dv (simp.py:2381)
+__global__ void k_loop_stepprev(const double* __restrict__ xn, const double* __restrict__ x,
+                                const double* __restrict__ dc, const double* __restrict__ dv, size_t n, double lm,
+                                double* dxp, double* gp) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dxp[i] = xn[i] - x[i];
+    gp[i] = dc[i] + lm * dv[i];
+}
+
 __global__ void k_reduce_sum(const double* __restrict__ partials, int nb, double* out) {
     __shared__ double sred[4];
     double acc = 0;
@@ -303,10 +427,241 @@ struct lp_simp {
     lp_mma* mma = nullptr;
     double c_ref = 0.0;
     DBuf df0, dcon;
+    // the production loop (lp_simp_set_loop)
+    struct Loop {
+        bool on = false;
+        lp_loop_opts o{};
+        std::vector<int> schedule;
+        double beta = 1.0, move_k = 0.2, lam_last = 0.0, lam_p = 0.0, c0 = 0.0, target = 0.0, n_design = 0.0;
+        int since_beta = 0, it = 0;
+        bool have_prev = false, converged = false, weighted = false;
+        std::vector<double> comp_hist;
+        DBuf pres, wdes, xt, xe, g, gv, ratio, dxp, gp, wcase;
+    } L;
     ~lp_simp() {
         if (mma) lp_mma_destroy(mma);
     }
 };
+
+namespace {
+
+// iterations at projection sharpness beta (simp.py:1436-1440)
+int loop_hold(const lp_simp::Loop& L, double beta) {
+    const int k = (int)std::lround(std::log2(std::max(beta, 1.0)));
+    return k < (int)L.schedule.size() ? L.schedule[k] : L.o.beta_every;
+}
+
+double reduce_host(lp_simp* S, const double* partials, int nb, int slot) {
+    hipStream_t s = S->g->stream;
+    double* scal = S->scalar.as<double>();
+    hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kBlock), 0, s, partials, nb, scal + slot);
+    LP_HIP(hipGetLastError());
+    double v = 0.0;
+    LP_HIP(hipMemcpyAsync(&v, scal + slot, 8, hipMemcpyDeviceToHost, s));
+    LP_HIP(hipStreamSynchronize(s));
+    return v;
+}
+
+void loop_filter(lp_simp* S, const double* v, const double* post, double* out) {
+    const Geom& g0 = S->g->g0;
+    hipLaunchKernelGGL(k_filter, dim3(nblk(S->ncell)), dim3(kBlock), 0, S->g->stream, v, (const double*)nullptr, post,
+                       (const uint8_t*)nullptr, S->taps.as<Tap>(), S->ntaps, g0.nx, g0.ny, g0.nz, out, (double*)nullptr,
+                       (const Bisect*)nullptr);
+    LP_HIP(hipGetLastError());
+}
+
+// one iteration of simp._optimize_run's loop (simp.py:1840-2449)
+int loop_iterate(lp_simp* S, lp_simp_stats& st) {
+    lp_simp::Loop& L = S->L;
+    Grid& G = *S->g;
+    hipStream_t s = G.stream;
+    const size_t n = S->ncell;
+    const Geom& g0 = G.g0;
+    const size_t vn = 3 * g0.nn;
+    const unsigned nb = nblk(n);
+    const auto t0 = std::chrono::steady_clock::now();
+    const lp_loop_opts& o = L.o;
+    const double beta = L.beta, eta_e = o.eta + o.erode;
+    const uint8_t* pas = S->has_passive ? S->passive.as<uint8_t>() : nullptr;
+    const uint8_t* pres = L.pres.as<uint8_t>();
+    L.it += 1;
+    // designs and moduli (simp.py:1857-1861)
+    loop_filter(S, S->x.as<double>(), S->hs_inv.as<double>(), L.xt.as<double>());
+    hipLaunchKernelGGL(k_loop_physical, dim3(nb), dim3(kBlock), 0, s, L.xt.as<double>(), pres, pas,
+                       L.wdes.as<double>(), n, beta, o.eta, eta_e, o.penal, o.emin, S->xphys.as<double>(),
+                       L.xe.as<double>(), G.lv[0].e64.as<double>(), S->partials.as<double>());
+    LP_HIP(hipGetLastError());
+    const double vol_now = reduce_host(S, S->partials.as<double>(), (int)nb, 2);
+    int rc = build_hierarchy(G);
+    if (rc == LP_SINGULAR && !G.strict) {
+        G.strict = true;
+        rc = build_hierarchy(G);
+    }
+    if (rc != LP_OK) return rc;
+    G.built = true;
+    if (G.opt.smoother == LP_SMOOTH_CHEBYSHEV)
+        for (int i = 0; i < G.nlevels; ++i) lp_estimate_lmax(S->g, i, nullptr);
+    hipLaunchKernelGGL(k_mask_loads, dim3(nblk(vn), S->nrhs), dim3(kBlock), 0, s, S->loads.as<double>(),
+                       G.lv[0].mask.as<uint8_t>(), G.b.as<double>(), g0.nn, vn);
+    LP_HIP(hipGetLastError());
+    std::vector<int> iters(S->nrhs), ok(S->nrhs);
+    pcg(G, S->nrhs, S->warm && S->have_u, iters.data(), ok.data());
+    for (int k = 0; k < S->nrhs; ++k)
+        if (!ok[k]) {
+            set_error("linear solve did not converge; the structure may be a mechanism (check supports)");
+            return LP_NOT_CONVERGED;
+        }
+    S->have_u = true;
+    // compliance and dc_xp on the analysed design (simp.py:1897-1904)
+    hipLaunchKernelGGL(k_sens, dim3(nb), dim3(kBlock), 0, s, G.x.as<double>(), G.ke64.as<double>(), g0, S->nrhs, vn,
+                       L.xe.as<double>(), G.lv[0].e64.as<double>(), o.penal, o.emin,
+                       L.weighted ? L.wcase.as<double>() : (const double*)nullptr, S->ce.as<double>(),
+                       S->dc.as<double>(), S->partials.as<double>());
+    LP_HIP(hipGetLastError());
+    const double comp = reduce_host(S, S->partials.as<double>(), (int)nb, 0);
+    if (L.c0 == 0.0) L.c0 = comp;
+    // dc = H (dc_xp P'_e / hs), dv = H (w P' / hs) (H symmetric)
+    hipLaunchKernelGGL(k_loop_chain, dim3(nb), dim3(kBlock), 0, s, S->dc.as<double>(), L.xt.as<double>(), pres, pas,
+                       L.wdes.as<double>(), S->hs_inv.as<double>(), n, beta, o.eta, eta_e, L.g.as<double>(),
+                       L.gv.as<double>());
+    LP_HIP(hipGetLastError());
+    loop_filter(S, L.g.as<double>(), nullptr, S->dc.as<double>());
+    loop_filter(S, L.gv.as<double>(), nullptr, S->dv.as<double>());
+    // clamps, ratio and the measured step (simp.py:2228-2245)
+    double sec = -1.0;
+    double move;
+    hipLaunchKernelGGL(k_loop_clamp, dim3(nb), dim3(kBlock), 0, s, S->dc.as<double>(), S->dv.as<double>(),
+                       L.ratio.as<double>(), n, L.have_prev && o.adaptive_move ? L.dxp.as<double>() : (const double*)nullptr,
+                       L.gp.as<double>(), L.lam_p, S->partials.as<double>());
+    LP_HIP(hipGetLastError());
+    if (o.adaptive_move) {
+        if (L.have_prev) {
+            const double a_s = reduce_host(S, S->partials.as<double>(), (int)nb, 3);
+            const double b_s = reduce_host(S, S->partials.as<double>() + nb, (int)nb, 4);
+            if (a_s < 0) {
+                const double t_s = b_s > a_s ? a_s / (a_s - b_s) : INFINITY;
+                sec = std::min(t_s, 99.0);
+                L.move_k = std::min(L.move_k * std::min(std::max(t_s, 0.5), 1.2), o.move);
+            }
+        }
+        move = L.move_k;
+    } else {
+        move = beta <= 4 ? o.move : o.move / 2;
+    }
+    // the volume multiplier (simp.py:2303-2373)
+    int n_eval = 0;
+    auto stepped = [&](double lm) -> double {
+        ++n_eval;
+        hipLaunchKernelGGL(k_loop_step, dim3(nb), dim3(kBlock), 0, s, S->x.as<double>(), L.ratio.as<double>(), pres,
+                           pas, n, move, std::sqrt(lm), S->xnew.as<double>());
+        LP_HIP(hipGetLastError());
+        loop_filter(S, S->xnew.as<double>(), S->hs_inv.as<double>(), L.g.as<double>());
+        hipLaunchKernelGGL(k_loop_volume, dim3(nb), dim3(kBlock), 0, s, L.g.as<double>(), pres, pas,
+                           L.wdes.as<double>(), n, beta, o.eta, S->partials.as<double>());
+        LP_HIP(hipGetLastError());
+        return reduce_host(S, S->partials.as<double>(), (int)nb, 1) - L.target;
+    };
+    const double s_lo = std::log(1e-12), s_hi = std::log(1e12);
+    double sm = L.lam_last > 0 ? std::min(std::max(std::log(L.lam_last), s_lo), s_hi) : 0.0;
+    double lm = std::exp(sm);
+    double f = stepped(lm);
+    double s1 = sm, s2 = sm, f1 = 0, f2 = 0;
+    bool h1 = false, h2 = false;
+    if (f > 0) { f1 = f; h1 = true; } else { f2 = f; h2 = true; }
+    double reach = 0.25;
+    while ((!h1 || !h2) && f != 0.0) {
+        if (!h1) {
+            if (s1 <= s_lo) break;
+            s1 = std::max(s1 - reach, s_lo);
+            sm = s1;
+        } else {
+            if (s2 >= s_hi) break;
+            s2 = std::min(s2 + reach, s_hi);
+            sm = s2;
+        }
+        reach *= 4.0;
+        lm = std::exp(sm);
+        f = stepped(lm);
+        if (f > 0) { f1 = f; s1 = sm; h1 = true; } else { f2 = f; s2 = sm; h2 = true; }
+    }
+    if (h1 && h2) {
+        int side = 0;
+        for (int k = 0; k < 60; ++k) {
+            if (std::fabs(f) <= 1e-13 * L.target || f2 == f1 ||
+                s2 - s1 <= 4e-16 * std::max(std::max(std::fabs(s1), std::fabs(s2)), 1.0))
+                break;
+            sm = (s1 * f2 - s2 * f1) / (f2 - f1);
+            if (!(s1 < sm && sm < s2)) sm = 0.5 * (s1 + s2);
+            lm = std::exp(sm);
+            f = stepped(lm);
+            if (f > 0) {
+                s1 = sm; f1 = f;
+                if (side == 1) f2 *= 0.5;
+                side = 1;
+            } else {
+                s2 = sm; f2 = f;
+                if (side == -1) f1 *= 0.5;
+                side = -1;
+            }
+        }
+    }
+    L.lam_last = lm;
+    const bool vol_held = h1 && h2;
+    // change, the step for the next measurement, x = xnew
+    LP_HIP(hipMemsetAsync(S->chg.p, 0, 8, s));
+    hipLaunchKernelGGL(k_change, dim3(std::min(nb, 1024u)), dim3(kBlock), 0, s, S->xnew.as<double>(), S->x.as<double>(),
+                       n, S->chg.as<unsigned long long>());
+    LP_HIP(hipGetLastError());
+    if (o.adaptive_move) {
+        hipLaunchKernelGGL(k_loop_stepprev, dim3(nb), dim3(kBlock), 0, s, S->xnew.as<double>(), S->x.as<double>(),
+                           S->dc.as<double>(), S->dv.as<double>(), n, lm, L.dxp.as<double>(), L.gp.as<double>());
+        LP_HIP(hipGetLastError());
+        L.have_prev = true;
+        L.lam_p = lm;
+    }
+    std::swap(S->x.p, S->xnew.p);
+    double change = 0.0;
+    LP_HIP(hipMemcpyAsync(&change, S->chg.p, 8, hipMemcpyDeviceToHost, s));
+    LP_HIP(hipStreamSynchronize(s));
+    L.since_beta += 1;
+    L.comp_hist.push_back(comp);
+    st.compliance = comp;
+    st.change = change;
+    st.volume = L.n_design > 0 ? vol_now / L.n_design : 0.0;
+    st.cg_iters = *std::max_element(iters.begin(), iters.end());
+    st.cg_iters_total = 0;
+    for (int k = 0; k < S->nrhs; ++k) st.cg_iters_total += iters[k];
+    st.oc_evals = n_eval;
+    st.beta = beta;
+    st.move = move;
+    st.step_fit = sec;
+    st.lambda = lm;
+    st.vol_held = vol_held ? 1 : 0;
+    st.converged = 0;
+    // continuation and the stop tests (simp.py:2413-2449)
+    const bool measured = o.adaptive_move != 0;
+    const bool free_step = !measured || change < 0.999 * move;
+    if (beta < o.beta_max && ((change < o.tol * 2 && free_step) || L.since_beta >= loop_hold(L, beta))) {
+        L.beta = std::min(beta * 2, o.beta_max);
+        L.since_beta = 0;
+        L.have_prev = false;
+        L.move_k = L.move_k / 2;
+    } else if (beta >= o.beta_max && L.it >= 3) {
+        if (change < o.tol && free_step) {
+            L.converged = true;
+        } else if (measured && vol_held) {
+            if (L.since_beta >= loop_hold(L, beta)) L.converged = true;
+        } else if (L.since_beta >= o.stall_window && (int)L.comp_hist.size() > o.stall_window) {
+            const double old = L.comp_hist[L.comp_hist.size() - 1 - o.stall_window];
+            if (std::fabs(comp - old) <= o.stall_rtol * std::fabs(old)) L.converged = true;
+        }
+    }
+    st.converged = L.converged ? 1 : 0;
+    st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return LP_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -392,7 +747,7 @@ int lp_simp_create(lp_simp** out, lp_grid* g, int nrhs, const double* loads, con
             S->loads.alloc(vn * nrhs * 8);
             LP_HIP(hipMemcpy(S->loads.p, loads, vn * nrhs * 8, hipMemcpyHostToDevice));
             S->bis.alloc(sizeof(Bisect));
-            S->partials.alloc(nblk(n) * 8);
+            S->partials.alloc(2 * nblk(n) * 8);
             S->scalar.alloc(64);
             S->chg.alloc(8);
             // the volume target counts every cell of the box (top3d: volfrac * n)
@@ -418,6 +773,14 @@ int lp_simp_iterate(lp_simp* S, int niter, lp_simp_stats* stats) {
         const Geom& g0 = G.g0;
         const size_t vn = 3 * g0.nn;
         const unsigned nb = nblk(n);
+        if (S->L.on) {
+            for (int it = 0; it < niter; ++it) {
+                stats[it] = lp_simp_stats{};
+                const int rc = loop_iterate(S, stats[it]);
+                if (rc != LP_OK) return rc;
+            }
+            return LP_OK;
+        }
         for (int it = 0; it < niter; ++it) {
             const auto t0 = std::chrono::steady_clock::now();
             // moduli and hierarchy
@@ -450,7 +813,7 @@ int lp_simp_iterate(lp_simp* S, int niter, lp_simp_stats* stats) {
             // sensitivities and compliance
             hipLaunchKernelGGL(k_sens, dim3(nb), dim3(kBlock), 0, s, G.x.as<double>(), G.ke64.as<double>(), g0, S->nrhs,
                                vn, S->xphys.as<double>(), G.lv[0].e64.as<double>(), S->penal, S->emin,
-                               S->ce.as<double>(), S->dc.as<double>(), S->partials.as<double>());
+                               (const double*)nullptr, S->ce.as<double>(), S->dc.as<double>(), S->partials.as<double>());
             LP_HIP(hipGetLastError());
             double* scal = S->scalar.as<double>();
             hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kBlock), 0, s, S->partials.as<double>(), (int)nb, scal + 0);
@@ -592,6 +955,64 @@ int lp_project(long long n, const double* xt, double beta, double eta, double* o
     });
 }
 
+int lp_simp_set_loop(lp_simp* S, const lp_loop_opts* o, const uint8_t* preserve, const double* case_weights) {
+    return guarded_s([&]() -> int {
+        if (!S || !o || o->beta_start <= 0 || o->beta_max < o->beta_start || o->beta_every < 1 || o->move <= 0 ||
+            o->penal <= 0 || o->emin <= 0 || o->emin >= 1 || o->stall_window < 1 || o->eta <= 0 || o->eta >= 1 ||
+            o->eta + o->erode <= 0 || o->eta + o->erode >= 1 || (o->schedule_len > 0 && !o->schedule)) {
+            set_error("lp_simp_set_loop: bad options");
+            return LP_ERR_ARG;
+        }
+        lp_simp::Loop& L = S->L;
+        const size_t n = S->ncell;
+        hipStream_t s = S->g->stream;
+        L.o = *o;
+        L.schedule.assign(o->schedule, o->schedule + std::max(o->schedule_len, 0));
+        for (DBuf* b : {&L.wdes, &L.xt, &L.xe, &L.g, &L.gv, &L.ratio, &L.dxp, &L.gp})
+            if (!b->p) b->alloc(n * 8);
+        if (!L.pres.p) L.pres.alloc(n);
+        std::vector<uint8_t> pr(n, 0), pas(n, 0);
+        if (preserve)
+            for (size_t i = 0; i < n; ++i) pr[i] = preserve[i] ? 1 : 0;
+        if (S->has_passive) LP_HIP(hipMemcpy(pas.data(), S->passive.p, n, hipMemcpyDeviceToHost));
+        std::vector<double> w(n), x0(n);
+        double nd = 0;
+        for (size_t i = 0; i < n; ++i) {
+            w[i] = pr[i] ? 0.0 : 1.0;
+            nd += w[i];
+            x0[i] = pr[i] ? 1.0 : (pas[i] ? 0.0 : S->volfrac);
+        }
+        if (nd == 0) {
+            set_error("lp_simp_set_loop: nothing to optimize: every cell is preserved");
+            return LP_ERR_ARG;
+        }
+        LP_HIP(hipMemcpy(L.pres.p, pr.data(), n, hipMemcpyHostToDevice));
+        LP_HIP(hipMemcpy(L.wdes.p, w.data(), n * 8, hipMemcpyHostToDevice));
+        LP_HIP(hipMemcpy(S->x.p, x0.data(), n * 8, hipMemcpyHostToDevice));
+        L.weighted = case_weights != nullptr;
+        if (L.weighted) {
+            if (!L.wcase.p) L.wcase.alloc(S->nrhs * 8);
+            LP_HIP(hipMemcpy(L.wcase.p, case_weights, S->nrhs * 8, hipMemcpyHostToDevice));
+        }
+        L.n_design = nd;
+        L.target = S->volfrac * nd;
+        L.beta = o->beta_start;
+        L.move_k = o->move;
+        L.lam_last = 0.0;
+        L.lam_p = 0.0;
+        L.c0 = 0.0;
+        L.since_beta = 0;
+        L.it = 0;
+        L.have_prev = false;
+        L.converged = false;
+        L.comp_hist.clear();
+        L.on = true;
+        S->have_u = false;
+        LP_HIP(hipStreamSynchronize(s));
+        return LP_OK;
+    });
+}
+
 int lp_simp_set_optimizer(lp_simp* S, int optimizer) {
     return guarded_s([&]() -> int {
         if (!S || (optimizer != 0 && optimizer != 1)) return LP_ERR_ARG;
@@ -600,6 +1021,7 @@ int lp_simp_set_optimizer(lp_simp* S, int optimizer) {
             return LP_ERR_ARG;
         }
         S->optimizer = optimizer;
+        S->L.on = false;   // back from the production loop, if it was on
         if (optimizer == 1 && !S->mma) {
             const size_t n = S->ncell;
             std::vector<double> lo(n, 0.0), hi(n, 1.0);

@@ -9,12 +9,19 @@ Mirrors:
 - ``density_filter(dims, radius, pad)`` / ``GridFilter`` (simp.py:1143-1189):
   ``h @ v``, ``h.T``, ``hs``, on the GPU.
 
+- the production loop ``simp.optimize`` (simp.py:1570-1640, its loop
+  simp.py:1840-2449) on a box: ``SimpOptions`` / ``optimize`` below
+  (Heaviside projection with continuation, the analysed eroded design, the
+  measured step and the volume multiplier by bracketing and Illinois false
+  position, all on the device).
+
 Several load cases are solved together and their compliances summed.
 """
 from __future__ import annotations
 
 import ctypes as C
 import time
+from dataclasses import dataclass, field
 
 import numpy as np
 
@@ -90,7 +97,7 @@ class GridSimp:
 
     def __init__(self, problem, *, precision="fp32", smoother="chebyshev", omega=0.6, degree=2, rtol=1e-8,
                  maxiter=500, coarse_cells=2000, max_levels=0, element=None, nu=0.3, penal=3.0, emin=1e-9,
-                 move=0.2, filter="top3d", warm_start=True, optimizer="oc"):
+                 move=0.2, filter="top3d", warm_start=True, optimizer="oc", volfrac=None, rmin=None):
         from .mg import GridElasticity
         from .fem import box_mesh
 
@@ -112,8 +119,11 @@ class GridSimp:
                                    element=element or hex8_element(), precision=precision)
         h = C.c_void_p()
         self._loads = np.ascontiguousarray(loads)
+        self.volfrac = problem.volfrac if volfrac is None else float(volfrac)
+        self.rmin = problem.rmin if rmin is None else float(rmin)
+        self.filter = filter
         _lib.check(_lib.lib().lp_simp_create(C.byref(h), self.grid._h, self.nrhs, _lib.ptr(self._loads), None,
-                                             problem.volfrac, problem.rmin, _lib.FILTER[filter], penal, emin, move,
+                                             self.volfrac, self.rmin, _lib.FILTER[filter], penal, emin, move,
                                              int(warm_start)), "lp_simp_create")
         self._h = h
         self.iterations = 0
@@ -140,9 +150,34 @@ class GridSimp:
         out = []
         for k in range(n):
             s = stats[k]
-            out.append(dict(compliance=s.compliance, change=s.change, volume=s.volume, cg_iters=s.cg_iters,
-                            cg_iters_total=s.cg_iters_total, oc_evals=s.oc_evals, seconds=s.seconds))
+            d = dict(compliance=s.compliance, change=s.change, volume=s.volume, cg_iters=s.cg_iters,
+                     cg_iters_total=s.cg_iters_total, oc_evals=s.oc_evals, seconds=s.seconds)
+            if self.optimizer == "loop":
+                d.update(beta=s.beta, move=s.move, step_fit=None if s.step_fit < 0 else s.step_fit, lam=s.lam,
+                         vol_held=bool(s.vol_held), converged=bool(s.converged))
+            out.append(d)
         return out
+
+    def set_loop(self, opts: "SimpOptions", preserve=None, case_weights=None):
+        """Switch to the production loop (lp_simp_set_loop) and restart it
+        from x = volume fraction (preserved cells 1)."""
+        sched = None if opts.beta_schedule is None else np.ascontiguousarray(opts.beta_schedule, np.int32)
+        o = _lib.LoopOpts(beta_start=opts.beta_start, beta_max=opts.beta_max, beta_every=opts.beta_every,
+                          eta=opts.eta, erode=opts.erode, move=opts.move, adaptive_move=int(opts.adaptive_move),
+                          tol=opts.tol, stall_rtol=opts.stall_rtol, stall_window=opts.stall_window,
+                          emin=E_MIN, penal=PENALTY,
+                          schedule=None if sched is None else sched.ctypes.data,
+                          schedule_len=0 if sched is None else len(sched))
+        self._pres = None if preserve is None else np.ascontiguousarray(np.asarray(preserve, bool).reshape(
+            self.problem.dims), np.uint8)
+        self._cw = None if case_weights is None else np.ascontiguousarray(case_weights, np.float64)
+        if self._cw is not None and len(self._cw) != self.nrhs:
+            raise ValueError(f"case_weights: {self.nrhs} load cases, {len(self._cw)} weights")
+        _lib.check(_lib.lib().lp_simp_set_loop(self._h, C.byref(o), None if self._pres is None else _lib.ptr(self._pres),
+                                               None if self._cw is None else _lib.ptr(self._cw)), "simp_set_loop")
+        self.optimizer = "loop"
+        self.loop_opts = opts
+        self.iterations = 0
 
     def _get(self, what):
         out = np.empty(self.problem.dims)
@@ -169,6 +204,131 @@ class GridSimp:
         out = np.empty((self.nrhs, 3) + self.problem.node_shape)
         _lib.check(_lib.lib().lp_simp_displacements(self._h, _lib.ptr(out)), "simp_displacements")
         return out
+
+
+E_MIN = 1e-6       # simp.py:42 (the production loop's void modulus)
+PENALTY = 3.0      # simp.py:55
+
+
+@dataclass
+class SimpOptions:
+    """The fields of simp.SimpOptions (simp.py:899-1000) that the loop on a
+    box reads, with the reference's defaults. solve_rtol is each
+    iteration's PCG tolerance (simp.py:929)."""
+    volume_fraction: float
+    filter_radius: float
+    max_iterations: int = 120
+    beta_max: float = 32.0
+    beta_every: int = 10
+    beta_schedule: tuple | None = None
+    move: float = 0.2
+    adaptive_move: bool = True
+    tol: float = 0.01
+    solve_rtol: float = 1e-4
+    stall_rtol: float = 2e-3
+    stall_window: int = 6
+    eta: float = 0.5
+    erode: float = 0.15
+    beta_start: float = 1.0
+
+
+@dataclass
+class SimpResult:
+    """simp.SimpResult's fields that the loop fills (simp.py:1093-1121):
+    x the design variables, xphys the blueprint P(H x / hs) at the final
+    sharpness, history one record per iteration (the reference's keys:
+    iteration, compliance, relative, volume, beta, change, seconds,
+    cgIterations, volumeSolves, move, stepFit)."""
+    x: np.ndarray
+    xphys: np.ndarray
+    compliance: float
+    volume_fraction: float
+    iterations: int
+    converged: bool
+    seconds: float
+    history: list = field(default_factory=list)
+
+
+def optimize(problem, opts: SimpOptions, *, preserve=None, case_weights=None, precision="mixed", pad=True,
+             on_iter=None, **kw) -> SimpResult:
+    """simp.optimize (simp.py:1570-1640) for a box problem with the plain
+    compliance objective, every iteration on the device (loop
+    simp.py:1840-2449). preserve: cells held solid (domain.preserve);
+    case_weights: the load cases' weights (domain.case_weights). The
+    delivered design is the blueprint at the loop's end: the reference's
+    post-processing of it (opening the eroded design, pruning dead
+    members) is not part of this loop. kw go to GridSimp (precision,
+    coarse_cells, element, ...)."""
+    kw.setdefault("rtol", opts.solve_rtol)
+    s = GridSimp(problem, precision=precision, filter="padded" if pad else "top3d", volfrac=opts.volume_fraction,
+                 rmin=opts.filter_radius, **kw)
+    s.set_loop(opts, preserve=preserve, case_weights=case_weights)
+    t0 = time.time()
+    history = []
+    converged = False
+    c0 = None
+    for it in range(1, opts.max_iterations + 1):
+        st = s.iterate(1)[0]
+        c0 = st["compliance"] if c0 is None else c0
+        rec = {"iteration": it, "compliance": st["compliance"], "relative": st["compliance"] / c0 if c0 else 1.0,
+               "volume": st["volume"], "beta": st["beta"], "change": st["change"], "seconds": time.time() - t0,
+               "cgIterations": st["cg_iters"], "volumeSolves": st["oc_evals"], "lambda": st["lam"],
+               "volHeld": st["vol_held"]}
+        if opts.adaptive_move:
+            rec["move"] = st["move"]
+            if st["step_fit"] is not None:
+                rec["stepFit"] = st["step_fit"]
+        history.append(rec)
+        if on_iter is not None:
+            on_iter(rec)
+        if st["converged"]:
+            converged = True
+            break
+    x = s.x
+    xphys = blueprint(x, opts, pad=pad, beta=_final_beta(history, opts), preserve=preserve)
+    return SimpResult(x=x, xphys=xphys, compliance=history[-1]["compliance"] if history else 0.0,
+                      volume_fraction=float(xphys[~_pres_mask(preserve, x.shape)].mean()),
+                      iterations=len(history), converged=converged, seconds=time.time() - t0, history=history)
+
+
+def _pres_mask(preserve, shape):
+    return np.zeros(shape, bool) if preserve is None else np.asarray(preserve, bool).reshape(shape)
+
+
+def _final_beta(history, opts):
+    """The sharpness after the last iteration's continuation test
+    (simp.py:2413-2421): the records hold each iteration's own beta."""
+    if not history:
+        return opts.beta_start
+    beta = history[-1]["beta"]
+    since = 0
+    for h in reversed(history):
+        if h["beta"] != beta:
+            break
+        since += 1
+    last = history[-1]
+    free = not opts.adaptive_move or last["change"] < 0.999 * last.get("move", opts.move)
+    if beta < opts.beta_max and ((last["change"] < opts.tol * 2 and free) or since >= _beta_hold(opts, beta)):
+        beta = min(beta * 2, opts.beta_max)
+    return beta
+
+
+def _beta_hold(opts, beta):
+    s = opts.beta_schedule
+    k = int(round(np.log2(max(beta, 1.0))))
+    return s[k] if s is not None and k < len(s) else opts.beta_every
+
+
+def blueprint(x, opts: SimpOptions, *, pad=True, beta=None, preserve=None):
+    """The blueprint xp = P(H x / hs, beta, eta), preserved cells 1
+    (simp.physical, simp.py:1775-1781), on the device."""
+    x = np.asarray(x, float)
+    f = GridFilter(x.shape, opts.filter_radius)
+    hs = np.full(x.size, sum(w for _, w in f.taps)) if pad else f.sum()
+    xt = (f @ x.ravel()) / hs
+    xp = project(xt, opts.beta_max if beta is None else beta, opts.eta).reshape(x.shape)
+    xp[_pres_mask(preserve, x.shape)] = 1.0
+    return xp
 
 
 def optimize_top3d(problem, iterations=None, maxit=200, on_iter=None, **kw):

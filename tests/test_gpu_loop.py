@@ -1,0 +1,106 @@
+"""The production loop on the device (lp_simp_set_loop, loadpath_amd.simp.
+optimize) against the oracle's restatement of simp._optimize_run
+(oracle/loop.py), and the properties the reference's tests assert of
+simp.optimize (tests/test_loadpath.py:106-118, 1190-1223): volume on
+target to rounding, the step limit never above opts.move, bit-identical
+reruns, compliance falling at the sharpest stage."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem():
+    from loadpath_amd.problems import BoxProblem, nodes
+
+    nx, ny, nz = 16, 4, 6
+    return BoxProblem("cantilever-2case", (nx, ny, nz), 0.3, 1.5,
+                      [(nodes(0, range(ny + 1), range(nz + 1)), "xyz")],
+                      [[(nodes(nx, range(ny + 1), 0), "z", -1.0)],
+                       [(nodes(nx, range(ny + 1), nz), "y", 0.4)]])
+
+
+def _preserve(dims):
+    pres = np.zeros(dims, bool)
+    pres[-1, :, 0] = True            # a pad under the first load
+    return pres
+
+
+def _oracle_problem(p):
+    from oracle import simp as os_
+
+    return os_.Problem(p.name, p.dims, p.volfrac, p.rmin, list(p.fixed), [list(c) for c in p.cases])
+
+
+def test_loop_matches_oracle(gpu):
+    """FP64 device loop (PCG to 1e-11) against the oracle's direct solves,
+    25 iterations: two beta doublings, the measured step, weighted cases
+    and a preserved pad."""
+    from loadpath_amd.simp import SimpOptions, optimize
+    from oracle.loop import LoopOptions, optimize_loop
+
+    p = _problem()
+    pres = _preserve(p.dims)
+    w = [1.0, 0.5]
+    n_it = 25
+    opts = SimpOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=n_it, solve_rtol=1e-11)
+    r = optimize(p, opts, preserve=pres, case_weights=w, precision="fp64", element="im", maxiter=3000,
+                 coarse_cells=50)
+    ro = optimize_loop(_oracle_problem(p), LoopOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=n_it),
+                       preserve=pres.ravel(), weights=w, element="im")
+    assert len(r.history) == len(ro["history"]) == n_it
+    for a, b in zip(r.history, ro["history"]):
+        print(a["iteration"], a["compliance"], b["compliance"], a["beta"], a["move"], b["move"], a["change"],
+              b["change"], a["volumeSolves"], b["volumeSolves"])
+        assert a["beta"] == b["beta"]
+        assert a["compliance"] == pytest.approx(b["compliance"], rel=1e-7)
+        assert a["volume"] == pytest.approx(b["volume"], abs=1e-9)
+        assert a["move"] == pytest.approx(b["move"], abs=1e-7)
+        assert a["change"] == pytest.approx(b["change"], abs=1e-6)
+        assert a["volHeld"] == b["volHeld"]
+        assert a["lambda"] == pytest.approx(b["lambda"], rel=1e-5)
+    assert np.abs(r.x - ro["x"]).max() < 1e-6
+    assert np.abs(r.xphys - ro["xphys"]).max() < 1e-5
+
+
+def test_loop_properties_mixed_precision(gpu):
+    """The mixed-precision path on a larger cantilever to its stop: volume on
+    target where the step held it, the limit at most opts.move, the stage
+    at beta_max settling, and a rerun bit-identical."""
+    from loadpath_amd.problems import cantilever_3d
+    from loadpath_amd.simp import SimpOptions, optimize
+
+    p = cantilever_3d(48, 12, 16, volfrac=0.2, rmin=2.0)
+    opts = SimpOptions(volume_fraction=0.2, filter_radius=2.0, max_iterations=90)
+    runs = [optimize(p, opts, precision="mixed", coarse_cells=200) for _ in range(2)]
+    r = runs[0]
+    h = r.history
+    print(len(h), r.converged, [round(x["compliance"], 3) for x in h[-6:]])
+    assert r.converged
+    assert all(x["move"] <= 0.2 + 1e-12 for x in h)
+    assert [x["beta"] for x in h] == sorted(x["beta"] for x in h)
+    pairs = [(a, b) for a, b in zip(h, h[1:]) if a["volHeld"] and a["beta"] == b["beta"]]
+    assert len(pairs) > 20
+    assert max(abs(b["volume"] - 0.2) for a, b in pairs) < 1e-10
+    last = [x for x in h if x["beta"] >= 32][3:]
+    assert len(last) >= 3
+    rise = max(b["compliance"] / a["compliance"] - 1 for a, b in zip(last, last[1:]))
+    assert rise < 1e-3, rise
+    assert last[-1]["compliance"] < last[0]["compliance"]
+    grey = ((r.xphys > 0.1) & (r.xphys < 0.9)).mean()
+    assert grey < 0.1, grey
+    assert np.array_equal(runs[0].x, runs[1].x)
+    assert [x["compliance"] for x in runs[0].history] == [x["compliance"] for x in runs[1].history]
+
+
+def test_loop_rejects_bad_options(gpu):
+    from loadpath_amd.problems import cantilever_3d
+    from loadpath_amd.simp import GridSimp, SimpOptions
+
+    s = GridSimp(cantilever_3d(8, 2, 4), coarse_cells=20)
+    with pytest.raises(Exception, match="bad options"):
+        s.set_loop(SimpOptions(volume_fraction=0.3, filter_radius=1.5, eta=1.2))
+    with pytest.raises(Exception, match="every cell is preserved"):
+        s.set_loop(SimpOptions(volume_fraction=0.3, filter_radius=1.5), preserve=np.ones((8, 2, 4), bool))
+    with pytest.raises(ValueError):
+        s.set_loop(SimpOptions(volume_fraction=0.3, filter_radius=1.5), case_weights=[1.0, 2.0])
