@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # sides, mean over the 52 launches of one mixed SIMP iteration): FETCH_SIZE x 2
 # + WRITE_SIZE x 1, the factors from a 1 GiB copy in the same run; from
 # profiles/counters_multiload_256x128x128.txt
-FINE_MFMA_TRAFFIC_BYTES = 2.6285e9
+FINE_MFMA_TRAFFIC_BYTES = 2.9107e9
 WORKLOAD = dict(nx=256, ny=128, nz=128, cases=8)
 
 
