@@ -133,6 +133,11 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
         // dinv of the staged-b kinds, multiplied in at the LDS store
         // (multiplying at the load would wait for each load in turn)
         float pdv[STAGE_B ? 2 : 1][3];
+        // the plane's loads do not depend on its mask (applied at the store):
+        // they are in flight together; outside the grid an offset with bit
+        // 31 set reads zero (an arithmetic form: a select of the offset
+        // becomes branches around the loads)
+        int pm[2];
         auto fetch = [&](int x) {
             const bool inx = x >= 0 && x < Nx;
 #pragma unroll
@@ -140,12 +145,12 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
                 const int e = tid + k * kThreads;
                 const int sj = e / TS, sk = e - (e / TS) * TS;
                 const int y = y0 - 1 + sj, z = z0 - 1 + sk;
-                const bool ok = inx && e < NSTAGE && y >= 0 && y < Ny && z >= 0 && z < Nz;
+                const unsigned oob = (unsigned)!(inx && e < NSTAGE && y >= 0 && y < Ny && z >= 0 && z < Nz) << 31;
                 const unsigned n = (unsigned)(x * nyz + y * Nz + z);
-                const int m = bload_u8(rm, ok ? n : kOOB);
+                pm[k] = bload_u8(rm, n | oob);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const unsigned off = ((m >> c) & 1) ? (c * nnu + n) * 4u : kOOB;
+                    const unsigned off = ((c * nnu + n) * 4u) | oob;
                     pre[k][c] = bload<float>(ru, off);
                     if (STAGE_B) pdv[STAGE_B ? k : 0][c] = bload<float>(rdv, off);
                 }
@@ -159,7 +164,7 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
                 if (e < NSTAGE) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c)
-                        p[c * PSTR + e] = STAGE_B ? (pdv[STAGE_B ? k : 0][c] * pre[k][c]) * s_b : pre[k][c];
+                        p[c * PSTR + e] = ((pm[k] >> c) & 1) ? (STAGE_B ? (pdv[STAGE_B ? k : 0][c] * pre[k][c]) * s_b : pre[k][c]) : 0.0f;
                 }
             }
         };
@@ -322,7 +327,7 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
             const int bh = bl == 2 ? 0 : bl + 1;
             const int bn = bh == 2 ? 0 : bh + 1;
             const bool more = X + 1 < x1;
-            if (more) fetch(X + 2);
+            fetch(more ? X + 2 : -2);   // past the chunk: outside the grid, no traffic
             en0 = ecell_load(X + 1, 0);
             en1 = ecell_load(X + 1, 1);
             contract(X, bl, bh);

@@ -97,7 +97,7 @@ class GridSimp:
 
     def __init__(self, problem, *, precision="fp32", smoother="chebyshev", omega=0.6, degree=2, rtol=1e-8,
                  maxiter=500, coarse_cells=2000, max_levels=0, element=None, nu=0.3, penal=3.0, emin=1e-9,
-                 move=0.2, filter="top3d", warm_start=True, optimizer="oc", volfrac=None, rmin=None):
+                 move=0.2, filter="top3d", warm_start=True, optimizer="oc", volfrac=None, rmin=None, active=None):
         from .mg import GridElasticity
         from .fem import box_mesh
 
@@ -122,7 +122,11 @@ class GridSimp:
         self.volfrac = problem.volfrac if volfrac is None else float(volfrac)
         self.rmin = problem.rmin if rmin is None else float(rmin)
         self.filter = filter
-        _lib.check(_lib.lib().lp_simp_create(C.byref(h), self.grid._h, self.nrhs, _lib.ptr(self._loads), None,
+        # active: the design cells (passive ones are held at 0), None: all
+        self._active = None if active is None else np.ascontiguousarray(
+            np.asarray(active, bool).reshape(problem.dims), np.uint8)
+        _lib.check(_lib.lib().lp_simp_create(C.byref(h), self.grid._h, self.nrhs, _lib.ptr(self._loads),
+                                             None if self._active is None else _lib.ptr(self._active),
                                              self.volfrac, self.rmin, _lib.FILTER[filter], penal, emin, move,
                                              int(warm_start)), "lp_simp_create")
         self._h = h
@@ -249,19 +253,20 @@ class SimpResult:
     history: list = field(default_factory=list)
 
 
-def optimize(problem, opts: SimpOptions, *, preserve=None, case_weights=None, precision="mixed", pad=True,
+def optimize(problem, opts: SimpOptions, *, preserve=None, void=None, case_weights=None, precision="mixed", pad=True,
              on_iter=None, **kw) -> SimpResult:
     """simp.optimize (simp.py:1570-1640) for a box problem with the plain
     compliance objective, every iteration on the device (loop
     simp.py:1840-2449). preserve: cells held solid (domain.preserve);
-    case_weights: the load cases' weights (domain.case_weights). The
+    void: cells held empty (opts.void), still counted in the volume
+    target; case_weights: the load cases' weights (domain.case_weights). The
     delivered design is the blueprint at the loop's end: the reference's
     post-processing of it (opening the eroded design, pruning dead
     members) is not part of this loop. kw go to GridSimp (precision,
     coarse_cells, element, ...)."""
     kw.setdefault("rtol", opts.solve_rtol)
     s = GridSimp(problem, precision=precision, filter="padded" if pad else "top3d", volfrac=opts.volume_fraction,
-                 rmin=opts.filter_radius, **kw)
+                 rmin=opts.filter_radius, active=None if void is None else ~np.asarray(void, bool), **kw)
     s.set_loop(opts, preserve=preserve, case_weights=case_weights)
     t0 = time.time()
     history = []
@@ -285,7 +290,7 @@ def optimize(problem, opts: SimpOptions, *, preserve=None, case_weights=None, pr
             converged = True
             break
     x = s.x
-    xphys = blueprint(x, opts, pad=pad, beta=_final_beta(history, opts), preserve=preserve)
+    xphys = blueprint(x, opts, pad=pad, beta=_final_beta(history, opts), preserve=preserve, void=void)
     return SimpResult(x=x, xphys=xphys, compliance=history[-1]["compliance"] if history else 0.0,
                       volume_fraction=float(xphys[~_pres_mask(preserve, x.shape)].mean()),
                       iterations=len(history), converged=converged, seconds=time.time() - t0, history=history)
@@ -319,15 +324,17 @@ def _beta_hold(opts, beta):
     return s[k] if s is not None and k < len(s) else opts.beta_every
 
 
-def blueprint(x, opts: SimpOptions, *, pad=True, beta=None, preserve=None):
-    """The blueprint xp = P(H x / hs, beta, eta), preserved cells 1
-    (simp.physical, simp.py:1775-1781), on the device."""
+def blueprint(x, opts: SimpOptions, *, pad=True, beta=None, preserve=None, void=None):
+    """The blueprint xp = P(H x / hs, beta, eta), preserved cells 1, void
+    cells 0 (simp.physical, simp.py:1775-1781), on the device."""
     x = np.asarray(x, float)
     f = GridFilter(x.shape, opts.filter_radius)
     hs = np.full(x.size, sum(w for _, w in f.taps)) if pad else f.sum()
     xt = (f @ x.ravel()) / hs
     xp = project(xt, opts.beta_max if beta is None else beta, opts.eta).reshape(x.shape)
     xp[_pres_mask(preserve, x.shape)] = 1.0
+    if void is not None:
+        xp[np.asarray(void, bool).reshape(x.shape)] = 0.0
     return xp
 
 

@@ -7,7 +7,7 @@ preserved-solid mask and the plain compliance objective (no mirror,
 extrusion, overhang filter, heat, gravity, buckling, blockage or member
 limits; those branches are inactive there):
 - blueprint xp = P(H x / hs, beta, eta) and analysed (eroded) design
-  xe = P(H x / hs, beta, eta + erode), preserved cells 1
+  xe = P(H x / hs, beta, eta + erode), preserved cells 1, void cells 0
                                                    simp.py:1775-1812
 - E = E_MIN + xe^3 (1 - E_MIN), E_MIN = 1e-6       simp.py:42, 55, 1861
 - c = sum_i w_i c_i, dc_xp = -sum_i w_i 3 xe^2 (1 - E_MIN) ce_i
@@ -158,12 +158,15 @@ class Loop:
     """The loop's per-iteration pieces on one problem (filter, physical and
     eroded designs, sensitivities), shared by the loop and the tests."""
 
-    def __init__(self, p, opts: LoopOptions, preserve=None, weights=None, solve=None, element="im"):
+    def __init__(self, p, opts: LoopOptions, preserve=None, weights=None, solve=None, element="im", void=None):
         self.p, self.opts = p, opts
         self.ev = Evaluator(p, element)
         n = self.ev.mesh.n_elem
         self.n = n
         self.pres = np.zeros(n, bool) if preserve is None else np.asarray(preserve, bool).ravel()
+        # void cells (opts.void): held at 0, still counted as design cells in
+        # the volume target (simp.py:1671-1682)
+        self.void = np.zeros(n, bool) if void is None else np.asarray(void, bool).ravel()
         self.design = ~self.pres
         self.wdesign = self.design.astype(float)
         self.target = opts.volume_fraction * float(self.wdesign.sum())
@@ -177,11 +180,13 @@ class Loop:
         xt = (self.filt @ x) / self.hs
         xp = project(xt, beta, self.opts.eta)
         xp[self.pres] = 1.0
+        xp[self.void] = 0.0
         return xt, xp
 
     def eroded(self, xt, beta):
         xe = project(xt, beta, self.eta_e)
         xe[self.pres] = 1.0
+        xe[self.void] = 0.0
         return xe
 
     def analyse(self, x, beta):
@@ -198,20 +203,24 @@ class Loop:
             dc_xp -= w * PENALTY * xe ** (PENALTY - 1) * (1 - E_MIN) * ce
         dproj_e = dproject(xt, beta, self.eta_e)
         dproj_e[self.pres] = 0.0
+        dproj_e[self.void] = 0.0
         dproj = dproject(xt, beta, self.opts.eta)
         dproj[self.pres] = 0.0
+        dproj[self.void] = 0.0
         dc = self.filt.T @ (dc_xp * dproj_e / self.hs)
         dv = self.filt.T @ (self.wdesign * dproj / self.hs)
         return comp, dc, dv, xp, us
 
 
-def optimize_loop(p, opts: LoopOptions, preserve=None, weights=None, solve=None, element="im", on_iter=None):
+def optimize_loop(p, opts: LoopOptions, preserve=None, weights=None, solve=None, element="im", on_iter=None,
+                  void=None):
     """simp._optimize_run's loop (simp.py:1840-2449) on a box. Returns
     dict(x, xphys, history, converged, iterations)."""
     t0 = time.time()
-    L = Loop(p, opts, preserve, weights, solve, element)
+    L = Loop(p, opts, preserve, weights, solve, element, void)
     x = np.full(L.n, opts.volume_fraction)
     x[L.pres] = 1.0
+    x[L.void] = 0.0
     beta = float(opts.beta_start)
     since_beta = 0
     measured = opts.adaptive_move
@@ -249,6 +258,7 @@ def optimize_loop(p, opts: LoopOptions, preserve=None, weights=None, solve=None,
         def stepped(lm_):
             xn = np.clip(x * ratio / np.sqrt(lm_), lo_x, hi_x)
             xn[L.pres] = 1.0
+            xn[L.void] = 0.0
             v = float(L.physical(xn, beta)[1] @ L.wdesign)
             return xn, v - L.target
 

@@ -32,35 +32,47 @@ def _oracle_problem(p):
     return os_.Problem(p.name, p.dims, p.volfrac, p.rmin, list(p.fixed), [list(c) for c in p.cases])
 
 
-def test_loop_matches_oracle(gpu):
-    """FP64 device loop (PCG to 1e-11) against the oracle's direct solves,
-    25 iterations: two beta doublings, the measured step, weighted cases
-    and a preserved pad."""
+@pytest.mark.parametrize("variant", ["measured", "fixed-step"])
+def test_loop_matches_oracle(gpu, variant):
+    """FP64 device loop (PCG to 1e-11) against the oracle's direct solves.
+    measured: 25 iterations, two beta doublings, the measured step, weighted
+    cases and a preserved pad. fixed-step: the fixed limits (move, move / 2
+    past beta 4), a per-level schedule, a thinner erosion and a void block."""
     from loadpath_amd.simp import SimpOptions, optimize
     from oracle.loop import LoopOptions, optimize_loop
 
     p = _problem()
     pres = _preserve(p.dims)
     w = [1.0, 0.5]
+    void = None
+    kw = {}
+    if variant == "fixed-step":
+        void = np.zeros(p.dims, bool)
+        void[6:10, :, 2:4] = True
+        kw = dict(adaptive_move=False, beta_schedule=(3, 4), erode=0.1)
     n_it = 25
-    opts = SimpOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=n_it, solve_rtol=1e-11)
-    r = optimize(p, opts, preserve=pres, case_weights=w, precision="fp64", element="im", maxiter=3000,
+    opts = SimpOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=n_it, solve_rtol=1e-11, **kw)
+    r = optimize(p, opts, preserve=pres, void=void, case_weights=w, precision="fp64", element="im", maxiter=3000,
                  coarse_cells=50)
-    ro = optimize_loop(_oracle_problem(p), LoopOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=n_it),
-                       preserve=pres.ravel(), weights=w, element="im")
-    assert len(r.history) == len(ro["history"]) == n_it
+    ro = optimize_loop(_oracle_problem(p), LoopOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=n_it,
+                                                       **kw),
+                       preserve=pres.ravel(), weights=w, element="im", void=None if void is None else void.ravel())
+    assert len(r.history) == len(ro["history"])
     for a, b in zip(r.history, ro["history"]):
-        print(a["iteration"], a["compliance"], b["compliance"], a["beta"], a["move"], b["move"], a["change"],
-              b["change"], a["volumeSolves"], b["volumeSolves"])
+        print(a["iteration"], a["compliance"], b["compliance"], a["beta"], a.get("move"), b.get("move"),
+              a["change"], b["change"], a["volumeSolves"], b["volumeSolves"])
         assert a["beta"] == b["beta"]
         assert a["compliance"] == pytest.approx(b["compliance"], rel=1e-7)
         assert a["volume"] == pytest.approx(b["volume"], abs=1e-9)
-        assert a["move"] == pytest.approx(b["move"], abs=1e-7)
+        assert a.get("move", 0.0) == pytest.approx(b.get("move", 0.0), abs=1e-7)
         assert a["change"] == pytest.approx(b["change"], abs=1e-6)
         assert a["volHeld"] == b["volHeld"]
         assert a["lambda"] == pytest.approx(b["lambda"], rel=1e-5)
+    assert r.converged == ro["converged"]
     assert np.abs(r.x - ro["x"]).max() < 1e-6
     assert np.abs(r.xphys - ro["xphys"]).max() < 1e-5
+    if void is not None:
+        assert (r.x[void] == 0).all() and (r.xphys[void] == 0).all()
 
 
 def test_loop_properties_mixed_precision(gpu):
