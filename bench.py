@@ -160,6 +160,54 @@ def run_studies_bench(log=None, max_iterations=40):
     }
 
 
+def run_loop_bench(steps, warmup, nx, ny, nz, log=None):
+    """The production loop (simp.optimize's loop: eroded analysis, projection
+    continuation, measured step, multiplier root to 1e-13) on the same
+    256x128x128 multi-load workload: mixed and FP64 iterations/s from the
+    same start, with the loop's own padded filter and E_MIN."""
+    from loadpath_amd.problems import multiload_256x128x128
+    from loadpath_amd.simp import GridSimp, SimpOptions
+    import torch
+
+    out = {}
+    for precision in ("mixed", "fp64"):
+        p = multiload_256x128x128(nx, ny, nz)
+        opts = SimpOptions(volume_fraction=p.volfrac, filter_radius=p.rmin, solve_rtol=1e-8)
+        s = GridSimp(p, precision=precision, smoother="chebyshev", rtol=1e-8, maxiter=500, max_levels=6,
+                     coarse_cells=0, element="im", filter="padded", volfrac=opts.volume_fraction,
+                     rmin=opts.filter_radius)
+        s.set_loop(opts)
+        hist = s.iterate(warmup) if warmup > 0 else []
+        if warmup > 0:
+            s.grid.autotune(s.nrhs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        timed = s.iterate(steps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if log:
+            for st in hist + timed:
+                log(f"  loop {precision}: {st}")
+        out[precision] = dict(value=steps / dt, timed=timed)
+        del s
+    m, f = out["mixed"], out["fp64"]
+    return {
+        "metric": "production-loop SIMP iter/s at 256x128x128 hex8 (eroded analysis, projection continuation, "
+                  "measured step, volume root to 1e-13)",
+        "value": round(m["value"], 4), "unit": "SIMP iter/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(1e3 / m["value"], 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 CG / f32 V-cycle / f16 MFMA fine operator", "data": "synthetic (seeded load cases)",
+        "config": {"workload": "multiload_256x128x128 as the default line, the production loop's options at their "
+                               "defaults (beta 1 -> 32, eta 0.5, erode 0.15, move 0.2 measured), padded filter "
+                               "r 2.5, E_MIN 1e-6, rtol 1e-8", "cells": nx * ny * nz},
+        "fp64_value": round(f["value"], 4), "speedup_vs_fp64": round(m["value"] / f["value"], 3),
+        "compliance_rel_err": max(abs(a["compliance"] - b["compliance"]) / abs(b["compliance"])
+                                  for a, b in zip(m["timed"], f["timed"])),
+        "volume_evaluations_per_step": [st["oc_evals"] for st in m["timed"]],
+        "cg_iters_per_step": [st["cg_iters_total"] for st in m["timed"]],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +220,8 @@ def main():
     ap.add_argument("--studies", action="store_true",
                     help="instead: the 32 uneven batched studies (concurrent, mixed, graphs, autotuned) against "
                          "the same studies one after another on the plain FP64 path")
+    ap.add_argument("--loop", action="store_true",
+                    help="instead: the production loop (simp.optimize's) on the same workload, mixed against FP64")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB elementwise copy (3 times): the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -185,6 +235,9 @@ def main():
         print(json.dumps(run_studies_bench(log)))
         return
     nx, ny, nz = a.size or (WORKLOAD["nx"], WORKLOAD["ny"], WORKLOAD["nz"])
+    if a.loop:
+        print(json.dumps(run_loop_bench(a.steps, a.warmup, nx, ny, nz, log)))
+        return
     if a.copy_calibration:
         import torch
 
