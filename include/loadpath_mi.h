@@ -41,7 +41,7 @@ extern "C" {
 /* precision of the V-cycle (the outer CG is always FP64) */
 #define LP_PREC_FP64 0       /* FP64 everywhere ("plain FP64") */
 #define LP_PREC_FP32 1       /* FP64 CG + FP32 V-cycle: the reference split (mg.py:28-32, gpu.py:483-521) */
-#define LP_PREC_MIXED 2      /* FP64 refinement residual + FP32 inner PCG + FP16-MFMA fine Ke.U */
+#define LP_PREC_MIXED 2      /* FP64 CG + FP32 V-cycle whose two finest operators are FP16-MFMA contractions */
 
 #define LP_SMOOTH_CHEBYSHEV 0  /* mg.py:796-835 */
 #define LP_SMOOTH_JACOBI 1     /* damped Jacobi (configs' V(2,2), omega 0.6) */
@@ -234,6 +234,21 @@ int lp_simp_displacements(lp_simp* s, double* out);
  * grid, zero outside), simp.GridFilter.__matmul__ (simp.py:1164-1172). */
 int lp_filter_apply(int nx, int ny, int nz, double radius, int reach, const double* v, double* out,
                     int device_ptr);
+
+/* verify.nodal_stress / nodal_von_mises (verify.py:197-206, 307-337) on a
+ * voxel grid of nx x ny x nz cells: each present cell's stress at its 8
+ * corners, modulus * C (B u_e / pitch, thermal taken off the normal
+ * strains), averaged at each node over its cells (over its counted cells,
+ * at least 1, when counted is given: verify's stress_mask). element 0 "im",
+ * 1 "plain". u: (nodes, 3) node-major, nodes (nx+1)(ny+1)(nz+1) in C order
+ * (the dof vector of a full-box HexMesh); modulus (cells) or NULL for
+ * modulus_scalar; thermal (cells) or NULL; present / counted (cells) uint8
+ * or NULL (every cell). out (nodes, 6) = (sx, sy, sz, txy, tyz, tzx);
+ * vm (nodes) or NULL = fem.von_mises of it (fem.py:311-314). Nodes no
+ * present cell touches read 0. */
+int lp_nodal_stress(int nx, int ny, int nz, double pitch, double nu, int element, const double* u,
+                    const double* modulus, double modulus_scalar, const double* thermal, const uint8_t* present,
+                    const uint8_t* counted, double* out, double* vm, int device_ptr);
 
 /* ---- diagnostics -------------------------------------------------------- */
 const char* lp_last_error(void);

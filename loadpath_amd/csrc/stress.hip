@@ -202,6 +202,82 @@ const void* upload(Grid& G, DBuf& stage, const void* p, size_t bytes, bool devic
 
 }  // namespace lp
 
+namespace lp {
+namespace {
+
+// verify.nodal_stress (verify.py:307-337), node-centric: one thread per node
+// sums, over its <= 8 cells in a fixed order, the cell's stress at that
+// corner, modulus * (CB_k u_c - thermal C [1 1 1 0 0 0]), and divides by the
+// number of cells it summed (the counted ones). The corner index k of the
+// node in a cell is the same for every thread of a loop step, so the 6 x 24
+// CB_k rows are uniform (scalar) loads.
+struct NsArgs {
+    int nx, ny, nz;
+    const double* u;         // (nodes, 3) node-major
+    const double* mod;       // (cells) or null: mod_scalar
+    double mod_scalar;
+    const double* thermal;   // (cells) or null
+    const uint8_t* present;  // (cells) or null: every cell
+    const uint8_t* counted;  // (cells) or null: the present cells, unmasked
+    const double* cb;        // (8, 6, 24) corners in fem.CORNERS order, / pitch
+    double cth[6];           // C [1 1 1 0 0 0] (E = 1)
+    double* out;             // (nodes, 6)
+    double* vm;              // (nodes) or null
+};
+
+__global__ void __launch_bounds__(256) k_nodal_stress(NsArgs A) {
+    const size_t Ny = (size_t)A.ny + 1, Nz = (size_t)A.nz + 1;
+    const size_t nn = ((size_t)A.nx + 1) * Ny * Nz;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    const int X = (int)(n / (Ny * Nz)), Y = (int)((n / Nz) % Ny), Z = (int)(n % Nz);
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    int cnt = 0, touched = 0;
+#pragma unroll 1
+    for (int k = 0; k < 8; ++k) {
+        // the cell whose corner k is this node
+        const int cx = X - (k & 1), cy = Y - ((k >> 1) & 1), cz = Z - ((k >> 2) & 1);
+        if (cx < 0 || cy < 0 || cz < 0 || cx >= A.nx || cy >= A.ny || cz >= A.nz) continue;
+        const size_t c = ((size_t)cx * A.ny + cy) * A.nz + cz;
+        if (A.present && !A.present[c]) continue;
+        ++touched;
+        const bool counts = !A.counted || A.counted[c];
+        if (!counts) continue;
+        ++cnt;
+        const double m = A.mod ? A.mod[c] : A.mod_scalar;
+        double ue[24];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const size_t node = ((size_t)(cx + (b & 1)) * Ny + (cy + ((b >> 1) & 1))) * Nz + (cz + ((b >> 2) & 1));
+#pragma unroll
+            for (int d = 0; d < 3; ++d) ue[3 * b + d] = A.u[3 * node + d];
+        }
+        const double* cb = A.cb + (size_t)k * 144;
+        const double th = A.thermal ? A.thermal[c] : 0.0;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            double t = 0;
+#pragma unroll
+            for (int j = 0; j < 24; ++j) t += cb[r * 24 + j] * ue[j];
+            acc[r] += (t - th * A.cth[r]) * m;
+        }
+    }
+    // the reference divides by the cells of the stress mask (at least 1)
+    // when one is given, else by every cell at the node
+    const double div = A.counted ? (double)(cnt > 0 ? cnt : 1) : (double)(touched > 0 ? touched : 1);
+#pragma unroll
+    for (int r = 0; r < 6; ++r) A.out[6 * n + r] = acc[r] / div;
+    if (A.vm) {
+        const double sx = acc[0] / div, sy = acc[1] / div, sz = acc[2] / div;
+        const double txy = acc[3] / div, tyz = acc[4] / div, tzx = acc[5] / div;
+        A.vm[n] = sqrt(0.5 * ((sx - sy) * (sx - sy) + (sy - sz) * (sy - sz) + (sz - sx) * (sz - sx)) +
+                       3.0 * (txy * txy + tyz * tyz + tzx * tzx));
+    }
+}
+
+}  // namespace
+}  // namespace lp
+
 using namespace lp;
 
 struct lp_grid : public Grid {};
@@ -310,6 +386,71 @@ int lp_element_product(lp_grid* g, int nrhs, const double* a, const double* b, d
         LP_HIP(hipGetLastError());
         if (!device_ptr) LP_HIP(hipMemcpyAsync(out, dout, g0.ncell * nrhs * 8, hipMemcpyDeviceToHost, s));
         LP_HIP(hipStreamSynchronize(s));
+        return LP_OK;
+    });
+}
+
+int lp_nodal_stress(int nx, int ny, int nz, double pitch, double nu, int element, const double* u,
+                    const double* modulus, double modulus_scalar, const double* thermal, const uint8_t* present,
+                    const uint8_t* counted, double* out, double* vm, int device_ptr) {
+    return api_call([&]() -> int {
+        if (nx < 1 || ny < 1 || nz < 1 || !(pitch > 0.0) || !(nu > -1.0 && nu < 0.5) || (element != 0 && element != 1) ||
+            !u || !out) {
+            set_error("lp_nodal_stress: bad argument");
+            return LP_ERR_ARG;
+        }
+        const size_t nc = (size_t)nx * ny * nz;
+        const size_t nn = (size_t)(nx + 1) * (ny + 1) * (nz + 1);
+        // C B at the corners (lp_hex8_cb: x outermost) into fem.CORNERS order
+        // (bit0 = x), divided by the pitch
+        std::vector<double> cbx(8 * 144), cb(8 * 144);
+        hex8_cb_host(nu, element, 1, cbx.data());
+        for (int k = 0; k < 8; ++k) {
+            const int src = 4 * (k & 1) + 2 * ((k >> 1) & 1) + ((k >> 2) & 1);
+            for (int i = 0; i < 144; ++i) cb[k * 144 + i] = cbx[src * 144 + i] / pitch;
+        }
+        const double lam = nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = 1.0 / (2.0 * (1.0 + nu));
+        NsArgs A{};
+        A.nx = nx;
+        A.ny = ny;
+        A.nz = nz;
+        for (int r = 0; r < 6; ++r) A.cth[r] = r < 3 ? 3.0 * lam + 2.0 * mu : 0.0;
+        A.mod_scalar = modulus_scalar;
+        DBuf du, dm, dt, dp, dc, dcb, dout, dvm;
+        auto up = [&](DBuf& b, const void* h, size_t bytes) -> const void* {
+            if (!h) return nullptr;
+            if (device_ptr) return h;
+            b.alloc(bytes);
+            LP_HIP(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice));
+            return b.p;
+        };
+        A.u = static_cast<const double*>(up(du, u, nn * 3 * 8));
+        A.mod = static_cast<const double*>(up(dm, modulus, nc * 8));
+        A.thermal = static_cast<const double*>(up(dt, thermal, nc * 8));
+        A.present = static_cast<const uint8_t*>(up(dp, present, nc));
+        A.counted = static_cast<const uint8_t*>(up(dc, counted, nc));
+        dcb.alloc(cb.size() * 8);
+        LP_HIP(hipMemcpy(dcb.p, cb.data(), cb.size() * 8, hipMemcpyHostToDevice));
+        A.cb = dcb.as<double>();
+        if (device_ptr) {
+            A.out = out;
+            A.vm = vm;
+        } else {
+            dout.alloc(nn * 6 * 8);
+            A.out = dout.as<double>();
+            if (vm) {
+                dvm.alloc(nn * 8);
+                A.vm = dvm.as<double>();
+            }
+        }
+        hipLaunchKernelGGL(k_nodal_stress, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, nullptr, A);
+        LP_HIP(hipGetLastError());
+        if (!device_ptr) {
+            LP_HIP(hipMemcpy(out, A.out, nn * 6 * 8, hipMemcpyDeviceToHost));
+            if (vm) LP_HIP(hipMemcpy(vm, A.vm, nn * 8, hipMemcpyDeviceToHost));
+        } else {
+            LP_HIP(hipDeviceSynchronize());
+        }
         return LP_OK;
     });
 }

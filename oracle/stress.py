@@ -25,6 +25,17 @@ Pinned by (tests/test_oracle_stress.py):
   seeded elements of a small clamped bar, centre and corner sampling;
 - the compliance gradient -p x^(p-1) (1-Emin) u_e^T Ke u_e against the same
   finite differences (the self-adjoint case).
+
+Also verify.nodal_stress / nodal_von_mises (verify.py:197-206, 307-337):
+each cell's stress at its own 8 corners (fem.CORNERS order, bit0 = x),
+modulus * C (B u_e / pitch, thermal taken off the normal strains), summed
+at the corner's node and divided by the number of cells there (the cells
+of a stress mask only, when one is given); fem.von_mises (fem.py:311-314).
+Pinned (tests/test_oracle_stress.py) by the reference's own closed forms
+(tests/test_hex8_im.py:27-84): the constant-strain patch test returns
+C eps at every node, and a one-cell-deep strip in pure bending reads the
+top-fibre stress M c / I with the "im" element; a free thermal expansion
+reads zero stress; and a per-cell loop over corners (no np.add.at).
 """
 from __future__ import annotations
 
@@ -139,3 +150,30 @@ def pnorm_value(mesh, ke, fixed, f, xphys, cbs, penal=3.0, emin=1e-9, p=P_DEFAUL
     u = np.zeros(mesh.n_dof)
     u[free] = spla.spsolve(k[free][:, free].tocsc(), f[free])
     return pnorm_forward(u[mesh.edof], x, cbs, p, q, lim, kappa)[0]
+
+
+def nodal_stress(mesh, u, modulus, nu: float, element: str = "im", thermal=None, mask=None):
+    """verify.nodal_stress (verify.py:307-337) on a HexMesh: (n_nodes, 6)."""
+    c = isotropic_c(1.0, nu)
+    mod = np.broadcast_to(np.asarray(modulus, float), (mesh.n_elem,))[:, None]
+    if mask is not None:
+        mod = mod * np.asarray(mask, float)[:, None]
+    ue = np.asarray(u, float)[mesh.edof]
+    acc = np.zeros((mesh.n_nodes, 6))
+    for k in range(8):
+        a, b, cc = 2.0 * (k & 1) - 1.0, 2.0 * ((k >> 1) & 1) - 1.0, 2.0 * ((k >> 2) & 1) - 1.0
+        strain = ue @ (hex8_b(a, b, cc, nu, element) / mesh.pitch).T
+        if thermal is not None:
+            strain[:, :3] -= np.asarray(thermal, float)[:, None]
+        np.add.at(acc, mesh.enode[:, k], (strain @ c.T) * mod)
+    if mask is not None:
+        cnt = np.bincount(mesh.enode[np.asarray(mask, bool)].ravel(), minlength=mesh.n_nodes)
+        return acc / np.maximum(cnt, 1)[:, None]
+    cnt = np.bincount(mesh.enode.ravel(), minlength=mesh.n_nodes)
+    return acc / cnt[:, None]
+
+
+def von_mises(stress):
+    """fem.von_mises (fem.py:311-314)."""
+    sx, sy, sz, txy, tyz, tzx = np.asarray(stress, float).T
+    return np.sqrt(0.5 * ((sx - sy) ** 2 + (sy - sz) ** 2 + (sz - sx) ** 2) + 3.0 * (txy ** 2 + tyz ** 2 + tzx ** 2))

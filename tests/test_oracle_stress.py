@@ -117,3 +117,102 @@ def test_compliance_gradient_matches_central_differences():
         xm[e] -= h
         fd = (comp(xp)[0] - comp(xm)[0]) / (2 * h)
         assert dc[e] == pytest.approx(fd, rel=1e-5)
+
+
+# ---- verify.nodal_stress (oracle/stress.py nodal_stress) -------------------
+
+def _mesh(ijk, pitch=1.0, origin=(0.0, 0.0, 0.0)):
+    from oracle.fem import Mesh
+
+    return Mesh(np.asarray(ijk), pitch, origin)
+
+
+@pytest.mark.parametrize("element", ["im", "plain"])
+def test_nodal_stress_constant_strain_patch(element):
+    """A linear field on a 3x3x3 block returns C eps at every node
+    (the reference's tests/test_hex8_im.py:27-43)."""
+    e, nu = 1000.0, 0.3
+    mesh = _mesh(np.argwhere(np.ones((3, 3, 3), bool)), 0.7, (0.1, -0.2, 0.3))
+    a = np.array([[1e-3, 2e-4, -3e-4], [5e-4, -2e-3, 1e-4], [-1e-4, 3e-4, 7e-4]])
+    x = mesh.node_ijk * mesh.pitch
+    u = (x @ a.T).ravel()
+    eps = np.array([a[0, 0], a[1, 1], a[2, 2], a[0, 1] + a[1, 0], a[1, 2] + a[2, 1], a[2, 0] + a[0, 2]])
+    sig = isotropic_c(e, nu) @ eps
+    s = S.nodal_stress(mesh, u, e, nu, element)
+    assert np.allclose(s, sig[None], rtol=0, atol=1e-9 * np.abs(sig).max())
+    # a free thermal expansion of the same size reads no stress
+    th = 4e-4
+    s0 = S.nodal_stress(mesh, (x * th).ravel(), e, nu, element, thermal=np.full(mesh.n_elem, th))
+    assert np.abs(s0).max() < 1e-9 * e * th
+
+
+def _pure_bending(t, element, w=2, length=12):
+    """A strip t cells deep under a couple on its end face, the reference's
+    set-up (tests/test_hex8_im.py:46-73): top-fibre stress / (M c / I)."""
+    import scipy.sparse.linalg as spla
+    from oracle.fem import assemble
+
+    e, nu = 1000.0, 0.3
+    mesh = _mesh([(i, j, k) for i in range(length) for j in range(t) for k in range(w)])
+    x = mesh.node_ijk.astype(float)
+    yc = t / 2.0
+    root, end = np.isclose(x[:, 0], 0), np.isclose(x[:, 0], length)
+    fixed = np.zeros(mesh.n_dof, bool)
+    fixed[3 * np.nonzero(root)[0]] = True
+    n0 = np.nonzero(root & np.isclose(x[:, 1], 0) & np.isclose(x[:, 2], 0))[0][0]
+    n1 = np.nonzero(root & np.isclose(x[:, 1], 0) & np.isclose(x[:, 2], w))[0][0]
+    fixed[[3 * n0 + 1, 3 * n0 + 2, 3 * n1 + 1]] = True
+    f = np.zeros(mesh.n_dof)
+    for n in np.nonzero(end)[0]:
+        y, z = x[n, 1], x[n, 2]
+        fy = sum((y - yc) / 3.0 + (y + d - yc) / 6.0 for d in (-1, 1) if 0 <= y + d <= t)
+        fz = sum(0.5 for d in (-1, 1) if 0 <= z + d <= w)
+        f[3 * n] = fy * fz
+    k = assemble(mesh, hex8_ke(nu, element), np.full(mesh.n_elem, e))
+    free = ~fixed
+    u = np.zeros(mesh.n_dof)
+    u[free] = spla.spsolve(k[free][:, free].tocsc(), f[free])
+    s = S.nodal_stress(mesh, u, e, nu, element)
+    top = np.isclose(x[:, 1], t) & (x[:, 0] > 2) & (x[:, 0] < length - 2)
+    return s[top, 0].mean() / yc, (mesh, u, e, nu)
+
+
+def test_nodal_stress_pure_bending_closed_form():
+    """The incompatible-mode element reads M c / I one cell deep; the plain
+    trilinear one shear-locks (tests/test_hex8_im.py:76-83)."""
+    for t in (1, 2):
+        assert _pure_bending(t, "im")[0] == pytest.approx(1.0, abs=1e-6)
+    assert _pure_bending(1, "plain")[0] < 0.8
+
+
+def test_nodal_stress_matches_a_per_cell_loop():
+    """Against a plain loop over cells and corners (no np.add.at), with a
+    stress mask, per-cell moduli and a thermal strain."""
+    from oracle.fem import hex8_b
+
+    rng = np.random.default_rng(2)
+    g = rng.random((4, 3, 3)) < 0.8
+    mesh = _mesh(np.argwhere(g), 0.5)
+    nu = 0.3
+    u = rng.standard_normal(mesh.n_dof)
+    mod = rng.uniform(0.5, 2.0, mesh.n_elem)
+    th = rng.uniform(-1e-2, 1e-2, mesh.n_elem)
+    mask = rng.random(mesh.n_elem) < 0.7
+    s = S.nodal_stress(mesh, u, mod, nu, "im", thermal=th, mask=mask)
+    c = isotropic_c(1.0, nu)
+    acc = np.zeros((mesh.n_nodes, 6))
+    cnt = np.zeros(mesh.n_nodes)
+    for ci in range(mesh.n_elem):
+        if not mask[ci]:
+            continue
+        ue = u[mesh.edof[ci]]
+        for k in range(8):
+            b = hex8_b(2.0 * (k & 1) - 1.0, 2.0 * ((k >> 1) & 1) - 1.0, 2.0 * ((k >> 2) & 1) - 1.0, nu, "im") / 0.5
+            strain = b @ ue
+            strain[:3] -= th[ci]
+            acc[mesh.enode[ci, k]] += mod[ci] * (c @ strain)
+            cnt[mesh.enode[ci, k]] += 1
+    want = acc / np.maximum(cnt, 1)[:, None]
+    assert np.allclose(s, want, rtol=1e-12, atol=1e-12)
+    vm = S.von_mises(s)
+    assert np.allclose(vm, np.sqrt(np.einsum("ni,ij,nj->n", s, S.VM, s)), rtol=1e-12)
