@@ -404,6 +404,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
         };
         T acc0 = 0, acc1 = 0, acc2 = 0, nx0 = 0, nx1 = 0, nx2 = 0;
         unsigned long long kea = (unsigned long long)ke;
+        // the distinct Ke values in VGPRs: as kernel arguments they sit in
+        // SGPRs, and with the rest of the arguments the compiler spilled them
+        // to VGPR lanes and read one back (v_readlane) for nearly every FMA
+        T kvv[KE_NV];
+#pragma unroll
+        for (int i = 0; i < KE_NV; ++i) {
+            kvv[i] = kv.v[i];
+            asm volatile("" : "+v"(kvv[i]));
+        }
         // cell layer X between staged planes in buffers bl (X) and bh (X+1)
         auto layer = [&](int X, int bl, int bh, bool lo, const T (&ecv)[4]) {
             nx0 = 0; nx1 = 0; nx2 = 0;
@@ -423,7 +432,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
                     if (!SYM) asm volatile("" : "+s"(kea));
                     cptr<T> Kp = (cptr<T>)kea;
                     auto K = [&](int e) -> T {
-                        if (SYM) return KE_SGN[e] > 0 ? kv.v[KE_IDX[e]] : (KE_SGN[e] < 0 ? -kv.v[KE_IDX[e]] : T(0));
+                        if (SYM) return KE_SGN[e] > 0 ? kvv[KE_IDX[e]] : (KE_SGN[e] < 0 ? -kvv[KE_IDX[e]] : T(0));
                         return Kp[e];
                     };
                     // corner by corner (3 operand values live), each row's
