@@ -11,7 +11,10 @@ short and launch latency would dominate.
     res = run_studies(problems, precision="mixed", max_iterations=40)
     res["studies_per_s"], res["p50_s"], res["p99_s"]
 
-A study ends at the top3d stop (change <= 0.01) or at max_iterations.
+A study ends at the top3d stop (change <= 0.01) or at max_iterations; with
+``loop=True`` each study runs the production loop instead (simp.optimize's:
+eroded analysis, projection continuation, measured step, multiplier root;
+loadpath_amd.simp.optimize) and ends at its stop test.
 """
 from __future__ import annotations
 
@@ -23,20 +26,25 @@ import numpy as np
 __all__ = ["run_studies"]
 
 
-def _one(problem, precision, max_iterations, tol, graph, autotune, out, k, start_barrier):
+def _one(problem, precision, max_iterations, tol, graph, autotune, out, k, start_barrier, loop=False):
     try:
-        _run_one(problem, precision, max_iterations, tol, graph, autotune, out, k, start_barrier)
+        _run_one(problem, precision, max_iterations, tol, graph, autotune, out, k, start_barrier, loop)
     except Exception as e:  # reported by run_studies
         if start_barrier is not None:
             start_barrier.abort()
         out[k] = dict(name=problem.name, error=f"{type(e).__name__}: {e}")
 
 
-def _run_one(problem, precision, max_iterations, tol, graph, autotune, out, k, start_barrier):
+def _run_one(problem, precision, max_iterations, tol, graph, autotune, out, k, start_barrier, loop=False):
     from . import _lib
-    from .simp import GridSimp
+    from .simp import GridSimp, SimpOptions
 
-    s = GridSimp(problem, precision=precision, rtol=1e-8, maxiter=500, coarse_cells=500)
+    if loop:
+        s = GridSimp(problem, precision=precision, rtol=1e-8, maxiter=500, coarse_cells=500, filter="padded")
+        s.set_loop(SimpOptions(volume_fraction=problem.volfrac, filter_radius=problem.rmin,
+                               max_iterations=max_iterations))
+    else:
+        s = GridSimp(problem, precision=precision, rtol=1e-8, maxiter=500, coarse_cells=500)
     lib = _lib.lib()
     lib.lp_tune(s.grid._h, 3, 1 if graph else 0)
     if start_barrier is not None:
@@ -48,14 +56,14 @@ def _run_one(problem, precision, max_iterations, tol, graph, autotune, out, k, s
         hist.append(st)
         if autotune and len(hist) == 1:
             s.grid.autotune(s.nrhs)
-        if st["change"] <= tol:
+        if (st["converged"] if loop else st["change"] <= tol):
             break
     out[k] = dict(name=problem.name, seconds=time.perf_counter() - t0, iterations=len(hist),
                   compliance=hist[-1]["compliance"], history=hist, xphys=s.xphys)
 
 
 def run_studies(problems, precision="mixed", max_iterations=40, tol=0.01, concurrent=True, graph=True,
-                autotune=True):
+                autotune=True, loop=False):
     """Run the studies concurrently (one host thread and stream each) or one
     after another. Returns per-study results and the batch's studies/s and
     p50 / p99 time to converge (seconds per study, from its first
@@ -66,14 +74,14 @@ def run_studies(problems, precision="mixed", max_iterations=40, tol=0.01, concur
     if concurrent:
         barrier = threading.Barrier(n)
         threads = [threading.Thread(target=_one, args=(p, precision, max_iterations, tol, graph, autotune, out, k,
-                                                       barrier)) for k, p in enumerate(problems)]
+                                                       barrier, loop)) for k, p in enumerate(problems)]
         for t in threads:
             t.start()
         for t in threads:
             t.join()
     else:
         for k, p in enumerate(problems):
-            _one(p, precision, max_iterations, tol, graph, autotune, out, k, None)
+            _one(p, precision, max_iterations, tol, graph, autotune, out, k, None, loop)
     wall = time.perf_counter() - t0
     bad = [o for o in out if o is None or "error" in o]
     if bad:

@@ -40,6 +40,22 @@ def test_concurrent_studies_match_sequential(gpu):
         assert np.array_equal(sa["xphys"], sb["xphys"])
 
 
+def test_concurrent_production_loops_match_sequential(gpu):
+    """Four production-loop studies at once (one thread and stream each)
+    give what they give one after another, bit for bit."""
+    from loadpath_amd.batch import run_studies
+    from loadpath_amd.problems import batched_studies
+
+    ps = batched_studies(count=4, seed=11)
+    a = run_studies(ps, precision="mixed", max_iterations=6, concurrent=True, autotune=False, loop=True)
+    b = run_studies(ps, precision="mixed", max_iterations=6, concurrent=False, autotune=False, loop=True)
+    for sa, sb in zip(a["studies"], b["studies"]):
+        assert sa["iterations"] == sb["iterations"] == 6
+        assert [h["compliance"] for h in sa["history"]] == [h["compliance"] for h in sb["history"]]
+        assert [h["beta"] for h in sa["history"]] == [h["beta"] for h in sb["history"]]
+        assert np.array_equal(sa["xphys"], sb["xphys"])
+
+
 def test_autotune_keeps_results(gpu):
     from loadpath_amd.mg import GridElasticity
     from tests.conftest import cantilever_bar
