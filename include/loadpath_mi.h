@@ -250,6 +250,13 @@ int lp_nodal_stress(int nx, int ny, int nz, double pitch, double nu, int element
                     const double* modulus, double modulus_scalar, const double* thermal, const uint8_t* present,
                     const uint8_t* counted, double* out, double* vm, int device_ptr);
 
+/* 26-connected components of a cell mask, scipy.ndimage.label(mask,
+ * structure=ones((3,3,3))) as simp._Analysis._connected and _prune_dead use
+ * it (simp.py:264-282, 1276-1282): labels (nx, ny, nz) int32, 0 off the
+ * mask, components 1..count numbered by their first cell in C order.
+ * Host arrays. */
+int lp_label26(int nx, int ny, int nz, const uint8_t* mask, int* labels, int* count);
+
 /* ---- diagnostics -------------------------------------------------------- */
 const char* lp_last_error(void);
 int lp_version(void);

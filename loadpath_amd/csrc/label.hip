@@ -1,0 +1,133 @@
+// 26-connected component labelling of a cell mask on the device: the
+// island test of the SIMP analysis, scipy.ndimage.label(g, structure =
+// ones((3, 3, 3))) as simp._Analysis._connected and simp._prune_dead call it
+// (simp.py:264-282, 1276-1282), with the reference's numbering: components
+// 1..n in the order of their first cell in C order.
+//
+// Each foreground cell starts as its own root (its linear index). Rounds of
+// hooking (a cell's root is hooked to the smallest root among its 26
+// neighbours with an atomic min on the root) and pointer jumping (every
+// cell follows its root's root to the end) repeat until a round changes
+// nothing; the root of a component is then its smallest cell index, so the
+// result does not depend on the order the rounds ran in. Numbering: a
+// prefix count of the roots in C order.
+#include <vector>
+
+#include "lp_internal.h"
+
+namespace lp {
+namespace {
+
+constexpr int kB = 256;
+
+__global__ void k_label_init(const uint8_t* __restrict__ m, int n, int* lab) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) lab[i] = m[i] ? i : -1;
+}
+
+// the root of i (lab[r] == r), following parents
+__device__ __forceinline__ int find_root(const int* lab, int i) {
+    int r = lab[i];
+    while (r >= 0 && lab[r] != r) r = lab[r];
+    return r;
+}
+
+// hook: the root of each foreground cell takes the smallest root among the
+// cell's foreground neighbours (atomic min: roots only decrease)
+__global__ void k_label_hook(int nx, int ny, int nz, int* lab, int* changed) {
+    const int n = nx * ny * nz;
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n || lab[i] < 0) return;
+    const int x = i / (ny * nz), y = (i / nz) % ny, z = i % nz;
+    const int ri = find_root(lab, i);
+    int best = ri;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int X = x + dx, Y = y + dy, Z = z + dz;
+                if (X < 0 || Y < 0 || Z < 0 || X >= nx || Y >= ny || Z >= nz) continue;
+                const int j = (X * ny + Y) * nz + Z;
+                if (lab[j] < 0) continue;
+                const int rj = find_root(lab, j);
+                best = rj < best ? rj : best;
+            }
+    if (best < ri) {
+        atomicMin(&lab[ri], best);
+        *changed = 1;
+    }
+}
+
+// pointer jumping: every cell points at its root
+__global__ void k_label_jump(int n, int* lab) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n || lab[i] < 0) return;
+    lab[i] = find_root(lab, i);
+}
+
+// roots in C order -> 1..n: flag[i] = 1 where i is a root
+__global__ void k_label_roots(int n, const int* lab, int* flag) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) flag[i] = lab[i] == i ? 1 : 0;
+}
+
+__global__ void k_label_number(int n, const int* lab, const int* num, int* out) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) out[i] = lab[i] < 0 ? 0 : num[lab[i]];
+}
+
+}  // namespace
+}  // namespace lp
+
+using namespace lp;
+
+extern "C" {
+
+int lp_label26(int nx, int ny, int nz, const uint8_t* mask, int* labels, int* count) {
+    return api_call([&]() -> int {
+        if (nx < 1 || ny < 1 || nz < 1 || !mask || !labels || !count ||
+            (size_t)nx * ny * nz >= (size_t)0x7fffffff) {
+            set_error("lp_label26: bad argument");
+            return LP_ERR_ARG;
+        }
+        const int n = nx * ny * nz;
+        const unsigned nb = (unsigned)((n + kB - 1) / kB);
+        DBuf dm, dl, dc, dout;
+        dm.alloc(n);
+        dl.alloc((size_t)n * 4);
+        dc.alloc(4);
+        dout.alloc((size_t)n * 4);
+        LP_HIP(hipMemcpy(dm.p, mask, n, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_label_init, dim3(nb), dim3(kB), 0, nullptr, dm.as<uint8_t>(), n, dl.as<int>());
+        LP_HIP(hipGetLastError());
+        // rounds until one changes nothing: each lowers at least one root
+        // (roots only decrease, parents always point to smaller indices, so
+        // there are no cycles), and n + 1 rounds bound them
+        for (int round = 0; round < n + 1; ++round) {
+            int changed = 0;
+            LP_HIP(hipMemcpy(dc.p, &changed, 4, hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_label_hook, dim3(nb), dim3(kB), 0, nullptr, nx, ny, nz, dl.as<int>(), dc.as<int>());
+            hipLaunchKernelGGL(k_label_jump, dim3(nb), dim3(kB), 0, nullptr, n, dl.as<int>());
+            LP_HIP(hipGetLastError());
+            LP_HIP(hipMemcpy(&changed, dc.p, 4, hipMemcpyDeviceToHost));
+            if (!changed) break;
+        }
+        // number the roots in C order (host prefix sum over the root flags)
+        hipLaunchKernelGGL(k_label_roots, dim3(nb), dim3(kB), 0, nullptr, n, dl.as<int>(), dout.as<int>());
+        LP_HIP(hipGetLastError());
+        std::vector<int> flag(n);
+        LP_HIP(hipMemcpy(flag.data(), dout.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+        int k = 0;
+        for (int i = 0; i < n; ++i) flag[i] = flag[i] ? ++k : 0;
+        LP_HIP(hipMemcpy(dout.p, flag.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+        DBuf dnum;
+        dnum.alloc((size_t)n * 4);
+        hipLaunchKernelGGL(k_label_number, dim3(nb), dim3(kB), 0, nullptr, n, dl.as<int>(), dout.as<int>(),
+                           dnum.as<int>());
+        LP_HIP(hipGetLastError());
+        LP_HIP(hipMemcpy(labels, dnum.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+        *count = k;
+        return LP_OK;
+    });
+}
+
+}  // extern "C"

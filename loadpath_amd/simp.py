@@ -338,6 +338,29 @@ def blueprint(x, opts: SimpOptions, *, pad=True, beta=None, preserve=None, void=
     return xp
 
 
+def label26(mask):
+    """scipy.ndimage.label(mask, structure=np.ones((3, 3, 3))) on the device
+    (lp_label26): (labels int32 of mask's shape, count), components
+    numbered by their first cell in C order."""
+    _lib.require_gpu()
+    m = np.ascontiguousarray(np.asarray(mask, bool), np.uint8)
+    if m.ndim != 3:
+        raise ValueError("label26 takes a 3-D cell mask")
+    labels = np.empty(m.shape, np.int32)
+    n = C.c_int()
+    _lib.check(_lib.lib().lp_label26(*m.shape, _lib.ptr(m), _lib.ptr(labels), C.byref(n)), "label26")
+    return labels, n.value
+
+
+def connected(alive, anchor):
+    """The cells of `alive` in a 26-connected piece that holds an `anchor`
+    cell (simp._Analysis._connected, simp.py:264-282: the supports' cells)."""
+    alive = np.asarray(alive, bool)
+    lbl, _ = label26(alive)
+    keep = np.unique(lbl[alive & np.asarray(anchor, bool)])
+    return alive & np.isin(lbl, keep[keep > 0])
+
+
 def optimize_top3d(problem, iterations=None, maxit=200, on_iter=None, **kw):
     """benchmarks/reference.optimize_reference(p, "top3d") on the GPU
     (reference.py:214-257): stop at change <= 0.01 (or after `iterations`)."""
