@@ -87,6 +87,11 @@ void lp_grid_destroy(lp_grid* g);
  * caller then calls lp_estimate_lmax per level). */
 int lp_set_moduli(lp_grid* g, const double* e_cells, int flags);
 
+/* GridElasticity.rtol / .maxiter, which the reference's callers set before
+ * a solve (simp.py:734-746; the delivery solves at 1e-6, simp.py:2481):
+ * the tolerance and cap of the next lp_solve calls. */
+int lp_set_solve_tol(lp_grid* g, double rtol, int maxiter);
+
 /* GridElasticity._estimate_lmax (mg.py:771-794): 12 power iterations of
  * D^-1 A at `level`, times 1.3. start: host (3, nodes) start vector, or NULL
  * for the level's last vector (warm start) or a fixed hash start. */

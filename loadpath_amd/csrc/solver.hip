@@ -864,6 +864,18 @@ int lp_profile_stats(lp_grid* g, long long* launches, double* total_ms, double* 
     });
 }
 
+int lp_set_solve_tol(lp_grid* g, double rtol, int maxiter) {
+    return guarded([&]() -> int {
+        if (!g || !(rtol > 0.0) || maxiter < 1) {
+            set_error("lp_set_solve_tol: bad argument");
+            return LP_ERR_ARG;
+        }
+        g->opt.rtol = rtol;
+        g->opt.maxiter = maxiter;
+        return LP_OK;
+    });
+}
+
 int lp_tune(lp_grid* g, int param, int value) {
     if (!g) return LP_ERR_ARG;
     if (param == 0) {

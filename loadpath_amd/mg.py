@@ -86,8 +86,8 @@ class GridElasticity:
         _lib.require_gpu()
         self.mesh = mesh
         self.nu = nu
-        self.rtol = float(rtol)
-        self.maxiter = int(maxiter)
+        self._rtol = float(rtol)
+        self._maxiter = int(maxiter)
         self.coarse_cells = int(coarse_cells)
         self.degree = int(degree)
         self.dtype = np.dtype(dtype)
@@ -117,7 +117,7 @@ class GridElasticity:
         self.last_iters = 0
         self._lmax_started = set()
         self._mask = None
-        opt = _lib.Options(rtol=self.rtol, maxiter=self.maxiter, coarse_cells=self.coarse_cells, degree=self.degree,
+        opt = _lib.Options(rtol=self._rtol, maxiter=self._maxiter, coarse_cells=self.coarse_cells, degree=self.degree,
                            smoother=_lib.SMOOTH[smoother], omega=self.omega, precision=_lib.PREC[precision],
                            max_rhs=self.max_rhs, lmax_iters=12, max_levels=int(max_levels))
         self._opt = opt
@@ -254,6 +254,28 @@ class GridElasticity:
             _lib.check(rc, "solve")
         self._refresh(lmax=False)
         return x, its, ok
+
+    # rtol / maxiter: the reference's callers set them before a solve
+    # (simp.py:734-746); they reach the device handle at once
+    @property
+    def rtol(self):
+        return self._rtol
+
+    @rtol.setter
+    def rtol(self, v):
+        self._rtol = float(v)
+        if getattr(self, "_h", None) is not None:
+            _lib.check(_lib.lib().lp_set_solve_tol(self._h, self._rtol, self._maxiter), "set_solve_tol")
+
+    @property
+    def maxiter(self):
+        return self._maxiter
+
+    @maxiter.setter
+    def maxiter(self, v):
+        self._maxiter = int(v)
+        if getattr(self, "_h", None) is not None:
+            _lib.check(_lib.lib().lp_set_solve_tol(self._h, self._rtol, self._maxiter), "set_solve_tol")
 
     def solve_many(self, fs, x0s=None):
         fs = list(fs)

@@ -234,15 +234,18 @@ class SimpOptions:
     eta: float = 0.5
     erode: float = 0.15
     beta_start: float = 1.0
+    prune: bool = True
 
 
 @dataclass
 class SimpResult:
     """simp.SimpResult's fields that the loop fills (simp.py:1093-1121):
-    x the design variables, xphys the blueprint P(H x / hs) at the final
-    sharpness, history one record per iteration (the reference's keys:
-    iteration, compliance, relative, volume, beta, change, seconds,
-    cgIterations, volumeSolves, move, stepFit)."""
+    x the design variables, xphys the delivered design (deliver(); the
+    blueprint P(H x / hs) at the final sharpness when optimize(...,
+    delivered=False)), compliance / case_compliance / volume_fraction of it,
+    history one record per iteration (the reference's keys: iteration,
+    compliance, relative, volume, beta, change, seconds, cgIterations,
+    volumeSolves, move, stepFit)."""
     x: np.ndarray
     xphys: np.ndarray
     compliance: float
@@ -251,18 +254,19 @@ class SimpResult:
     converged: bool
     seconds: float
     history: list = field(default_factory=list)
+    case_compliance: list = field(default_factory=list)
 
 
 def optimize(problem, opts: SimpOptions, *, preserve=None, void=None, case_weights=None, precision="mixed", pad=True,
-             on_iter=None, **kw) -> SimpResult:
+             on_iter=None, delivered=True, **kw) -> SimpResult:
     """simp.optimize (simp.py:1570-1640) for a box problem with the plain
     compliance objective, every iteration on the device (loop
     simp.py:1840-2449). preserve: cells held solid (domain.preserve);
     void: cells held empty (opts.void), still counted in the volume
-    target; case_weights: the load cases' weights (domain.case_weights). The
-    delivered design is the blueprint at the loop's end: the reference's
-    post-processing of it (opening the eroded design, pruning dead
-    members) is not part of this loop. kw go to GridSimp (precision,
+    target; case_weights: the load cases' weights (domain.case_weights).
+    delivered: hand over the reference's delivered design (the opening of
+    the eroded design with its cores kept, pruned when opts.prune:
+    deliver()); False: the blueprint at the loop's end. kw go to GridSimp (precision,
     coarse_cells, element, ...)."""
     kw.setdefault("rtol", opts.solve_rtol)
     s = GridSimp(problem, precision=precision, filter="padded" if pad else "top3d", volfrac=opts.volume_fraction,
@@ -290,7 +294,13 @@ def optimize(problem, opts: SimpOptions, *, preserve=None, void=None, case_weigh
             converged = True
             break
     x = s.x
-    xphys = blueprint(x, opts, pad=pad, beta=_final_beta(history, opts), preserve=preserve, void=void)
+    beta = _final_beta(history, opts)
+    if delivered:
+        d = deliver(s, x, beta, opts, preserve=preserve, void=void, pad=pad, prune=opts.prune)
+        return SimpResult(x=x, xphys=d["xphys"], compliance=d["compliance"], volume_fraction=d["volume_fraction"],
+                          iterations=len(history), converged=converged, seconds=time.time() - t0, history=history,
+                          case_compliance=d["case_compliance"])
+    xphys = blueprint(x, opts, pad=pad, beta=beta, preserve=preserve, void=void)
     return SimpResult(x=x, xphys=xphys, compliance=history[-1]["compliance"] if history else 0.0,
                       volume_fraction=float(xphys[~_pres_mask(preserve, x.shape)].mean()),
                       iterations=len(history), converged=converged, seconds=time.time() - t0, history=history)
@@ -359,6 +369,128 @@ def connected(alive, anchor):
     lbl, _ = label26(alive)
     keep = np.unique(lbl[alive & np.asarray(anchor, bool)])
     return alive & np.isin(lbl, keep[keep > 0])
+
+
+PRUNE_BELOW = 1e-3   # simp.py:1230
+PRUNE_PASSES = 3     # simp.py:1240
+
+
+def _corner_touch(node_mask, dims):
+    """Cells with any of their 8 corners on a marked node."""
+    nx, ny, nz = dims
+    out = np.zeros(dims, bool)
+    for b in range(8):
+        bx, by, bz = b & 1, (b >> 1) & 1, (b >> 2) & 1
+        out |= node_mask[bx:bx + nx, by:by + ny, bz:bz + nz]
+    return out
+
+
+def anchor_cells(problem, preserve=None):
+    """_Analysis.anchor (simp.py:155) on a box: cells touching a support or a
+    loaded node, and the preserved cells."""
+    dims = problem.dims
+    nodes = problem.fixed_grid().any(axis=0) | (np.abs(problem.load_grids()).sum(axis=(0, 1)) > 0)
+    return (_corner_touch(nodes, dims) | _pres_mask(preserve, dims)).ravel()
+
+
+def keep_cores(dims, opened, analysed, blueprint, radius, pitch=1.0):
+    """simp._keep_cores (simp.py:1317-1358) on the device: where any cell
+    was analysed as solid, the blueprint within radius / 2.6 + half a cell
+    of the analysed design comes back (max with the opening). The ball's
+    dilation is the cone filter with every weight positive."""
+    if not ((analysed > 0.5) & (opened < 0.5)).any():
+        return opened
+    reach = radius / 2.6 + 0.5 * pitch
+    n = int(np.ceil(reach / pitch))
+    # centre distance <= reach (the reference's ball, 1e-9 relative slack)
+    ball = GridFilter(dims, reach / pitch * np.sqrt(1.0 + 1e-9), reach=n)
+    near = (ball @ (analysed > 0.5).astype(float)) > 0
+    return np.where(near, np.maximum(opened, blueprint), opened)
+
+
+def _held_pieces(xp, anchor, dims):
+    lbl, _ = label26(((xp > 0.5) | anchor).reshape(dims))
+    return len(np.unique(lbl.ravel()[anchor]))
+
+
+def deliver(s: "GridSimp", x, beta, opts: SimpOptions, *, preserve=None, void=None, pad=True, prune=True):
+    """The design handed over after the loop (simp.py:2452-2548, plain
+    problem), on the device: the opening of the eroded design with the
+    analysed cores kept (simp._keep_cores), preserved cells solid, solved at
+    rtol 1e-6, dead members pruned (simp._prune_dead). s: the study's
+    GridSimp (its grid solves the delivery). Returns dict(xphys,
+    compliance, case_compliance, volume_fraction)."""
+    dims = s.problem.dims
+    x = np.asarray(x, float).ravel()
+    pres = _pres_mask(preserve, dims).ravel()
+    vmask = np.zeros(x.size, bool) if void is None else np.asarray(void, bool).ravel()
+    design = ~pres
+    f = GridFilter(dims, opts.filter_radius)
+    hs = np.full(x.size, sum(w for _, w in f.taps)) if pad else f.sum()
+    xt = (f @ x) / hs
+    bp = project(xt, beta, opts.eta)
+    xp = bp.copy()
+    xp[pres] = 1.0
+    if opts.erode > 0:
+        xe = project(xt, beta, opts.eta + opts.erode)
+        xe[pres] = 1.0
+        xe[vmask] = 0.0
+        opened = project((f @ xe) / hs, beta, opts.eta - opts.erode)
+        xp = keep_cores(dims, opened, xe, bp, opts.filter_radius)
+        xp[pres] = 1.0
+    xp[vmask] = 0.0
+    g = s.grid
+    g.rtol = 1e-6
+    _, loads = s.problem.dof_vectors(g.mesh)
+    weights = np.ones(len(loads)) if getattr(s, "_cw", None) is None else s._cw
+
+    def solve(xv):
+        g.set_moduli(E_MIN + xv ** PENALTY * (1 - E_MIN))
+        return g.solve_many(list(loads))
+
+    us = solve(xp)
+    if prune:
+        anchor = anchor_cells(s.problem, preserve)
+        for _ in range(PRUNE_PASSES):
+            solid = design & (xp > 0.5) & ~pres
+            if not solid.any():
+                break
+            live = np.zeros(x.size, bool)
+            for u in us:
+                w = g.strain_energy(u) * (xp ** PENALTY)
+                ref = np.median(w[xp > 0.5])
+                if ref <= 0:
+                    live[:] = True
+                    break
+                live |= w >= PRUNE_BELOW * ref
+            dead = solid & ~live
+            if dead.sum() < 0.002 * solid.sum():
+                break
+            lbl, nlab = label26(dead.reshape(dims))
+            lbl = lbl.ravel()
+            if nlab == 0:
+                break
+            sizes = np.bincount(lbl[dead], minlength=nlab + 1)
+            dead &= sizes[lbl] >= 4
+            if not dead.any():
+                break
+            before = (xp, us)
+            held = _held_pieces(xp, anchor, dims)
+            xp = xp.copy()
+            xp[dead] = 0.0
+            loose = (xp > 0.5) & ~connected(((xp > 0.5) | anchor).reshape(dims), anchor.reshape(dims)).ravel() & ~pres
+            xp[loose] = 0.0
+            if _held_pieces(xp, anchor, dims) > held:
+                xp, us = before
+                break
+            try:
+                us = solve(xp)
+            except RuntimeError:
+                xp, us = before
+                break
+    cases = [float(fv @ u) for fv, u in zip(loads, us)]
+    return dict(xphys=xp.reshape(dims), compliance=float(np.dot(weights, cases)), case_compliance=cases,
+                volume_fraction=float(xp[design].mean()))
 
 
 def optimize_top3d(problem, iterations=None, maxit=200, on_iter=None, **kw):

@@ -303,3 +303,124 @@ def optimize_loop(p, opts: LoopOptions, preserve=None, weights=None, solve=None,
     xt, xp = L.physical(x, beta)
     return dict(x=x.reshape(p.dims), xphys=xp.reshape(p.dims), history=history, converged=converged,
                 iterations=it, beta=beta)
+
+
+# ---- delivery: the design handed over after the loop (simp.py:2452-2548) ----
+
+PRUNE_BELOW = 1e-3    # simp.py:1230
+PRUNE_PASSES = 3      # simp.py:1240
+_CONN26 = np.ones((3, 3, 3), bool)
+
+
+def keep_cores(dims, opened, analysed, blueprint, radius, pitch=1.0):
+    """simp._keep_cores (simp.py:1317-1358): where any cell was analysed as
+    solid, the blueprint within radius / 2.6 + half a cell of the analysed
+    design comes back (max with the opening)."""
+    from scipy import ndimage
+
+    lost = analysed > 0.5
+    if not ((analysed > 0.5) & (opened < 0.5)).any():
+        return opened
+    reach = radius / 2.6 + 0.5 * pitch
+    n = int(np.ceil(reach / pitch))
+    g = np.mgrid[-n:n + 1, -n:n + 1, -n:n + 1]
+    ball = (g ** 2).sum(axis=0) * pitch ** 2 <= reach ** 2 * (1.0 + 1e-9)
+    near = ndimage.binary_dilation(lost.reshape(dims), structure=ball).ravel()
+    return np.where(near, np.maximum(opened, blueprint), opened)
+
+
+def _label26(mask, dims):
+    from scipy import ndimage
+
+    return ndimage.label(mask.reshape(dims), structure=_CONN26)[0].ravel()
+
+
+def anchor_cells(L: Loop):
+    """_Analysis.anchor (simp.py:155): cells touching a support or a loaded
+    node, and the preserved cells."""
+    mesh = L.ev.mesh
+    held = np.zeros(mesh.n_nodes, bool)
+    held[np.unique(np.flatnonzero(L.ev.fixed) // 3)] = True
+    loaded = np.zeros(mesh.n_nodes, bool)
+    for f in L.ev.fs:
+        loaded |= np.abs(f.reshape(-1, 3)).sum(axis=1) > 0
+    return held[mesh.enode].any(axis=1) | loaded[mesh.enode].any(axis=1) | L.pres
+
+
+def connected(alive, anchor, dims):
+    """_Analysis._connected (simp.py:264-282)."""
+    lbl = _label26(alive, dims)
+    keep_l = np.unique(lbl[alive & anchor])
+    return alive & np.isin(lbl, keep_l[keep_l > 0])
+
+
+def held_pieces(xp, anchor, dims) -> int:
+    """simp._held_pieces (simp.py:1361-1370)."""
+    lbl = _label26((xp > 0.5) | anchor, dims)
+    return len(np.unique(lbl[anchor]))
+
+
+def prune_dead(L: Loop, xp, us):
+    """simp._prune_dead (simp.py:1244-1314) for the plain problem: cells in
+    clusters of >= 4 that carry under 1e-3 of the median energy density in
+    every case are removed, then what that cut loose from the anchors; a
+    pass that parts the design or leaves a mechanism is undone."""
+    dims = L.p.dims
+    anchor = anchor_cells(L)
+    for _ in range(PRUNE_PASSES):
+        solid = L.design & (xp > 0.5) & ~L.pres
+        if not solid.any():
+            break
+        live = np.zeros(len(xp), bool)
+        for u in us:
+            w = element_energy(u, L.ev.edof, L.ev.ke) * (xp ** PENALTY)
+            ref = np.median(w[xp > 0.5])
+            if ref <= 0:
+                live[:] = True
+                break
+            live |= w >= PRUNE_BELOW * ref
+        dead = solid & ~live
+        if dead.sum() < 0.002 * solid.sum():
+            break
+        lbl = _label26(dead, dims)
+        n = int(lbl.max())
+        if n == 0:
+            break
+        sizes = np.bincount(lbl[dead], minlength=n + 1)
+        dead &= sizes[lbl] >= 4
+        if not dead.any():
+            break
+        before = (xp, us)
+        held = held_pieces(xp, anchor, dims)
+        xp = xp.copy()
+        xp[dead] = 0.0
+        loose = (xp > 0.5) & ~connected((xp > 0.5) | anchor, anchor, dims) & ~L.pres
+        xp[loose] = 0.0
+        if held_pieces(xp, anchor, dims) > held:
+            xp, us = before
+            break
+        us = L.solve(E_MIN + xp ** PENALTY * (1 - E_MIN))
+    return xp, us
+
+
+def deliver(L: Loop, x, beta, prune=True):
+    """The delivered design after the loop (simp.py:2452-2548, plain
+    problem): the opening of the eroded design with the analysed cores kept,
+    preserved cells solid, solved, dead members pruned. Returns dict(xphys,
+    compliance, case_compliance, volume_fraction)."""
+    o = L.opts
+    xt = (L.filt @ x) / L.hs
+    xp = project(xt, beta, o.eta)
+    xp[L.pres] = 1.0
+    if o.erode > 0:
+        xe = L.eroded(xt, beta)
+        opened = project((L.filt @ xe) / L.hs, beta, o.eta - o.erode)
+        xp = keep_cores(L.p.dims, opened, xe, project(xt, beta, o.eta), o.filter_radius)
+        xp[L.pres] = 1.0
+    xp[L.void] = 0.0
+    us = L.solve(E_MIN + xp ** PENALTY * (1 - E_MIN))
+    if prune:
+        xp, us = prune_dead(L, xp, us)
+    cases = [float(f @ u) for f, u in zip(L.ev.fs, us)]
+    return dict(xphys=xp, compliance=float(np.dot(L.weights, cases)), case_compliance=cases,
+                volume_fraction=float(xp[L.design].mean()))

@@ -53,7 +53,7 @@ def test_loop_matches_oracle(gpu, variant):
     n_it = 25
     opts = SimpOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=n_it, solve_rtol=1e-11, **kw)
     r = optimize(p, opts, preserve=pres, void=void, case_weights=w, precision="fp64", element="im", maxiter=3000,
-                 coarse_cells=50)
+                 coarse_cells=50, delivered=False)
     ro = optimize_loop(_oracle_problem(p), LoopOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=n_it,
                                                        **kw),
                        preserve=pres.ravel(), weights=w, element="im", void=None if void is None else void.ravel())
@@ -84,7 +84,7 @@ def test_loop_properties_mixed_precision(gpu):
 
     p = cantilever_3d(48, 12, 16, volfrac=0.2, rmin=2.0)
     opts = SimpOptions(volume_fraction=0.2, filter_radius=2.0, max_iterations=90)
-    runs = [optimize(p, opts, precision="mixed", coarse_cells=200) for _ in range(2)]
+    runs = [optimize(p, opts, precision="mixed", coarse_cells=200, delivered=False) for _ in range(2)]
     r = runs[0]
     h = r.history
     print(len(h), r.converged, [round(x["compliance"], 3) for x in h[-6:]])
@@ -105,6 +105,20 @@ def test_loop_properties_mixed_precision(gpu):
     assert [x["compliance"] for x in runs[0].history] == [x["compliance"] for x in runs[1].history]
 
 
+def test_optimize_delivers(gpu):
+    """optimize() hands over the delivered design by default: the opening
+    of the eroded design, solid where preserved, its compliance solved."""
+    from loadpath_amd.simp import SimpOptions, optimize
+
+    p = _problem()
+    pres = _preserve(p.dims)
+    opts = SimpOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=40)
+    r = optimize(p, opts, preserve=pres, precision="mixed", coarse_cells=50)
+    assert r.xphys.shape == p.dims and (r.xphys[pres] == 1.0).all()
+    assert len(r.case_compliance) == 2 and r.compliance == pytest.approx(sum(r.case_compliance), rel=1e-12)
+    assert 0.0 < r.volume_fraction < 1.0
+
+
 def test_loop_rejects_bad_options(gpu):
     from loadpath_amd.problems import cantilever_3d
     from loadpath_amd.simp import GridSimp, SimpOptions
@@ -116,3 +130,29 @@ def test_loop_rejects_bad_options(gpu):
         s.set_loop(SimpOptions(volume_fraction=0.3, filter_radius=1.5), preserve=np.ones((8, 2, 4), bool))
     with pytest.raises(ValueError):
         s.set_loop(SimpOptions(volume_fraction=0.3, filter_radius=1.5), case_weights=[1.0, 2.0])
+
+
+def test_delivery_matches_oracle(gpu):
+    """The delivered design (opening with cores kept, solve at 1e-6, dead
+    members pruned: simp.py:2452-2548) from the same final design on the
+    device and in the oracle."""
+    from loadpath_amd.simp import GridSimp, SimpOptions, deliver
+    from oracle.loop import Loop, LoopOptions, deliver as o_deliver, optimize_loop
+
+    p = _problem()
+    pres = _preserve(p.dims)
+    w = [1.0, 0.5]
+    lo = LoopOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=70)
+    ro = optimize_loop(_oracle_problem(p), lo, preserve=pres.ravel(), weights=w, element="im")
+    want = o_deliver(Loop(_oracle_problem(p), lo, preserve=pres.ravel(), weights=w, element="im"),
+                     ro["x"].ravel(), ro["beta"])
+    s = GridSimp(p, precision="fp64", element="im", rtol=1e-10, maxiter=3000, coarse_cells=50, filter="padded",
+                 volfrac=0.3, rmin=1.5)
+    opts = SimpOptions(volume_fraction=0.3, filter_radius=1.5)
+    s.set_loop(opts, preserve=pres, case_weights=w)
+    got = deliver(s, ro["x"], ro["beta"], opts, preserve=pres)
+    print(ro["iterations"], ro["beta"], want["compliance"], got["compliance"],
+          int((want["xphys"] > 0.5).sum()), int((got["xphys"] > 0.5).sum()))
+    assert np.abs(got["xphys"] - want["xphys"].reshape(p.dims)).max() < 1e-6
+    assert got["compliance"] == pytest.approx(want["compliance"], rel=1e-6)
+    assert got["volume_fraction"] == pytest.approx(want["volume_fraction"], abs=1e-9)

@@ -131,3 +131,50 @@ def test_loop_invariants_on_a_small_cantilever():
     rise = max(b["compliance"] / a["compliance"] - 1 for a, b in zip(last, last[1:]))
     assert rise < 1e-3, rise
     assert last[-1]["compliance"] < last[0]["compliance"]
+
+
+def test_keep_cores_against_a_brute_force_ball():
+    """simp._keep_cores: max(opened, blueprint) on the cells whose centre is
+    within radius / 2.6 + 0.5 of an analysed-solid cell, else the opening."""
+    from oracle.loop import keep_cores
+
+    rng = np.random.default_rng(4)
+    dims = (9, 7, 8)
+    n = int(np.prod(dims))
+    opened = rng.random(n)
+    analysed = np.where(rng.random(n) < 0.1, 0.9, 0.1)
+    analysed[0] = 0.9
+    opened[0] = 0.2                      # a lost cell, so the rule applies
+    blueprint = rng.random(n)
+    radius = 2.5
+    got = keep_cores(dims, opened, analysed, blueprint, radius)
+    reach = radius / 2.6 + 0.5
+    ijk = np.argwhere(np.ones(dims, bool))
+    solid = ijk[analysed > 0.5]
+    want = opened.copy()
+    for i, c in enumerate(ijk):
+        if (((solid - c) ** 2).sum(axis=1) <= reach ** 2 * (1 + 1e-9)).any():
+            want[i] = max(opened[i], blueprint[i])
+    assert np.array_equal(got, want)
+    # nothing lost: the opening itself
+    assert keep_cores(dims, np.ones(n), analysed, blueprint, radius) is not None
+    assert np.array_equal(keep_cores(dims, np.ones(n), analysed, blueprint, radius), np.ones(n))
+
+
+def test_prune_removes_a_dead_tendril():
+    """A cantilever bar carrying the load plus a free side branch of 5 cells:
+    the branch carries no energy and is pruned, the bar stays
+    (simp._prune_dead, simp.py:1244-1314)."""
+    from oracle.loop import Loop, LoopOptions, prune_dead, E_MIN, PENALTY
+
+    p = cantilever_3d(nx=12, ny=2, nz=8, volfrac=0.3, rmin=1.5)
+    L = Loop(p, LoopOptions(volume_fraction=0.3, filter_radius=1.5))
+    xp = np.zeros(p.dims)
+    xp[:, :, 0:2] = 1.0                  # the bar along the bottom, under the tip load
+    xp[5, :, 2:7] = 1.0                  # a branch standing up from it, free at the top
+    xp = xp.ravel()
+    us = L.solve(E_MIN + xp ** PENALTY * (1 - E_MIN))
+    out, _ = prune_dead(L, xp, us)
+    out = out.reshape(p.dims)
+    assert (out[:, :, 0:2] == 1.0).all()
+    assert out[5, :, 3:7].sum() < xp.reshape(p.dims)[5, :, 3:7].sum()
