@@ -275,6 +275,73 @@ __global__ void __launch_bounds__(256) k_nodal_stress(NsArgs A) {
     }
 }
 
+// verify._MaterialAverage (verify.py:214-305), radii up to 3.3 cells: the
+// samples of each live cell (8 quarter points, or the centre past 1.5
+// cells), their tensors trilinear in the cell's nodal tensors and their
+// weights fill * volume / samples; then each node's weighted mean over the
+// samples within the radius, one thread per node over the cells in reach
+// in a fixed order (its own tensor when there are none).
+__global__ void __launch_bounds__(256) k_avg_samples(int nx, int ny, int nz, int ns, const double* __restrict__ sig,
+                                                     const double* __restrict__ fill, double vol, double* ssig,
+                                                     double* sw) {
+    const size_t nc = (size_t)nx * ny * nz;
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int X = (int)(c / ((size_t)ny * nz)), Y = (int)((c / nz) % ny), Z = (int)(c % nz);
+    const double f = fill ? fill[c] : 1.0;
+    sw[c] = f > 0.0 ? f * vol / ns : 0.0;
+    if (!(f > 0.0)) return;
+    const size_t Ny = (size_t)ny + 1, Nz = (size_t)nz + 1;
+    for (int smp = 0; smp < ns; ++smp) {
+        double q[3];
+        for (int d = 0; d < 3; ++d) q[d] = ns == 8 ? 0.25 + 0.5 * ((smp >> d) & 1) : 0.5;
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < 8; ++k) {
+            const int kx = k & 1, ky = (k >> 1) & 1, kz = (k >> 2) & 1;
+            const double w = (kx ? q[0] : 1.0 - q[0]) * (ky ? q[1] : 1.0 - q[1]) * (kz ? q[2] : 1.0 - q[2]);
+            const size_t node = ((size_t)(X + kx) * Ny + (Y + ky)) * Nz + (Z + kz);
+#pragma unroll
+            for (int r = 0; r < 6; ++r) acc[r] += w * sig[6 * node + r];
+        }
+#pragma unroll
+        for (int r = 0; r < 6; ++r) ssig[(c * ns + smp) * 6 + r] = acc[r];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_avg_ball(int nx, int ny, int nz, int ns, double pitch, double radius,
+                                                  int reach, const double* __restrict__ sig,
+                                                  const double* __restrict__ ssig, const double* __restrict__ sw,
+                                                  double* out) {
+    const size_t Ny = (size_t)ny + 1, Nz = (size_t)nz + 1;
+    const size_t nn = ((size_t)nx + 1) * Ny * Nz;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    const int X = (int)(n / (Ny * Nz)), Y = (int)((n / Nz) % Ny), Z = (int)(n % Nz);
+    const double r2 = radius * radius;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    double tot = 0.0;
+    for (int cx = max(X - reach - 1, 0); cx <= min(X + reach, nx - 1); ++cx)
+        for (int cy = max(Y - reach - 1, 0); cy <= min(Y + reach, ny - 1); ++cy)
+            for (int cz = max(Z - reach - 1, 0); cz <= min(Z + reach, nz - 1); ++cz) {
+                const size_t c = ((size_t)cx * ny + cy) * nz + cz;
+                const double w = sw[c];
+                if (!(w > 0.0)) continue;
+                for (int smp = 0; smp < ns; ++smp) {
+                    const double qx = ns == 8 ? 0.25 + 0.5 * (smp & 1) : 0.5;
+                    const double qy = ns == 8 ? 0.25 + 0.5 * ((smp >> 1) & 1) : 0.5;
+                    const double qz = ns == 8 ? 0.25 + 0.5 * ((smp >> 2) & 1) : 0.5;
+                    const double dx = (cx + qx - X) * pitch, dy = (cy + qy - Y) * pitch, dz = (cz + qz - Z) * pitch;
+                    if (dx * dx + dy * dy + dz * dz > r2) continue;
+                    const double* sp = ssig + (c * ns + smp) * 6;
+#pragma unroll
+                    for (int r = 0; r < 6; ++r) acc[r] += w * sp[r];
+                    tot += w;
+                }
+            }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) out[6 * n + r] = tot > 0.0 ? acc[r] / tot : sig[6 * n + r];
+}
+
 }  // namespace
 }  // namespace lp
 
@@ -451,6 +518,45 @@ int lp_nodal_stress(int nx, int ny, int nz, double pitch, double nu, int element
         } else {
             LP_HIP(hipDeviceSynchronize());
         }
+        return LP_OK;
+    });
+}
+
+int lp_material_average(int nx, int ny, int nz, double pitch, double radius, const double* fill, const double* sig,
+                        double* out) {
+    return api_call([&]() -> int {
+        if (nx < 1 || ny < 1 || nz < 1 || !(pitch > 0.0) || !(radius > 0.0) || !sig || !out) {
+            set_error("lp_material_average: bad argument");
+            return LP_ERR_ARG;
+        }
+        const double rho = radius / pitch;
+        if (rho > 3.3) {
+            set_error("lp_material_average: radius over 3.3 cells (block aggregation) is not supported");
+            return LP_ERR_ARG;
+        }
+        const int ns = rho <= 1.5 ? 8 : 1;
+        const int reach = (int)std::ceil(rho);
+        const size_t nc = (size_t)nx * ny * nz;
+        const size_t nn = (size_t)(nx + 1) * (ny + 1) * (nz + 1);
+        DBuf dsig, dfill, dss, dsw, dout;
+        dsig.alloc(nn * 6 * 8);
+        LP_HIP(hipMemcpy(dsig.p, sig, nn * 6 * 8, hipMemcpyHostToDevice));
+        if (fill) {
+            dfill.alloc(nc * 8);
+            LP_HIP(hipMemcpy(dfill.p, fill, nc * 8, hipMemcpyHostToDevice));
+        }
+        dss.alloc(nc * ns * 6 * 8);
+        dsw.alloc(nc * 8);
+        dout.alloc(nn * 6 * 8);
+        hipLaunchKernelGGL(k_avg_samples, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, nullptr, nx, ny, nz, ns,
+                           dsig.as<double>(), fill ? dfill.as<double>() : (const double*)nullptr,
+                           pitch * pitch * pitch, dss.as<double>(), dsw.as<double>());
+        LP_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_avg_ball, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, nullptr, nx, ny, nz, ns,
+                           pitch, radius, reach, dsig.as<double>(), dss.as<double>(), dsw.as<double>(),
+                           dout.as<double>());
+        LP_HIP(hipGetLastError());
+        LP_HIP(hipMemcpy(out, dout.p, nn * 6 * 8, hipMemcpyDeviceToHost));
         return LP_OK;
     });
 }

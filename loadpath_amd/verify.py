@@ -77,3 +77,38 @@ def nodal_von_mises(stiff, u, modulus, thermal=None):
     """verify.nodal_von_mises (verify.py:197-206): fem.von_mises of the
     nodal-averaged stress."""
     return nodal_stress(stiff, u, modulus, thermal, vm_out=True)[1]
+
+
+
+class MaterialAverage:
+    """verify._MaterialAverage(node_xyz, enode, fill, pitch, radius)
+    (verify.py:214-305) on the device, for a HexMesh and radii up to 3.3
+    cells (the reference's block aggregation past that is not built): ()
+    applied to nodal tensors sig (n_nodes, 6) gives each node's stress
+    averaged over the material within radius (lp_material_average). fill:
+    the filled share of each element (None: full)."""
+
+    def __init__(self, mesh, fill=None, radius=1.0):
+        self.mesh = mesh
+        self.radius = float(radius)
+        if self.radius / float(mesh.pitch) > 3.3:
+            raise NotImplementedError("material average over 3.3 cells (block aggregation) is not supported")
+        self._dims, _, self._nodes, cells = _grid_of(mesh)
+        self._fill = np.zeros(int(np.prod(self._dims)))
+        self._fill[cells] = 1.0 if fill is None else np.broadcast_to(np.asarray(fill, float), (mesh.n_elem,))
+
+    def __call__(self, sig):
+        _lib.require_gpu()
+        nn = int(np.prod([d + 1 for d in self._dims]))
+        sg = np.zeros((nn, 6))
+        sg[self._nodes] = np.asarray(sig, float).reshape(-1, 6)
+        out = np.empty((nn, 6))
+        _lib.check(_lib.lib().lp_material_average(*self._dims, float(self.mesh.pitch), self.radius,
+                                                  _lib.ptr(self._fill), _lib.ptr(sg), _lib.ptr(out)),
+                   "material_average")
+        return out[self._nodes]
+
+
+def material_average(mesh, sig, fill=None, radius=1.0):
+    """MaterialAverage(mesh, fill, radius)(sig)."""
+    return MaterialAverage(mesh, fill, radius)(sig)

@@ -177,3 +177,40 @@ def von_mises(stress):
     """fem.von_mises (fem.py:311-314)."""
     sx, sy, sz, txy, tyz, tzx = np.asarray(stress, float).T
     return np.sqrt(0.5 * ((sx - sy) ** 2 + (sy - sz) ** 2 + (sz - sx) ** 2) + 3.0 * (txy ** 2 + tyz ** 2 + tzx ** 2))
+
+
+def material_average(node_xyz, enode, fill, pitch: float, radius: float, sig):
+    """verify._MaterialAverage (verify.py:214-305) applied to nodal tensors
+    sig (n_nodes, 6), for radii under 3.3 cells (no block aggregation): the
+    nodal tensors interpolated trilinearly to each live cell's samples (its
+    8 quarter points while radius <= 1.5 cells, else its centre), each
+    sample weighted by the cell's fill * volume / samples, and every node's
+    weighted mean over the samples within radius (its own value when there
+    are none)."""
+    from scipy.spatial import cKDTree
+
+    node_xyz = np.asarray(node_xyz, float)
+    enode = np.asarray(enode)
+    corners = np.array([[k & 1, (k >> 1) & 1, (k >> 2) & 1] for k in range(8)], float)
+    n, ne = len(node_xyz), len(enode)
+    f = np.ones(ne) if fill is None else np.asarray(fill, float)
+    live = np.nonzero(f > 0)[0]
+    rho = radius / pitch
+    if rho > 3.3:
+        raise NotImplementedError("block aggregation (radius over 3.3 cells) is not restated")
+    q = 0.25 + 0.5 * corners if rho <= 1.5 else np.full((1, 3), 0.5)
+    ns = len(q)
+    wts = np.prod(np.where(corners[None, :, :] > 0.5, q[:, None, :], 1.0 - q[:, None, :]), axis=2)
+    xs = np.einsum("sk,ekd->esd", wts, node_xyz[enode[live]]).reshape(-1, 3)
+    w = np.repeat(f[live] * pitch ** 3 / ns, ns)
+    sig = np.asarray(sig, float)
+    ssig = np.einsum("sk,ekc->esc", wts, sig[enode[live]]).reshape(-1, 6)
+    pairs = cKDTree(node_xyz).sparse_distance_matrix(cKDTree(xs), radius, output_type="coo_matrix")
+    out = np.zeros((n, 6))
+    tot = np.zeros(n)
+    np.add.at(out, pairs.row, w[pairs.col, None] * ssig[pairs.col])
+    np.add.at(tot, pairs.row, w[pairs.col])
+    lone = tot <= 0
+    out = out / np.where(lone, 1.0, tot)[:, None]
+    out[lone] = sig[lone]
+    return out

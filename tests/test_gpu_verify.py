@@ -59,3 +59,36 @@ def test_nodal_stress_closed_forms(gpu):
         x = m.node_ijk.astype(float)
         top = np.isclose(x[:, 1], t) & (x[:, 0] > 2) & (x[:, 0] < 10)
         assert s[top, 0].mean() / (t / 2.0) == pytest.approx(1.0, abs=1e-6)
+
+
+@pytest.mark.parametrize("radius", [0.9, 1.4, 2.6, 3.2])
+def test_material_average_matches_oracle(gpu, radius):
+    """verify._MaterialAverage on the device against the oracle's
+    restatement: a holed voxel set, partial and empty fills, quarter-point
+    samples (radius <= 1.5 cells) and centre samples past that."""
+    from loadpath_amd.verify import MaterialAverage
+    from oracle import stress as S
+    from oracle.fem import Mesh
+
+    rng = np.random.default_rng(11)
+    mesh = Mesh(np.argwhere(rng.random((11, 8, 9)) < 0.8) + np.array([2, 0, 5]), 0.6, (1.0, -2.0, 0.5))
+    xyz = mesh.origin + mesh.node_ijk * mesh.pitch
+    fill = rng.uniform(0.0, 1.0, mesh.n_elem)
+    fill[rng.random(mesh.n_elem) < 0.2] = 0.0
+    sig = rng.standard_normal((mesh.n_nodes, 6))
+    r = radius * mesh.pitch
+    want = S.material_average(xyz, mesh.enode, fill, mesh.pitch, r, sig)
+    avg = MaterialAverage(mesh, fill, r)
+    for s in (sig, 3.0 * sig[::-1].copy()):
+        got = avg(s)
+        ref = want if s is sig else S.material_average(xyz, mesh.enode, fill, mesh.pitch, r, s)
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_material_average_rejects_block_radii(gpu):
+    from loadpath_amd.verify import MaterialAverage
+    from oracle.fem import Mesh
+
+    mesh = Mesh(np.argwhere(np.ones((3, 3, 3), bool)), 1.0)
+    with pytest.raises(NotImplementedError):
+        MaterialAverage(mesh, None, 3.5)

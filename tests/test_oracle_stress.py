@@ -216,3 +216,48 @@ def test_nodal_stress_matches_a_per_cell_loop():
     assert np.allclose(s, want, rtol=1e-12, atol=1e-12)
     vm = S.von_mises(s)
     assert np.allclose(vm, np.sqrt(np.einsum("ni,ij,nj->n", s, S.VM, s)), rtol=1e-12)
+
+
+# ---- verify._MaterialAverage (oracle/stress.py material_average) -----------
+
+@pytest.mark.parametrize("radius", [0.9, 1.4, 2.6])
+def test_material_average_against_a_brute_force_ball(radius):
+    from oracle.fem import Mesh
+
+    rng = np.random.default_rng(5)
+    mesh = Mesh(np.argwhere(rng.random((6, 5, 4)) < 0.8), 0.7, (0.3, 0.0, -0.4))
+    xyz = mesh.origin + mesh.node_ijk * mesh.pitch
+    fill = rng.uniform(0.0, 1.0, mesh.n_elem)
+    fill[rng.random(mesh.n_elem) < 0.2] = 0.0
+    sig = rng.standard_normal((mesh.n_nodes, 6))
+    r = radius * mesh.pitch
+    got = S.material_average(xyz, mesh.enode, fill, mesh.pitch, r, sig)
+    rho = radius
+    qs = [(0.25 + 0.5 * (k & 1), 0.25 + 0.5 * ((k >> 1) & 1), 0.25 + 0.5 * ((k >> 2) & 1)) for k in range(8)] \
+        if rho <= 1.5 else [(0.5, 0.5, 0.5)]
+    want = np.zeros_like(sig)
+    for i in range(mesh.n_nodes):
+        acc, tot = np.zeros(6), 0.0
+        for c in range(mesh.n_elem):
+            if fill[c] <= 0:
+                continue
+            for q in qs:
+                wk = np.array([(q[0] if k & 1 else 1 - q[0]) * (q[1] if (k >> 1) & 1 else 1 - q[1])
+                               * (q[2] if (k >> 2) & 1 else 1 - q[2]) for k in range(8)])
+                xs = wk @ xyz[mesh.enode[c]]
+                if np.linalg.norm(xs - xyz[i]) <= r:
+                    w = fill[c] * mesh.pitch ** 3 / len(qs)
+                    acc += w * (wk @ sig[mesh.enode[c]])
+                    tot += w
+        want[i] = acc / tot if tot > 0 else sig[i]
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_material_average_keeps_a_uniform_field():
+    from oracle.fem import Mesh
+
+    mesh = Mesh(np.argwhere(np.ones((5, 4, 3), bool)), 1.0)
+    sig = np.tile(np.array([1.0, -2.0, 0.5, 0.3, 0.0, -0.7]), (mesh.n_nodes, 1))
+    fill = np.linspace(0.1, 1.0, mesh.n_elem)
+    got = S.material_average(mesh.node_ijk * 1.0, mesh.enode, fill, 1.0, 1.2, sig)
+    assert np.allclose(got, sig, rtol=0, atol=1e-14)
