@@ -255,14 +255,16 @@ int lp_nodal_stress(int nx, int ny, int nz, double pitch, double nu, int element
                     const double* modulus, double modulus_scalar, const double* thermal, const uint8_t* present,
                     const uint8_t* counted, double* out, double* vm, int device_ptr);
 
-/* verify._MaterialAverage (verify.py:214-305) applied to nodal tensors, radii
- * up to 3.3 cells: sig (nodes, 6) node-major on the (nx+1)(ny+1)(nz+1) grid
- * (lp_nodal_stress's output), fill (cells) the filled share of each cell
- * (0 = no material; NULL: every cell full). Each live cell's samples (its 8
- * quarter points while radius <= 1.5 cells, else its centre) carry the
- * tensor trilinear in its corners and the weight fill * pitch^3 / samples;
- * out (nodes, 6) = each node's weighted mean over the samples within
- * radius, or its own tensor when none is. Host arrays. */
+/* verify._MaterialAverage (verify.py:214-305) applied to nodal tensors: sig
+ * (nodes, 6) node-major on the (nx+1)(ny+1)(nz+1) grid (lp_nodal_stress's
+ * output), fill (cells) the filled share of each cell (0 = no material;
+ * NULL: every cell full). Each live cell's samples (its 8 quarter points
+ * while radius <= 1.5 cells, else its centre) carry the tensor trilinear in
+ * its corners and the weight fill * pitch^3 / samples; past 3.3 cells the
+ * samples of each block of ceil(radius / 3.3 cells)^3 cells (counted from
+ * the smallest live cell along each axis) merge into one at their weighted
+ * centroid. out (nodes, 6) = each node's weighted mean over the samples
+ * within radius, or its own tensor when none is. Host arrays. */
 int lp_material_average(int nx, int ny, int nz, double pitch, double radius, const double* fill, const double* sig,
                         double* out);
 

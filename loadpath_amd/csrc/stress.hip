@@ -16,6 +16,10 @@
 // design iteration; its cost is a few field passes). The adjoint right-hand
 // side is assembled node-centrically from a per-cell buffer: each node sums
 // its <= 8 cells in a fixed order (no atomics, deterministic).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
 #include "lp_internal.h"
 
 namespace lp {
@@ -342,6 +346,87 @@ __global__ void __launch_bounds__(256) k_avg_ball(int nx, int ny, int nz, int ns
     for (int r = 0; r < 6; ++r) out[6 * n + r] = tot > 0.0 ? acc[r] / tot : sig[6 * n + r];
 }
 
+__device__ __forceinline__ int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+// Past 3.3 cells (centre samples): blocks of blk^3 cells counted from the
+// smallest live cell index along each axis (c0), one thread per block
+// merging its live cells into the summed weight, the weighted centroid (in
+// cell units) and the weighted mean of the cells' centre tensors.
+__global__ void __launch_bounds__(256) k_avg_blocks(int nx, int ny, int nz, int blk, int c0x, int c0y, int c0z,
+                                                    int bx, int by, int bz, const double* __restrict__ sig,
+                                                    const double* __restrict__ fill, double vol, double* bw,
+                                                    double* bpos, double* bsig) {
+    const size_t nb = (size_t)bx * by * bz;
+    const size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int KX = (int)(b / ((size_t)by * bz)), KY = (int)((b / bz) % by), KZ = (int)(b % bz);
+    const size_t Ny = (size_t)ny + 1, Nz = (size_t)nz + 1;
+    double tw = 0.0, pos[3] = {0, 0, 0}, acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int X = c0x + KX * blk; X < min(c0x + (KX + 1) * blk, nx); ++X)
+        for (int Y = c0y + KY * blk; Y < min(c0y + (KY + 1) * blk, ny); ++Y)
+            for (int Z = c0z + KZ * blk; Z < min(c0z + (KZ + 1) * blk, nz); ++Z) {
+                const size_t c = ((size_t)X * ny + Y) * nz + Z;
+                const double f = fill ? fill[c] : 1.0;
+                if (!(f > 0.0)) continue;
+                const double w = f * vol;
+                tw += w;
+                pos[0] += w * (X + 0.5);
+                pos[1] += w * (Y + 0.5);
+                pos[2] += w * (Z + 0.5);
+                for (int k = 0; k < 8; ++k) {
+                    const size_t node = ((size_t)(X + (k & 1)) * Ny + (Y + ((k >> 1) & 1))) * Nz + (Z + (k >> 2));
+#pragma unroll
+                    for (int r = 0; r < 6; ++r) acc[r] += (0.125 * w) * sig[6 * node + r];
+                }
+            }
+    bw[b] = tw;
+    const double inv = tw > 0.0 ? 1.0 / tw : 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) bpos[3 * b + d] = pos[d] * inv;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) bsig[6 * b + r] = acc[r] * inv;
+}
+
+// each node's weighted mean over the block samples within the radius, the
+// blocks visited in a fixed order over every block whose cells reach
+// within rho of the node
+__global__ void __launch_bounds__(256) k_avg_block_ball(int nx, int ny, int nz, int blk, int c0x, int c0y, int c0z,
+                                                        int bx, int by, int bz, double pitch, double radius,
+                                                        int reach, const double* __restrict__ sig,
+                                                        const double* __restrict__ bw,
+                                                        const double* __restrict__ bpos,
+                                                        const double* __restrict__ bsig, double* out) {
+    const size_t Ny = (size_t)ny + 1, Nz = (size_t)nz + 1;
+    const size_t nn = ((size_t)nx + 1) * Ny * Nz;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    const int X = (int)(n / (Ny * Nz)), Y = (int)((n / Nz) % Ny), Z = (int)(n % Nz);
+    const double r2 = radius * radius;
+    // block K holds cells c0 + K blk .. c0 + K blk + blk - 1; a cell within
+    // reach of the node lies in [X - reach - 1, X + reach]
+    const int lo[3] = {max(floor_div(X - reach - 1 - c0x, blk), 0), max(floor_div(Y - reach - 1 - c0y, blk), 0),
+                       max(floor_div(Z - reach - 1 - c0z, blk), 0)};
+    const int hi[3] = {min(floor_div(X + reach - c0x, blk), bx - 1), min(floor_div(Y + reach - c0y, blk), by - 1),
+                       min(floor_div(Z + reach - c0z, blk), bz - 1)};
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    double tot = 0.0;
+    for (int KX = lo[0]; KX <= hi[0]; ++KX)
+        for (int KY = lo[1]; KY <= hi[1]; ++KY)
+            for (int KZ = lo[2]; KZ <= hi[2]; ++KZ) {
+                const size_t b = ((size_t)KX * by + KY) * bz + KZ;
+                const double w = bw[b];
+                if (!(w > 0.0)) continue;
+                const double dx = (bpos[3 * b] - X) * pitch, dy = (bpos[3 * b + 1] - Y) * pitch,
+                             dz = (bpos[3 * b + 2] - Z) * pitch;
+                if (dx * dx + dy * dy + dz * dz > r2) continue;
+#pragma unroll
+                for (int r = 0; r < 6; ++r) acc[r] += w * bsig[6 * b + r];
+                tot += w;
+            }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) out[6 * n + r] = tot > 0.0 ? acc[r] / tot : sig[6 * n + r];
+}
+
 }  // namespace
 }  // namespace lp
 
@@ -530,12 +615,14 @@ int lp_material_average(int nx, int ny, int nz, double pitch, double radius, con
             return LP_ERR_ARG;
         }
         const double rho = radius / pitch;
-        if (rho > 3.3) {
-            set_error("lp_material_average: radius over 3.3 cells (block aggregation) is not supported");
+        const double rblk = std::ceil(rho / 3.3);
+        if (!(rho < 1e6) || rblk > 1e6) {
+            set_error("lp_material_average: radius out of range");
             return LP_ERR_ARG;
         }
         const int ns = rho <= 1.5 ? 8 : 1;
         const int reach = (int)std::ceil(rho);
+        const int blk = (int)rblk;
         const size_t nc = (size_t)nx * ny * nz;
         const size_t nn = (size_t)(nx + 1) * (ny + 1) * (nz + 1);
         DBuf dsig, dfill, dss, dsw, dout;
@@ -544,6 +631,39 @@ int lp_material_average(int nx, int ny, int nz, double pitch, double radius, con
         if (fill) {
             dfill.alloc(nc * 8);
             LP_HIP(hipMemcpy(dfill.p, fill, nc * 8, hipMemcpyHostToDevice));
+        }
+        if (blk > 1) {
+            // the smallest live cell index along each axis: blocks are
+            // counted from the smallest sample coordinate (verify.py:266)
+            int c0[3] = {nx, ny, nz};
+            for (size_t c = 0; c < nc; ++c) {
+                if (fill && !(fill[c] > 0.0)) continue;
+                const int idx[3] = {(int)(c / ((size_t)ny * nz)), (int)((c / nz) % ny), (int)(c % nz)};
+                for (int d = 0; d < 3; ++d) c0[d] = std::min(c0[d], idx[d]);
+            }
+            if (c0[0] == nx) {  // no material: every node keeps its own tensor
+                std::memcpy(out, sig, nn * 6 * 8);
+                return LP_OK;
+            }
+            const int bx = (nx - c0[0] + blk - 1) / blk, by = (ny - c0[1] + blk - 1) / blk,
+                      bz = (nz - c0[2] + blk - 1) / blk;
+            const size_t nb = (size_t)bx * by * bz;
+            DBuf dbw, dbp, dbs;
+            dbw.alloc(nb * 8);
+            dbp.alloc(nb * 3 * 8);
+            dbs.alloc(nb * 6 * 8);
+            dout.alloc(nn * 6 * 8);
+            hipLaunchKernelGGL(k_avg_blocks, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, nullptr, nx, ny, nz,
+                               blk, c0[0], c0[1], c0[2], bx, by, bz, dsig.as<double>(),
+                               fill ? dfill.as<double>() : (const double*)nullptr, pitch * pitch * pitch,
+                               dbw.as<double>(), dbp.as<double>(), dbs.as<double>());
+            LP_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_avg_block_ball, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, nullptr, nx, ny,
+                               nz, blk, c0[0], c0[1], c0[2], bx, by, bz, pitch, radius, reach, dsig.as<double>(),
+                               dbw.as<double>(), dbp.as<double>(), dbs.as<double>(), dout.as<double>());
+            LP_HIP(hipGetLastError());
+            LP_HIP(hipMemcpy(out, dout.p, nn * 6 * 8, hipMemcpyDeviceToHost));
+            return LP_OK;
         }
         dss.alloc(nc * ns * 6 * 8);
         dsw.alloc(nc * 8);

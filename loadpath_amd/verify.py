@@ -82,17 +82,15 @@ def nodal_von_mises(stiff, u, modulus, thermal=None):
 
 class MaterialAverage:
     """verify._MaterialAverage(node_xyz, enode, fill, pitch, radius)
-    (verify.py:214-305) on the device, for a HexMesh and radii up to 3.3
-    cells (the reference's block aggregation past that is not built): ()
-    applied to nodal tensors sig (n_nodes, 6) gives each node's stress
-    averaged over the material within radius (lp_material_average). fill:
-    the filled share of each element (None: full)."""
+    (verify.py:214-305) on the device, for a HexMesh: () applied to nodal
+    tensors sig (n_nodes, 6) gives each node's stress averaged over the
+    material within radius (lp_material_average), with the reference's
+    samples (quarter points, centres, blocks of cells past 3.3 cells).
+    fill: the filled share of each element (None: full)."""
 
     def __init__(self, mesh, fill=None, radius=1.0):
         self.mesh = mesh
         self.radius = float(radius)
-        if self.radius / float(mesh.pitch) > 3.3:
-            raise NotImplementedError("material average over 3.3 cells (block aggregation) is not supported")
         self._dims, _, self._nodes, cells = _grid_of(mesh)
         self._fill = np.zeros(int(np.prod(self._dims)))
         self._fill[cells] = 1.0 if fill is None else np.broadcast_to(np.asarray(fill, float), (mesh.n_elem,))

@@ -181,12 +181,15 @@ def von_mises(stress):
 
 def material_average(node_xyz, enode, fill, pitch: float, radius: float, sig):
     """verify._MaterialAverage (verify.py:214-305) applied to nodal tensors
-    sig (n_nodes, 6), for radii under 3.3 cells (no block aggregation): the
-    nodal tensors interpolated trilinearly to each live cell's samples (its
-    8 quarter points while radius <= 1.5 cells, else its centre), each
-    sample weighted by the cell's fill * volume / samples, and every node's
-    weighted mean over the samples within radius (its own value when there
-    are none)."""
+    sig (n_nodes, 6): the nodal tensors interpolated trilinearly to each
+    live cell's samples (its 8 quarter points while radius <= 1.5 cells,
+    else its centre), each sample weighted by the cell's fill * volume /
+    samples; past 3.3 cells the samples are merged by blocks of blk^3 cells
+    (blk = ceil(radius / 3.3 cells), blocks counted from the smallest sample
+    coordinate) into one sample at the block's weighted centroid carrying
+    the weighted mean tensor and the summed weight (verify.py:264-274); then
+    every node's weighted mean over the samples within radius (its own
+    value when there are none)."""
     from scipy.spatial import cKDTree
 
     node_xyz = np.asarray(node_xyz, float)
@@ -196,8 +199,6 @@ def material_average(node_xyz, enode, fill, pitch: float, radius: float, sig):
     f = np.ones(ne) if fill is None else np.asarray(fill, float)
     live = np.nonzero(f > 0)[0]
     rho = radius / pitch
-    if rho > 3.3:
-        raise NotImplementedError("block aggregation (radius over 3.3 cells) is not restated")
     q = 0.25 + 0.5 * corners if rho <= 1.5 else np.full((1, 3), 0.5)
     ns = len(q)
     wts = np.prod(np.where(corners[None, :, :] > 0.5, q[:, None, :], 1.0 - q[:, None, :]), axis=2)
@@ -205,6 +206,16 @@ def material_average(node_xyz, enode, fill, pitch: float, radius: float, sig):
     w = np.repeat(f[live] * pitch ** 3 / ns, ns)
     sig = np.asarray(sig, float)
     ssig = np.einsum("sk,ekc->esc", wts, sig[enode[live]]).reshape(-1, 6)
+    blk = int(np.ceil(rho / 3.3))
+    if blk > 1 and len(xs):
+        key = np.floor((xs - xs.min(axis=0)) / (blk * pitch) + 1e-9).astype(np.int64)
+        _, gid = np.unique(key, axis=0, return_inverse=True)
+        gid = np.asarray(gid).ravel()
+        nb = int(gid.max()) + 1
+        wb = np.bincount(gid, w, nb)
+        xs = np.stack([np.bincount(gid, w * xs[:, d], nb) for d in range(3)], axis=1) / wb[:, None]
+        ssig = np.stack([np.bincount(gid, w * ssig[:, c], nb) for c in range(6)], axis=1) / wb[:, None]
+        w = wb
     pairs = cKDTree(node_xyz).sparse_distance_matrix(cKDTree(xs), radius, output_type="coo_matrix")
     out = np.zeros((n, 6))
     tot = np.zeros(n)

@@ -61,11 +61,12 @@ def test_nodal_stress_closed_forms(gpu):
         assert s[top, 0].mean() / (t / 2.0) == pytest.approx(1.0, abs=1e-6)
 
 
-@pytest.mark.parametrize("radius", [0.9, 1.4, 2.6, 3.2])
+@pytest.mark.parametrize("radius", [0.9, 1.4, 2.6, 3.2, 3.7, 7.1])
 def test_material_average_matches_oracle(gpu, radius):
     """verify._MaterialAverage on the device against the oracle's
     restatement: a holed voxel set, partial and empty fills, quarter-point
-    samples (radius <= 1.5 cells) and centre samples past that."""
+    samples (radius <= 1.5 cells), centre samples past that and blocks of
+    2^3 and 3^3 cells past 3.3 cells."""
     from loadpath_amd.verify import MaterialAverage
     from oracle import stress as S
     from oracle.fem import Mesh
@@ -84,11 +85,3 @@ def test_material_average_matches_oracle(gpu, radius):
         ref = want if s is sig else S.material_average(xyz, mesh.enode, fill, mesh.pitch, r, s)
         assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
 
-
-def test_material_average_rejects_block_radii(gpu):
-    from loadpath_amd.verify import MaterialAverage
-    from oracle.fem import Mesh
-
-    mesh = Mesh(np.argwhere(np.ones((3, 3, 3), bool)), 1.0)
-    with pytest.raises(NotImplementedError):
-        MaterialAverage(mesh, None, 3.5)

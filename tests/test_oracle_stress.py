@@ -261,3 +261,45 @@ def test_material_average_keeps_a_uniform_field():
     fill = np.linspace(0.1, 1.0, mesh.n_elem)
     got = S.material_average(mesh.node_ijk * 1.0, mesh.enode, fill, 1.0, 1.2, sig)
     assert np.allclose(got, sig, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("radius", [3.7, 7.1])
+def test_material_average_blocks_on_a_linear_field(radius):
+    """Past 3.3 cells the samples merge by blocks: for a linear nodal field
+    the trilinear samples are exact, a block's merged tensor is the field at
+    its filled centroid, and a node's average is the field at the weighted
+    centroid of the blocks in its ball. Blocks built here by a dictionary
+    over integer cell keys, balls by brute force."""
+    from oracle.fem import Mesh
+
+    rng = np.random.default_rng(9)
+    mesh = Mesh(np.argwhere(rng.random((13, 11, 9)) < 0.7) + np.array([1, 2, 0]), 0.5, (0.2, 0.1, -0.3))
+    xyz = mesh.origin + mesh.node_ijk * mesh.pitch
+    fill = rng.uniform(0.05, 1.0, mesh.n_elem)
+    fill[rng.random(mesh.n_elem) < 0.15] = 0.0
+    a = rng.standard_normal((6, 3))
+    b = rng.standard_normal(6)
+    sig = xyz @ a.T + b
+    r = radius * mesh.pitch
+    got = S.material_average(xyz, mesh.enode, fill, mesh.pitch, r, sig)
+    blk = int(np.ceil(radius / 3.3))
+    live = fill > 0
+    lo = mesh.elem_ijk[live].min(axis=0)
+    blocks = {}
+    for e in np.nonzero(live)[0]:
+        k = tuple((mesh.elem_ijk[e] - lo) // blk)
+        w = fill[e] * mesh.pitch ** 3
+        c = mesh.origin + (mesh.elem_ijk[e] + 0.5) * mesh.pitch
+        acc = blocks.setdefault(k, [0.0, np.zeros(3)])
+        acc[0] += w
+        acc[1] += w * c
+    wb = np.array([v[0] for v in blocks.values()])
+    cb = np.array([v[1] / v[0] for v in blocks.values()])
+    for i in range(mesh.n_nodes):
+        inside = np.linalg.norm(cb - xyz[i], axis=1) <= r
+        if inside.any():
+            want = a @ (wb[inside] @ cb[inside] / wb[inside].sum()) + b
+        else:
+            want = sig[i]
+        assert np.allclose(got[i], want, rtol=1e-10, atol=1e-10)
+    assert len(blocks) < live.sum()
