@@ -69,7 +69,10 @@ int lp_hex8_ke(double nu, int element, double* ke_out /* 576 */);
 /* C B at the stress sample points, E = 1, unit pitch (constrained.py:383-387):
  * points 0 = the centre, fem.hex8_b_center (fem.py:307-308), 1 point;
  * points 1 = the 8 corners, x outermost, fem.hex8_b with the condensed
- * bubbles (fem.py:227-237). out (npts, 6, 24); divide by the pitch. */
+ * bubbles (fem.py:227-237); points 2 = the 8 quarter points (natural
+ * +-0.5, fem.CORNERS order: bit 0 = x) of constrained._Averaged
+ * (constrained.py:176-183), with the bubbles. out (npts, 6, 24); divide
+ * by the pitch. */
 int lp_hex8_cb(double nu, int element, int points, double* out);
 
 /* ---- grid handle: mg.GridElasticity (mg.py:411-511) --------------------- */
@@ -267,6 +270,24 @@ int lp_nodal_stress(int nx, int ny, int nz, double pitch, double nu, int element
  * within radius, or its own tensor when none is. Host arrays. */
 int lp_material_average(int nx, int ny, int nz, double pitch, double radius, const double* fill, const double* sig,
                         double* out);
+
+/* The constrained loop's material-averaged stress reading and its partials
+ * (constrained._Averaged, constrained.py:146-252, as the stress branch reads
+ * it at 517-605; no thin walls), on a box of nx x ny x nz cells of side
+ * pitch. u: ncase displacement fields (nodes, 3) node-major on the
+ * (nx+1)(ny+1)(nz+1) grid; xe (cells) the eroded densities (material
+ * max(xe, 0); 0 off the part); readable (nodes) the nodes read. Samples at
+ * each cell's quarter points while radius <= 1.5 cells, else its centre,
+ * with C B (element 0 "im", 1 "plain") on the cells that moved; bond NULL
+ * or {ratio, normal x, y, z, k} (Limits.bond). kappa: in the previous
+ * normalization per case, out this one (constrained._normalize at
+ * iteration it). Out per case: pm, smax; rhs (ncase, nodes, 3) the adjoint
+ * right-hand side (fixed dofs not zeroed) and gx (ncase, cells) the
+ * explicit dg/dxe, both NULL to skip. Host arrays. */
+int lp_stress_averaged(int nx, int ny, int nz, double pitch, double nu, int element, double radius, int ncase,
+                       const double* u, const double* xe, const uint8_t* readable, double p, double q,
+                       double limit, const double* bond, double* kappa, int it, double* pm, double* smax,
+                       double* rhs, double* gx);
 
 /* 26-connected components of a cell mask, scipy.ndimage.label(mask,
  * structure=ones((3,3,3))) as simp._Analysis._connected and _prune_dead use

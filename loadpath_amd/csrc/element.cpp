@@ -138,9 +138,11 @@ static void hex8_parts_host(double nu, int element, double* ke, double* r_out) {
 void hex8_ke_host(double nu, int element, double* ke) { hex8_parts_host(nu, element, ke, nullptr); }
 
 // C B at the stress sample points (constrained.py:383-387): the centre with
-// the trilinear B alone (fem.hex8_b_center, fem.py:307-308), or the 8 corners
-// (x outermost, then y, then z) with the bubbles' strain added (fem.hex8_b,
-// fem.py:227-237). out (npts, 6, 24) at E = 1 and unit pitch.
+// the trilinear B alone (fem.hex8_b_center, fem.py:307-308), the 8 corners
+// (x outermost, then y, then z), or the 8 quarter points (natural +-0.5,
+// bit 0 = x; constrained.py:176-183), the last two with the bubbles' strain
+// added (fem.hex8_b, fem.py:227-237). out (npts, 6, 24) at E = 1 and unit
+// pitch.
 void hex8_cb_host(double nu, int element, int points, double* out) {
     double c[6][6];
     isotropic(nu, c);
@@ -149,10 +151,12 @@ void hex8_cb_host(double nu, int element, int points, double* out) {
     const int npts = points == 0 ? 1 : 8;
     for (int k = 0; k < npts; ++k) {
         double nat[3] = {0.0, 0.0, 0.0};
-        if (points != 0) {
+        if (points == 1) {
             nat[0] = (k & 4) ? 1.0 : -1.0;
             nat[1] = (k & 2) ? 1.0 : -1.0;
             nat[2] = (k & 1) ? 1.0 : -1.0;
+        } else if (points == 2) {  // the quarter points, fem.CORNERS order (bit 0 = x)
+            for (int d = 0; d < 3; ++d) nat[d] = ((k >> d) & 1) ? 0.5 : -0.5;
         }
         double b[6 * 24], ba[6 * 9];
         b_at(nat, b, ba);
@@ -190,7 +194,7 @@ void child_prolongations_host(double* pk) {
 }  // namespace lp
 
 extern "C" int lp_hex8_cb(double nu, int element, int points, double* out) {
-    if (!out || (element != 0 && element != 1) || (points != 0 && points != 1) || !(nu > -1.0 && nu < 0.5)) {
+    if (!out || (element != 0 && element != 1) || (points < 0 || points > 2) || !(nu > -1.0 && nu < 0.5)) {
         lp::set_error("lp_hex8_cb: bad argument");
         return LP_ERR_ARG;
     }

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "lp_internal.h"
 
@@ -427,6 +428,237 @@ __global__ void __launch_bounds__(256) k_avg_block_ball(int nx, int ny, int nz, 
     for (int r = 0; r < 6; ++r) out[6 * n + r] = tot > 0.0 ? acc[r] / tot : sig[6 * n + r];
 }
 
+// ---- the constrained loop's averaged stress reading ----------------------
+// constrained._Averaged (constrained.py:146-252) as the stress branch reads it
+// (constrained.py:517-605), on a box of cells: samples at each cell's quarter
+// points (rho <= 1.5 cells) or its centre, sample stress C B(s) u_e / pitch on
+// the cells that moved, sample weight w0 mat; each readable node averages the
+// samples within the radius. Node-centric gathers throughout: the ball sum
+// of a node visits the cells in reach, the transposed sum of a sample visits
+// the nodes in reach, and the adjoint right-hand side of a node visits its
+// <= 8 cells; no atomics, so the sums run in a fixed order.
+struct AvArgs {
+    int nx, ny, nz, ns, reach;
+    double pitch, radius, w0;
+    const double* u;         // (nodes, 3) one load case
+    const double* mat;       // (cells) material share, max(xe, 0)
+    const uint8_t* readable; // (nodes)
+    const double* cb;        // (ns, 6, 24) C B / pitch
+    double* ss;              // (cells, ns, 6) sample stress
+    uint8_t* moved;          // (cells)
+    double* node;            // (nodes, 10): sig[6], rho, dinv, s_rel (von Mises), s_rel (bond)
+    double* ng;              // (nodes, 9): gsig dinv [6], (gsig . sig) dinv, grho dinv, grho rho dinv
+    double* hw;              // (cells, ns, 6) v * (ball^T gsig dinv)
+    double* gx;              // (cells) explicit dg/dxe
+    double* rhs;             // (nodes, 3)
+};
+
+struct AvLaw {
+    double p, q, lim, kappa, pm, nc;
+    int bond;
+    double lim_b, k2, m[6], Q[36];
+};
+
+__device__ __forceinline__ double av_off(int ns, int k, int d) { return ns == 8 ? 0.25 + 0.5 * ((k >> d) & 1) : 0.5; }
+
+__global__ void __launch_bounds__(256) k_av_samples(AvArgs A) {
+    const size_t nc = (size_t)A.nx * A.ny * A.nz;
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int X = (int)(c / ((size_t)A.ny * A.nz)), Y = (int)((c / A.nz) % A.ny), Z = (int)(c % A.nz);
+    const size_t Ny = (size_t)A.ny + 1, Nz = (size_t)A.nz + 1;
+    double ue[24];
+    bool mv = false;
+    for (int j = 0; j < 8; ++j) {
+        const size_t node = ((size_t)(X + (j & 1)) * Ny + (Y + ((j >> 1) & 1))) * Nz + (Z + (j >> 2));
+        for (int d = 0; d < 3; ++d) {
+            ue[3 * j + d] = A.u[3 * node + d];
+            mv = mv || ue[3 * j + d] != 0.0;
+        }
+    }
+    A.moved[c] = mv ? 1 : 0;
+    for (int k = 0; k < A.ns; ++k)
+        for (int i = 0; i < 6; ++i) {
+            double s = 0.0;
+            if (mv)
+                for (int j = 0; j < 24; ++j) s += A.cb[(k * 6 + i) * 24 + j] * ue[j];
+            A.ss[(c * A.ns + k) * 6 + i] = s;
+        }
+}
+
+__device__ __forceinline__ void av_vm(const double* sig, double* sv, double& f) {
+    const double h = 0.5 * (sig[0] + sig[1] + sig[2]);
+    sv[0] = 1.5 * sig[0] - h;
+    sv[1] = 1.5 * sig[1] - h;
+    sv[2] = 1.5 * sig[2] - h;
+    for (int i = 3; i < 6; ++i) sv[i] = 3.0 * sig[i];
+    double t = 0.0;
+    for (int i = 0; i < 6; ++i) t += sv[i] * sig[i];
+    f = sqrt(fmax(t, 1e-30));
+}
+
+__device__ __forceinline__ void av_bond(const AvLaw& L, const double* sig, double* sb, double& f) {
+    double sn = 0.0;
+    for (int i = 0; i < 6; ++i) sn += sig[i] * L.m[i];
+    sn = fmax(sn, 0.0);
+    for (int i = 0; i < 6; ++i) {
+        double t = 0.0;
+        for (int j = 0; j < 6; ++j) t += sig[j] * L.Q[j * 6 + i];
+        sb[i] = sn * L.m[i] + L.k2 * t;
+    }
+    double t = 0.0;
+    for (int i = 0; i < 6; ++i) t += sb[i] * sig[i];
+    f = sqrt(fmax(t, 1e-30));
+}
+
+__global__ void __launch_bounds__(256) k_av_nodes(AvArgs A, AvLaw L) {
+    const size_t Ny = (size_t)A.ny + 1, Nz = (size_t)A.nz + 1;
+    const size_t nn = ((size_t)A.nx + 1) * Ny * Nz;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    double* o = A.node + 10 * n;
+    if (!A.readable[n]) {
+        for (int i = 0; i < 10; ++i) o[i] = 0.0;
+        return;
+    }
+    const int X = (int)(n / (Ny * Nz)), Y = (int)((n / Nz) % Ny), Z = (int)(n % Nz);
+    const double r2 = A.radius * A.radius;
+    double D = 0.0, R = 0.0, S[6] = {0, 0, 0, 0, 0, 0};
+    for (int cx = max(X - A.reach - 1, 0); cx <= min(X + A.reach, A.nx - 1); ++cx)
+        for (int cy = max(Y - A.reach - 1, 0); cy <= min(Y + A.reach, A.ny - 1); ++cy)
+            for (int cz = max(Z - A.reach - 1, 0); cz <= min(Z + A.reach, A.nz - 1); ++cz) {
+                const size_t c = ((size_t)cx * A.ny + cy) * A.nz + cz;
+                const double mt = A.mat[c];
+                const double v = A.w0 * mt;
+                if (v == 0.0) continue;
+                for (int k = 0; k < A.ns; ++k) {
+                    const double dx = (cx + av_off(A.ns, k, 0) - X) * A.pitch,
+                                 dy = (cy + av_off(A.ns, k, 1) - Y) * A.pitch,
+                                 dz = (cz + av_off(A.ns, k, 2) - Z) * A.pitch;
+                    if (dx * dx + dy * dy + dz * dz > r2) continue;
+                    D += v;
+                    R += v * mt;
+                    const double* sp = A.ss + (c * A.ns + k) * 6;
+                    for (int i = 0; i < 6; ++i) S[i] += v * sp[i];
+                }
+            }
+    const double dinv = D > 0.0 ? 1.0 / D : 0.0;
+    const double rho = R * dinv;
+    double sig[6], sh[6], f;
+    for (int i = 0; i < 6; ++i) sig[i] = S[i] * dinv;
+    for (int i = 0; i < 6; ++i) o[i] = sig[i];
+    o[6] = rho;
+    o[7] = dinv;
+    const double rq = pow(rho, L.q);
+    av_vm(sig, sh, f);
+    o[8] = rq * f / L.lim;
+    o[9] = 0.0;
+    if (L.bond) {
+        av_bond(L, sig, sh, f);
+        o[9] = rq * f / L.lim_b;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_av_nodegrad(AvArgs A, AvLaw L) {
+    const size_t nn = ((size_t)A.nx + 1) * ((size_t)A.ny + 1) * ((size_t)A.nz + 1);
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    double* g = A.ng + 9 * n;
+    if (!A.readable[n]) {
+        for (int i = 0; i < 9; ++i) g[i] = 0.0;
+        return;
+    }
+    const double* o = A.node + 10 * n;
+    const double rho = o[6], dinv = o[7];
+    const double rq = pow(rho, L.q);
+    double gsig[6] = {0, 0, 0, 0, 0, 0}, grho = 0.0, sh[6], f;
+    for (int part = 0; part < (L.bond ? 2 : 1); ++part) {
+        double lim;
+        if (part == 0) {
+            av_vm(o, sh, f);
+            lim = L.lim;
+        } else {
+            av_bond(L, o, sh, f);
+            lim = L.lim_b;
+        }
+        const double sr = rq * f / lim;
+        const double a = L.kappa * pow(sr / fmax(L.pm, 1e-30), L.p - 1.0) / L.nc;
+        const double s = a * rq / (f * lim);
+        for (int i = 0; i < 6; ++i) gsig[i] += s * sh[i];
+        grho += a * L.q * pow(fmax(rho, 1e-12), L.q - 1.0) * f / lim;
+    }
+    double gs = 0.0;
+    for (int i = 0; i < 6; ++i) {
+        g[i] = gsig[i] * dinv;
+        gs += gsig[i] * o[i];
+    }
+    g[6] = gs * dinv;
+    g[7] = grho * dinv;
+    g[8] = grho * rho * dinv;
+}
+
+__global__ void __launch_bounds__(256) k_av_cells(AvArgs A) {
+    const size_t nc = (size_t)A.nx * A.ny * A.nz;
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int X = (int)(c / ((size_t)A.ny * A.nz)), Y = (int)((c / A.nz) % A.ny), Z = (int)(c % A.nz);
+    const size_t Ny = (size_t)A.ny + 1, Nz = (size_t)A.nz + 1;
+    const double r2 = A.radius * A.radius;
+    const double mt = A.mat[c];
+    const double v = A.w0 * mt;
+    const bool mv = A.moved[c] != 0;
+    double t = 0.0;
+    for (int k = 0; k < A.ns; ++k) {
+        const double px = X + av_off(A.ns, k, 0), py = Y + av_off(A.ns, k, 1), pz = Z + av_off(A.ns, k, 2);
+        double hs[6] = {0, 0, 0, 0, 0, 0}, T1 = 0.0, T2 = 0.0, T3 = 0.0;
+        // nodes within rho of a point in the cell lie in [X - reach, X + 1 + reach]
+        for (int nx_ = max(X - A.reach, 0); nx_ <= min(X + 1 + A.reach, A.nx); ++nx_)
+            for (int ny_ = max(Y - A.reach, 0); ny_ <= min(Y + 1 + A.reach, A.ny); ++ny_)
+                for (int nz_ = max(Z - A.reach, 0); nz_ <= min(Z + 1 + A.reach, A.nz); ++nz_) {
+                    const double dx = (px - nx_) * A.pitch, dy = (py - ny_) * A.pitch, dz = (pz - nz_) * A.pitch;
+                    if (dx * dx + dy * dy + dz * dz > r2) continue;
+                    const size_t n = ((size_t)nx_ * Ny + ny_) * Nz + nz_;
+                    if (!A.readable[n]) continue;
+                    const double* g = A.ng + 9 * n;
+                    for (int i = 0; i < 6; ++i) hs[i] += g[i];
+                    T1 += g[6];
+                    T2 += g[7];
+                    T3 += g[8];
+                }
+        const double* sp = A.ss + (c * A.ns + k) * 6;
+        double sdot = 0.0;
+        for (int i = 0; i < 6; ++i) sdot += sp[i] * hs[i];
+        t += (sdot - T1) + (2.0 * mt * T2 - T3);
+        double* h = A.hw + (c * A.ns + k) * 6;
+        for (int i = 0; i < 6; ++i) h[i] = mv ? v * hs[i] : 0.0;
+    }
+    A.gx[c] = A.w0 * t;
+}
+
+__global__ void __launch_bounds__(256) k_av_rhs(AvArgs A) {
+    const size_t Ny = (size_t)A.ny + 1, Nz = (size_t)A.nz + 1;
+    const size_t nn = ((size_t)A.nx + 1) * Ny * Nz;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    const int X = (int)(n / (Ny * Nz)), Y = (int)((n / Nz) % Ny), Z = (int)(n % Nz);
+    double r[3] = {0, 0, 0};
+    for (int j = 0; j < 8; ++j) {
+        const int cx = X - (j & 1), cy = Y - ((j >> 1) & 1), cz = Z - (j >> 2);
+        if (cx < 0 || cy < 0 || cz < 0 || cx >= A.nx || cy >= A.ny || cz >= A.nz) continue;
+        const size_t c = ((size_t)cx * A.ny + cy) * A.nz + cz;
+        if (!A.moved[c]) continue;
+        for (int k = 0; k < A.ns; ++k) {
+            const double* h = A.hw + (c * A.ns + k) * 6;
+            for (int d = 0; d < 3; ++d) {
+                double s = 0.0;
+                for (int i = 0; i < 6; ++i) s += h[i] * A.cb[(k * 6 + i) * 24 + 3 * j + d];
+                r[d] += s;
+            }
+        }
+    }
+    for (int d = 0; d < 3; ++d) A.rhs[3 * n + d] = r[d];
+}
+
 }  // namespace
 }  // namespace lp
 
@@ -677,6 +909,132 @@ int lp_material_average(int nx, int ny, int nz, double pitch, double radius, con
                            dout.as<double>());
         LP_HIP(hipGetLastError());
         LP_HIP(hipMemcpy(out, dout.p, nn * 6 * 8, hipMemcpyDeviceToHost));
+        return LP_OK;
+    });
+}
+
+int lp_stress_averaged(int nx, int ny, int nz, double pitch, double nu, int element, double radius, int ncase,
+                       const double* u, const double* xe, const uint8_t* readable, double p, double q,
+                       double limit, const double* bond, double* kappa, int it, double* pm, double* smax,
+                       double* rhs, double* gx) {
+    return api_call([&]() -> int {
+        if (nx < 1 || ny < 1 || nz < 1 || !(pitch > 0.0) || !(radius > 0.0) || ncase < 1 || !u || !xe ||
+            !readable || !(limit > 0.0) || !(p > 0.0) || !kappa || !pm || !smax || (element != 0 && element != 1) ||
+            !(nu > -1.0 && nu < 0.5) || (bond && !(bond[0] > 0.0)) || radius / pitch > 1e4 || (!rhs != !gx)) {
+            set_error("lp_stress_averaged: bad argument");
+            return LP_ERR_ARG;
+        }
+        const double rho = radius / pitch;
+        const int ns = rho <= 1.5 ? 8 : 1;
+        const size_t nc = (size_t)nx * ny * nz;
+        const size_t nn = (size_t)(nx + 1) * (ny + 1) * (nz + 1);
+        std::vector<double> cbh((size_t)ns * 6 * 24);
+        hex8_cb_host(nu, element, ns == 8 ? 2 : 0, cbh.data());
+        for (double& v : cbh) v /= pitch;
+        AvLaw L{};
+        L.p = p;
+        L.q = q;
+        L.lim = limit;
+        L.bond = bond ? 1 : 0;
+        if (bond) {
+            // constrained._bond_form (constrained.py:262-273): the traction on
+            // the layer plane t (3 x 6), m = n t, Q = t^T t - m m^T
+            double nrm[3] = {bond[1], bond[2], bond[3]};
+            const double len = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+            if (!(len > 0.0)) {
+                set_error("lp_stress_averaged: bond normal is zero");
+                return LP_ERR_ARG;
+            }
+            for (double& v : nrm) v /= len;
+            const double t[3][6] = {{nrm[0], 0, 0, nrm[1], 0, nrm[2]},
+                                    {0, nrm[1], 0, nrm[0], nrm[2], 0},
+                                    {0, 0, nrm[2], 0, nrm[1], nrm[0]}};
+            for (int i = 0; i < 6; ++i) L.m[i] = nrm[0] * t[0][i] + nrm[1] * t[1][i] + nrm[2] * t[2][i];
+            for (int i = 0; i < 6; ++i)
+                for (int j = 0; j < 6; ++j)
+                    L.Q[i * 6 + j] = t[0][i] * t[0][j] + t[1][i] * t[1][j] + t[2][i] * t[2][j] - L.m[i] * L.m[j];
+            L.lim_b = limit * bond[0];
+            L.k2 = bond[4] * bond[4];
+        }
+        size_t nread = 0;
+        for (size_t i = 0; i < nn; ++i) nread += readable[i] ? 1 : 0;
+        L.nc = (double)(std::max<size_t>(nread, 1) * (bond ? 2 : 1));
+        std::vector<double> mat(nc);
+        for (size_t i = 0; i < nc; ++i) mat[i] = std::max(xe[i], 0.0);
+        DBuf du, dmat, drd, dcb, dss, dmv, dnode, dng, dhw, dgx, drhs;
+        du.alloc(nn * 3 * 8);
+        dmat.alloc(nc * 8);
+        drd.alloc(nn);
+        dcb.alloc(cbh.size() * 8);
+        dss.alloc(nc * ns * 6 * 8);
+        dmv.alloc(nc);
+        dnode.alloc(nn * 10 * 8);
+        dng.alloc(nn * 9 * 8);
+        dhw.alloc(nc * ns * 6 * 8);
+        dgx.alloc(nc * 8);
+        drhs.alloc(nn * 3 * 8);
+        LP_HIP(hipMemcpy(dmat.p, mat.data(), nc * 8, hipMemcpyHostToDevice));
+        LP_HIP(hipMemcpy(drd.p, readable, nn, hipMemcpyHostToDevice));
+        LP_HIP(hipMemcpy(dcb.p, cbh.data(), cbh.size() * 8, hipMemcpyHostToDevice));
+        AvArgs A{};
+        A.nx = nx;
+        A.ny = ny;
+        A.nz = nz;
+        A.ns = ns;
+        A.reach = (int)std::ceil(rho);
+        A.pitch = pitch;
+        A.radius = radius;
+        A.w0 = pitch * pitch * pitch / ns;
+        A.u = du.as<double>();
+        A.mat = dmat.as<double>();
+        A.readable = drd.as<uint8_t>();
+        A.cb = dcb.as<double>();
+        A.ss = dss.as<double>();
+        A.moved = dmv.as<uint8_t>();
+        A.node = dnode.as<double>();
+        A.ng = dng.as<double>();
+        A.hw = dhw.as<double>();
+        A.gx = dgx.as<double>();
+        A.rhs = drhs.as<double>();
+        const dim3 gc((unsigned)((nc + 255) / 256)), gn((unsigned)((nn + 255) / 256)), blk(256);
+        std::vector<double> node(nn * 10);
+        for (int cs = 0; cs < ncase; ++cs) {
+            LP_HIP(hipMemcpy(du.p, u + (size_t)cs * nn * 3, nn * 3 * 8, hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_av_samples, gc, blk, 0, nullptr, A);
+            hipLaunchKernelGGL(k_av_nodes, gn, blk, 0, nullptr, A, L);
+            LP_HIP(hipGetLastError());
+            LP_HIP(hipMemcpy(node.data(), dnode.p, nn * 10 * 8, hipMemcpyDeviceToHost));
+            // s_max and the P-mean over the readable nodes (constrained.py:562-565)
+            double mx = 0.0;
+            bool any = false;
+            for (size_t i = 0; i < nn; ++i)
+                if (readable[i]) {
+                    mx = any ? std::max(mx, node[10 * i + 8]) : node[10 * i + 8];
+                    any = true;
+                    if (bond) mx = std::max(mx, node[10 * i + 9]);
+                }
+            const double sc = std::max(mx, 1e-30);
+            double sum = 0.0;
+            for (size_t i = 0; i < nn; ++i)
+                if (readable[i]) sum += std::pow(node[10 * i + 8] / sc, p);
+            if (bond)
+                for (size_t i = 0; i < nn; ++i)
+                    if (readable[i]) sum += std::pow(node[10 * i + 9] / sc, p);
+            const double pmv = sc * std::pow(sum / L.nc, 1.0 / p);
+            const double kr = mx / std::max(pmv, 1e-30);
+            kappa[cs] = it > 1 ? 0.5 * (kappa[cs] + kr) : kr;
+            pm[cs] = pmv;
+            smax[cs] = mx;
+            if (!rhs) continue;
+            L.kappa = kappa[cs];
+            L.pm = pmv;
+            hipLaunchKernelGGL(k_av_nodegrad, gn, blk, 0, nullptr, A, L);
+            hipLaunchKernelGGL(k_av_cells, gc, blk, 0, nullptr, A);
+            hipLaunchKernelGGL(k_av_rhs, gn, blk, 0, nullptr, A);
+            LP_HIP(hipGetLastError());
+            LP_HIP(hipMemcpy(rhs + (size_t)cs * nn * 3, drhs.p, nn * 3 * 8, hipMemcpyDeviceToHost));
+            LP_HIP(hipMemcpy(gx + (size_t)cs * nc, dgx.p, nc * 8, hipMemcpyDeviceToHost));
+        }
         return LP_OK;
     });
 }

@@ -14,6 +14,13 @@ minimum-mass study:
   solve (the grid's MG-PCG, one batched solve for all cases) and returns
   dg/dx = explicit - de * lambda_e^T Ke u_e.
 
+- ``StressAveraged(mesh, nu, radius, readable, p, q, limit, bond)``: the
+  averaged branch (constrained._Averaged, constrained.py:146-252, read at
+  517-605, no thin walls): each readable node's stress averaged over the
+  material within radius, relaxed by the density round it, the P-mean over
+  the nodes with Le's normalization, the adjoint right-hand side and the
+  explicit dg/dxe (lp_stress_averaged).
+
 x is the material share per element (constrained.material with no share:
 x itself); de = dE/dx of the interpolation in use.
 """
@@ -86,3 +93,74 @@ class StressPNorm:
         de = np.asarray(de, float)
         grads = [gx - de * self.grid.element_product(lam, u) for gx, lam, u in zip(ev["gx"], lams, us)]
         return ev["g"], grads
+
+
+class StressAveraged:
+    """constrained._Averaged(mesh, nu, radius, readable) and the stress
+    branch that reads it (constrained.py:517-605) on the device, for a
+    HexMesh with no thin walls. bond: Limits.bond, (ratio, normal, k), or
+    None. fixed: the dofs the adjoint right-hand side is zero on."""
+
+    def __init__(self, mesh, nu: float, radius: float, readable=None, p: float = P_DEFAULT, q: float = Q_DEFAULT,
+                 limit: float = 1.0, bond=None, element: str = "im", fixed=None):
+        from .verify import _grid_of
+
+        if element not in ("im", "plain"):
+            raise ValueError(f"element must be 'im' or 'plain', not {element!r}")
+        self.mesh = mesh
+        self.nu, self.radius = float(nu), float(radius)
+        self.p, self.q, self.limit = float(p), float(q), float(limit)
+        self.element = element
+        self.bond = None if bond is None else np.array([float(bond[0]), *map(float, bond[1]), float(bond[2])])
+        self._dims, _, self._nodes, self._cells = _grid_of(mesh)
+        nn = int(np.prod([d + 1 for d in self._dims]))
+        rd = np.ones(mesh.n_nodes, bool) if readable is None else np.asarray(readable, bool)
+        if rd.shape != (mesh.n_nodes,):
+            raise ValueError("readable must be (n_nodes,) bool")
+        self._readable = np.zeros(nn, np.uint8)
+        self._readable[self._nodes] = rd
+        self.fixed = None if fixed is None else np.asarray(fixed)
+
+    def evaluate(self, us, xe, kappa=None, it: int = 1, need_grad: bool = True) -> list:
+        """Per load case dict(g, pm, smax, kappa, rhs, gx): kappa is
+        constrained._normalize of the previous one (kappa) at iteration it."""
+        _lib.require_gpu()
+        us = [np.asarray(u, float) for u in us]
+        nr = len(us)
+        nn = len(self._readable)
+        ncell = int(np.prod(self._dims))
+        ug = np.zeros((nr, nn, 3))
+        for i, u in enumerate(us):
+            ug[i, self._nodes] = u.reshape(-1, 3)
+        xg = np.zeros(ncell)
+        xg[self._cells] = np.asarray(xe, float)
+        kap = np.ones(nr) if kappa is None else np.array(np.broadcast_to(np.asarray(kappa, float), (nr,)))
+        pm, smax = np.empty(nr), np.empty(nr)
+        rhs = np.empty((nr, nn, 3)) if need_grad else None
+        gx = np.empty((nr, ncell)) if need_grad else None
+        _lib.check(_lib.lib().lp_stress_averaged(
+            *self._dims, float(self.mesh.pitch), self.nu, 0 if self.element == "im" else 1, self.radius, nr,
+            _lib.ptr(ug), _lib.ptr(xg), _lib.ptr(self._readable), self.p, self.q, self.limit,
+            _lib.ptr(self.bond) if self.bond is not None else None, _lib.ptr(kap), int(it), _lib.ptr(pm),
+            _lib.ptr(smax), _lib.ptr(rhs) if need_grad else None, _lib.ptr(gx) if need_grad else None),
+            "stress_averaged")
+        out = []
+        for i in range(nr):
+            rec = dict(g=kap[i] * pm[i], pm=pm[i], smax=smax[i], kappa=kap[i])
+            if need_grad:
+                r = rhs[i, self._nodes].ravel()
+                if self.fixed is not None:
+                    r[self.fixed] = 0.0
+                rec["rhs"] = r
+                rec["gx"] = gx[i, self._cells]
+            out.append(rec)
+        return out
+
+    def gradient(self, grid, us, xe, de, kappa=None, it: int = 1):
+        """(records, dg/dxe per case): the adjoint solve on grid (a device
+        GridElasticity over the same mesh), dg/dxe = explicit - de lambda_e^T
+        Ke u_e (constrained.py:603-605)."""
+        ev = self.evaluate(us, xe, kappa, it)
+        lams = grid.solve_many([e["rhs"] for e in ev])
+        de = np.asarray(de, float)
+        return ev, [e["gx"] - de * grid.element_product(lam, u) for e, lam, u in zip(ev, lams, us)]

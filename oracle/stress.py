@@ -36,6 +36,21 @@ Pinned (tests/test_oracle_stress.py) by the reference's own closed forms
 C eps at every node, and a one-cell-deep strip in pure bending reads the
 top-fibre stress M c / I with the "im" element; a free thermal expansion
 reads zero stress; and a per-cell loop over corners (no np.add.at).
+
+Also the material-averaged stress reading of the constrained loop
+(constrained._Averaged, constrained.py:146-252, used at 517-605; no thin
+walls): samples at each cell's quarter points (natural +-0.5, fem.CORNERS
+order) while radius <= 1.5 cells, else its centre, sigma_s = C B(s) u_e /
+pitch on the cells that moved (any nonzero displacement at a corner) and 0
+elsewhere; weights v_s = pitch^3 / ns * mat; per readable node the ball of
+samples within radius: D = sum v, sig = sum v sigma / D, rho = sum v mat / D;
+vm (and the layer bond) from sig, s_rel = rho^q f / lim, pm as above over
+the readable nodes (twice as many terms with a bond), kappa normalized;
+the adjoint right-hand side through the samples of moved cells and the
+explicit dg/dx through the weights. Pinned by the rhs and dg/dx against
+central differences of pm (directional, random directions), with and
+without a bond, and by the closed form of a uniform stress (every node
+reads C eps, pm = vm / lim).
 """
 from __future__ import annotations
 
@@ -224,4 +239,105 @@ def material_average(node_xyz, enode, fill, pitch: float, radius: float, sig):
     lone = tot <= 0
     out = out / np.where(lone, 1.0, tot)[:, None]
     out[lone] = sig[lone]
+    return out
+
+
+def bond_form(normal):
+    """constrained._bond_form (constrained.py:262-273): (m, Q) with the
+    stress normal to layers stacked along `normal` m . sigma and the squared
+    shear on them sigma . Q . sigma (Voigt: sx, sy, sz, txy, tyz, tzx)."""
+    n = np.asarray(normal, float)
+    n = n / np.linalg.norm(n)
+    t = np.array([[n[0], 0, 0, n[1], 0, n[2]], [0, n[1], 0, n[0], n[2], 0], [0, 0, n[2], 0, n[1], n[0]]])
+    m = n @ t
+    return m, t.T @ t - np.outer(m, m)
+
+
+def normalize_kappa(prev, smax, pm, it):
+    """constrained._normalize (constrained.py:255-259)."""
+    k = smax / max(pm, 1e-30)
+    return 0.5 * (prev + k) if it > 1 else k
+
+
+def averaged_samples(nu, rho, element="im"):
+    """(natural coordinates (ns, 3), C B at them (ns, 6, 24) at unit pitch)
+    (constrained.py:176-183)."""
+    corners = np.array([[k & 1, (k >> 1) & 1, (k >> 2) & 1] for k in range(8)], float)
+    nat = corners - 0.5 if rho <= 1.5 else np.zeros((1, 3))
+    c0 = isotropic_c(1.0, nu)
+    return nat, np.array([c0 @ hex8_b(a, b, c, nu, element) for a, b, c in nat])
+
+
+def averaged_pnorm(mesh, nu, radius, readable, us, xe, *, p=P_DEFAULT, q=Q_DEFAULT, limit=1.0, bond=None,
+                   kappa_prev=None, it=1, element="im", fixed=None):
+    """The averaged stress branch of constrained._constrained_run
+    (constrained.py:517-605) over mesh (a Mesh), for displacements us (one
+    per load case, n_dof each), eroded densities xe, readable nodes (bool
+    per node) and bond = (ratio, normal, k) or None. Per case: dict(g, pm,
+    smax, kappa, rhs (n_dof; zero on `fixed`), gx (explicit dg/dxe))."""
+    import scipy.sparse as sp
+    from scipy.spatial import cKDTree
+
+    pitch = mesh.pitch
+    nat, cb = averaged_samples(nu, radius / pitch, element)
+    cb = cb / pitch
+    ns = len(nat)
+    ne = mesh.n_elem
+    xs = ((mesh.elem_ijk[:, None, :] + 0.5 + 0.5 * nat[None]) * pitch).reshape(-1, 3)
+    nodes = np.flatnonzero(readable)
+    pr = cKDTree(mesh.node_ijk[nodes] * pitch).sparse_distance_matrix(cKDTree(xs), radius, output_type="coo_matrix")
+    ball = sp.csr_matrix((np.ones(len(pr.row)), (pr.row, pr.col)), shape=(len(nodes), len(xs)))
+    ballT = ball.T.tocsr()
+    w0 = pitch ** 3 / ns
+    mat = np.maximum(np.asarray(xe, float), 0.0)
+    v = np.repeat(w0 * mat, ns)
+    xd = np.repeat(mat, ns)
+    dn = ball @ v
+    dinv = np.where(dn > 0, 1.0 / np.where(dn > 0, dn, 1.0), 0.0)
+    rho_n = (ball @ (v * xd)) * dinv
+    nc = max(len(nodes), 1) * (2 if bond is not None else 1)
+    if bond is not None:
+        bm, bq = bond_form(bond[1])
+        k2 = float(bond[2]) ** 2
+    edof = mesh.edof
+    out = []
+    for ci, u in enumerate(us):
+        u = np.asarray(u, float)
+        moved = np.flatnonzero(u.reshape(-1, 3).any(axis=1)[mesh.enode].any(axis=1))
+        ue = u[edof[moved]]
+        ss = np.zeros((ne, ns, 6))
+        for k in range(ns):
+            ss[moved, k] = ue @ cb[k].T
+        ss = ss.reshape(-1, 6)
+        sig = (ball @ (v[:, None] * ss)) * dinv[:, None]
+        sv = sig @ VM
+        parts = [(sv, np.sqrt(np.maximum(np.einsum("ij,ij->i", sv, sig), 1e-30)), limit)]
+        if bond is not None:
+            sn = np.maximum(sig @ bm, 0.0)
+            sb = sn[:, None] * bm + k2 * (sig @ bq)
+            parts.append((sb, np.sqrt(np.maximum(np.einsum("ij,ij->i", sb, sig), 1e-30)), limit * float(bond[0])))
+        srs = [rho_n ** q * f / lim for _, f, lim in parts]
+        smax = max((float(sr.max()) for sr in srs if len(sr)), default=0.0)
+        sc = max(smax, 1e-30)
+        pm = sc * (sum(float(np.sum((sr / sc) ** p)) for sr in srs) / nc) ** (1.0 / p)
+        prev = 1.0 if kappa_prev is None else float(np.atleast_1d(kappa_prev)[ci])
+        kappa = normalize_kappa(prev, smax, pm, it)
+        gsig = np.zeros_like(sig)
+        grho = np.zeros(len(sig))
+        for (sh, f, lim), sr in zip(parts, srs):
+            a = kappa * (sr / max(pm, 1e-30)) ** (p - 1) / nc
+            gsig += (a * rho_n ** q / (f * lim))[:, None] * sh
+            grho += a * q * np.maximum(rho_n, 1e-12) ** (q - 1) * f / lim
+        hs = ballT @ (gsig * dinv[:, None])
+        hw = v[:, None] * hs
+        t_sig = np.einsum("ij,ij->i", ss, hs) - ballT @ (np.einsum("ij,ij->i", gsig, sig) * dinv)
+        t_rho = 2.0 * xd * (ballT @ (grho * dinv)) - ballT @ (grho * rho_n * dinv)
+        gx = w0 * (t_sig + t_rho).reshape(ne, ns).sum(axis=1)
+        hw = hw.reshape(ne, ns, 6)
+        rhs = np.zeros(mesh.n_dof)
+        for k in range(ns):
+            np.add.at(rhs, edof[moved].ravel(), (hw[moved, k] @ cb[k]).ravel())
+        if fixed is not None:
+            rhs[np.asarray(fixed)] = 0.0
+        out.append(dict(g=kappa * pm, pm=pm, smax=smax, kappa=kappa, rhs=rhs, gx=gx))
     return out

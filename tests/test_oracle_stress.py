@@ -303,3 +303,65 @@ def test_material_average_blocks_on_a_linear_field(radius):
             want = sig[i]
         assert np.allclose(got[i], want, rtol=1e-10, atol=1e-10)
     assert len(blocks) < live.sum()
+
+
+# ---- the constrained loop's averaged stress reading (constrained.py:146-252, 517-605)
+
+def _averaged_case(seed=3, shape=(5, 4, 3)):
+    from oracle.fem import Mesh
+
+    rng = np.random.default_rng(seed)
+    g = rng.random(shape) < 0.85
+    g[0, 0, 0] = True
+    mesh = Mesh(np.argwhere(g), 0.7)
+    u = rng.standard_normal(mesh.n_dof)
+    xe = rng.uniform(0.2, 1.0, mesh.n_elem)
+    readable = rng.random(mesh.n_nodes) < 0.8
+    return rng, mesh, u, xe, readable
+
+
+@pytest.mark.parametrize("radius,bond", [(1.2, None), (2.0, None), (1.2, (0.6, (0.3, -0.2, 0.9), 0.8))])
+def test_averaged_gradients_against_central_differences(radius, bond):
+    """The adjoint right-hand side is kappa dpm/du and the explicit part
+    kappa dpm/dxe at fixed u: directional central differences of pm along
+    random directions (every cell moved)."""
+    rng, mesh, u, xe, readable = _averaged_case()
+    r = radius * mesh.pitch
+    kw = dict(p=8.0, q=1.0, limit=1.7, bond=bond)
+    ev = S.averaged_pnorm(mesh, 0.3, r, readable, [u], xe, **kw)[0]
+
+    def pm(uu, xx):
+        return S.averaged_pnorm(mesh, 0.3, r, readable, [uu], xx, **kw)[0]["pm"]
+
+    for _ in range(3):
+        du = rng.standard_normal(mesh.n_dof)
+        h = 1e-5
+        fd = (pm(u + h * du, xe) - pm(u - h * du, xe)) / (2 * h)
+        assert ev["rhs"] @ du / ev["kappa"] == pytest.approx(fd, rel=1e-6)
+        dx = rng.standard_normal(mesh.n_elem)
+        fd = (pm(u, xe + h * dx) - pm(u, xe - h * dx)) / (2 * h)
+        assert ev["gx"] @ dx / ev["kappa"] == pytest.approx(fd, rel=1e-6)
+
+
+def test_averaged_uniform_stress_reads_c_eps():
+    """An affine displacement field on a full, fully dense block: every
+    sample reads C eps, so every node does, rho = 1 and pm = s_max =
+    vm(C eps) / lim; kappa = 1 at the first iteration, the mean of the two
+    readings after it."""
+    from oracle.fem import Mesh, isotropic_c
+
+    mesh = Mesh(np.argwhere(np.ones((4, 3, 3), bool)), 0.5)
+    a = np.array([[1e-3, 2e-4, -3e-4], [5e-4, -2e-3, 1e-4], [-1e-4, 3e-4, 7e-4]])
+    u = (mesh.node_ijk * mesh.pitch @ a.T).ravel()
+    eps = np.array([a[0, 0], a[1, 1], a[2, 2], a[0, 1] + a[1, 0], a[1, 2] + a[2, 1], a[2, 0] + a[0, 2]])
+    sig = isotropic_c(1.0, 0.3) @ eps
+    vm = np.sqrt(sig @ S.VM @ sig)
+    for radius in (0.8, 1.9):
+        ev = S.averaged_pnorm(mesh, 0.3, radius * mesh.pitch, np.ones(mesh.n_nodes, bool), [u],
+                              np.ones(mesh.n_elem), limit=2.0)[0]
+        assert ev["pm"] == pytest.approx(vm / 2.0, rel=1e-10)
+        assert ev["smax"] == pytest.approx(vm / 2.0, rel=1e-10)
+        assert ev["kappa"] == pytest.approx(1.0, rel=1e-10)
+        ev2 = S.averaged_pnorm(mesh, 0.3, radius * mesh.pitch, np.ones(mesh.n_nodes, bool), [u],
+                               np.ones(mesh.n_elem), limit=2.0, kappa_prev=[3.0], it=2)[0]
+        assert ev2["kappa"] == pytest.approx(2.0, rel=1e-10)
