@@ -326,6 +326,9 @@ int lp_mma_create(lp_mma** out, long long n, int m, const double* xmin, const do
                   double asyinit, double asyincr, double asydecr);
 int lp_mma_destroy(lp_mma* mma);
 int lp_mma_tighten(lp_mma* mma, double factor);
+/* MMA.move = move (the move limit of the next update; the constrained loop
+ * halves and grows it between iterations, constrained.py:783-795). */
+int lp_mma_set_move(lp_mma* mma, double move);
 int lp_mma_update(lp_mma* mma, const double* x, double f0, const double* df0, const double* f, const double* df,
                   double* xnew, int device_ptr);
 int lp_mma_get(lp_mma* mma, int what, double* out);

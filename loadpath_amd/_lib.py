@@ -132,6 +132,7 @@ _SIGS = {
     "lp_mma_create": (_I, [C.POINTER(_P), C.c_longlong, _I, _P, _P, _D, _D, _D, _D, _D]),
     "lp_mma_destroy": (_I, [_P]),
     "lp_mma_tighten": (_I, [_P, _D]),
+    "lp_mma_set_move": (_I, [_P, _D]),
     "lp_mma_update": (_I, [_P, _P, _D, _P, _P, _P, _P, _I]),
     "lp_mma_get": (_I, [_P, _I, _P]),
 }

@@ -522,6 +522,17 @@ int lp_mma_destroy(lp_mma* M) {
     return LP_OK;
 }
 
+int lp_mma_set_move(lp_mma* M, double move) {
+    return api_call([&]() -> int {
+        if (!M || !(move > 0)) {
+            set_error("lp_mma_set_move: bad argument");
+            return LP_ERR_ARG;
+        }
+        M->move = move;
+        return LP_OK;
+    });
+}
+
 int lp_mma_tighten(lp_mma* M, double factor) {
     return api_call([&]() -> int {
         if (!M) return LP_ERR_ARG;

@@ -34,6 +34,18 @@ class MMA:
         _lib.check(_lib.lib().lp_mma_create(C.byref(h), self.n, self.m, _lib.ptr(lo), _lib.ptr(hi), float(move),
                                             float(c), float(asyinit), float(asyincr), float(asydecr)), "mma_create")
         self._h = h
+        self._move = float(move)
+
+    @property
+    def move(self):
+        """The move limit of the next update (settable, as the reference's
+        loops set it)."""
+        return self._move
+
+    @move.setter
+    def move(self, value):
+        _lib.check(_lib.lib().lp_mma_set_move(self._h, float(value)), "mma_set_move")
+        self._move = float(value)
 
     def __del__(self):
         h = getattr(self, "_h", None)

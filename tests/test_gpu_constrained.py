@@ -1,0 +1,62 @@
+"""The constrained minimum-mass loop on the device (loadpath_amd.
+constrained.optimize_constrained) against the oracle's restatement of
+constrained._constrained_run (oracle/constrained.py): the same volumes,
+constraint readings, move limits and beta steps iteration by iteration,
+with the corner-sampled stress, the averaged stress and the layer bond."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem():
+    from loadpath_amd.problems import cantilever_3d
+
+    return cantilever_3d(8, 2, 4)
+
+
+def _oracle_problem(p):
+    from oracle import simp as os_
+
+    return os_.Problem(p.name, p.dims, p.volfrac, p.rmin, list(p.fixed), [list(c) for c in p.cases])
+
+
+@pytest.mark.parametrize("radius,bond", [(None, None), (1.3, None), (2.2, (0.6, (0.0, 0.0, 1.0), 0.8))])
+def test_constrained_loop_matches_oracle(gpu, radius, bond):
+    from loadpath_amd.constrained import Limits, optimize_constrained
+    from loadpath_amd.simp import SimpOptions
+    from oracle.constrained import Limits as OLimits
+    from oracle.constrained import optimize_constrained as oracle_loop
+    from oracle.loop import LoopOptions
+
+    p = _problem()
+    pres = np.zeros(p.dims, bool)
+    pres[-1, :, :2] = True
+    n_it = 24
+    kw = dict(volume_fraction=0.5, filter_radius=1.5, max_iterations=n_it, beta_every=6, beta_max=8.0)
+    r = optimize_constrained(p, 0.3, SimpOptions(solve_rtol=1e-11, **kw),
+                             Limits(stress=80.0, deflection=5.0, modulus=1000.0, radius=radius, bond=bond),
+                             preserve=pres, precision="fp64", maxiter=4000, coarse_cells=50)
+    ro = oracle_loop(_oracle_problem(p), LoopOptions(**kw),
+                     OLimits(stress=80.0, deflection=5.0, modulus=1000.0, radius=radius, bond=bond),
+                     preserve=pres.ravel())
+    assert len(r.history) == len(ro["history"])
+    for a, b in zip(r.history, ro["history"]):
+        assert a["beta"] == b["beta"]
+        assert a["volume"] == pytest.approx(b["volume"], abs=1e-6)
+        assert np.allclose(a["constraints"], b["constraints"], rtol=0, atol=1e-5)
+        assert a["move"] == pytest.approx(b["move"], rel=1e-12)
+    assert np.abs(r.x - ro["x"]).max() < 1e-5
+    assert (r.delivered is None) == (ro["delivered"] is None)
+
+
+def test_constrained_rejects_what_it_does_not_build(gpu):
+    from loadpath_amd.constrained import Limits, optimize_constrained
+    from loadpath_amd.simp import SimpOptions
+
+    p = _problem()
+    o = SimpOptions(volume_fraction=0.5, filter_radius=1.5, max_iterations=2)
+    with pytest.raises(NotImplementedError):
+        optimize_constrained(p, 0.3, o, Limits(stress=80.0, deflection=None, modulus=1.0, buckling=2.0))
+    with pytest.raises(ValueError):
+        optimize_constrained(p, 0.3, o, Limits(stress=None, deflection=None, modulus=1.0))

@@ -1,0 +1,57 @@
+"""The oracle's constrained minimum-mass loop (oracle/constrained.py,
+constrained._constrained_run on a box): its constraint gradients against
+central differences through the direct solve, and the loop's invariants."""
+import numpy as np
+import pytest
+
+from oracle.constrained import Constrained, Limits, optimize_constrained
+from oracle.loop import LoopOptions
+from oracle.simp import cantilever_3d
+
+
+def _setup(radius=None, bond=None):
+    p = cantilever_3d(8, 2, 4)
+    opts = LoopOptions(volume_fraction=0.5, filter_radius=1.5, max_iterations=50, beta_every=6, beta_max=8.0)
+    return p, opts, Limits(stress=80.0, deflection=5.0, modulus=1000.0, radius=radius, bond=bond)
+
+
+@pytest.mark.parametrize("radius,bond", [(None, None), (1.3, None), (2.2, (0.6, (0.0, 0.0, 1.0), 0.8))])
+def test_constraint_gradients_against_central_differences(radius, bond):
+    """Deflection: d(defl / limit)/dx; stress: kappa dpm/dx (kappa is held
+    as the loop holds it), each through filter, projection, erosion and the
+    solve, along random directions on the design cells."""
+    p, opts, lim = _setup(radius, bond)
+    pres = np.zeros(p.dims, bool)
+    pres[-1, :, :2] = True
+    L = Constrained(p, opts, lim, preserve=pres.ravel())
+    rng = np.random.default_rng(2)
+    x = rng.uniform(0.3, 0.9, L.n)
+    x[L.pres] = 1.0
+    beta = 2.0
+    cons, grads, rec, kappa, _, _ = L.constraints(x, beta, [1.0], 1)
+    h = 1e-6
+    for _ in range(2):
+        d = rng.standard_normal(L.n) * L.design
+        cp, _, rp, _, _, _ = L.constraints(x + h * d, beta, [1.0], 1)
+        cm, _, rm, _, _, _ = L.constraints(x - h * d, beta, [1.0], 1)
+        fd_defl = (cp[0] - cm[0]) / (2 * h)
+        assert grads[0] @ d == pytest.approx(fd_defl, rel=1e-6)
+        fd_pm = (rp["pm.0"] - rm["pm.0"]) / (2 * h)
+        assert grads[1] @ d == pytest.approx(kappa[0] * fd_pm, rel=1e-5)
+
+
+def test_loop_invariants():
+    """Beta only rises, the move stays within [0.02, 0.1], the delivered
+    design is the lightest feasible one at the sharpest projection and
+    meets every limit to the loop's slack."""
+    p, opts, lim = _setup()
+    r = optimize_constrained(p, opts, lim)
+    h = r["history"]
+    betas = [q["beta"] for q in h]
+    assert all(b1 >= b0 for b0, b1 in zip(betas, betas[1:]))
+    assert all(0.02 - 1e-15 <= q["move"] <= 0.1 + 1e-15 for q in h)
+    d = r["delivered"]
+    assert d is not None and max(d["constraints"]) <= 0.02
+    sharp = [q for q in h if q["beta"] >= opts.beta_max and max(q["constraints"]) <= 0.02]
+    assert d["volume"] == min(q["volume"] for q in sharp)
+    assert 0.0 < d["volume"] < 1.0
