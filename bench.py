@@ -213,6 +213,77 @@ def run_loop_bench(steps, warmup, nx, ny, nz, log=None):
     }
 
 
+def run_constrained_bench(steps, warmup, no_cpu=False, log=None):
+    """The constrained minimum-mass loop (constrained._constrained_run on a
+    box: MMA, deflection and stress limits, beta continuation) on the
+    48x24x24 cantilever: iterations/s with the corner-sampled stress and
+    with the material-averaged stress, each from the same start, timed from
+    the loop's own per-iteration clock after the warmup iterations; the
+    oracle's iteration (direct solves) timed on one iteration beside it."""
+    import numpy as np
+
+    from loadpath_amd.constrained import Limits, optimize_constrained
+    from loadpath_amd.problems import cantilever_48x24x24
+    from loadpath_amd.simp import SimpOptions
+
+    p = cantilever_48x24x24()
+    pres = np.zeros(p.dims, bool)
+    pres[-2:, :2, 10:15] = True          # a pad under the load
+    kw = dict(volume_fraction=0.3, filter_radius=1.5, max_iterations=warmup + steps, beta_every=10, beta_max=8.0)
+    limits = dict(stress=1.5, deflection=0.2, modulus=1000.0)
+    out = {}
+    for name, radius in (("corner", None), ("averaged", 1.3)):
+        r = optimize_constrained(p, 0.3, SimpOptions(solve_rtol=1e-6, **kw), Limits(radius=radius, **limits),
+                                 preserve=pres, precision="mixed", max_levels=4, coarse_cells=0)
+        h = r.history
+        t_w = h[warmup - 1]["seconds"] if warmup > 0 else 0.0
+        dt = h[warmup + steps - 1]["seconds"] - t_w
+        if log:
+            for q in h:
+                log(f"  constrained {name}: {q}")
+        out[name] = dict(value=steps / dt, hist=h[warmup:warmup + steps])
+    cpu = None
+    if not no_cpu:
+        from oracle.constrained import Constrained, Limits as OLimits
+        from oracle.loop import LoopOptions
+        from oracle.simp import Problem
+
+        # a half-scale cantilever (24x12x12, the same load and pad): the
+        # oracle's direct solve of the full grid takes minutes per iteration
+        from oracle.simp import nodes as onodes
+
+        op = Problem("cantilever-24x12x12", (24, 12, 12), p.volfrac, p.rmin,
+                     [(onodes(0, range(13), range(13)), "xyz")], [[(onodes(24, 0, 6), "z", -1.0)]])
+        hp = np.zeros((24, 12, 12), bool)
+        hp[-1:, :1, 5:8] = True
+        L = Constrained(op, LoopOptions(**kw), OLimits(**limits), preserve=hp.ravel())
+        x = np.full(L.n, 0.3)
+        x[L.pres] = 1.0
+        reps, t0 = 0, time.perf_counter()
+        while reps < 10 and (reps == 0 or time.perf_counter() - t0 < 10.0):
+            L.constraints(x, 1.0, [1.0], 1)
+            reps += 1
+        dt = (time.perf_counter() - t0) / reps
+        cpu = {"value": round(1.0 / dt, 5), "unit": "constrained iter/s", "cores": 1, "kind": "port",
+               "sample": "one iteration of the oracle's loop (filter, projections, direct solve of the load case "
+                         "and the adjoint, deflection and corner-stress partials) at the first iterate, on the "
+                         "half-scale 24x12x12 cantilever (3456 cells; the full grid's direct solve takes minutes)"}
+    c, a = out["corner"], out["averaged"]
+    return {
+        "metric": "constrained min-mass iter/s at 48x24x24 hex8 (MMA, deflection + corner-stress limits)",
+        "value": round(c["value"], 4), "unit": "constrained iter/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": round(1e3 / c["value"], 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 CG / f32 V-cycle / f16 MFMA fine operator, f64 stress and MMA",
+        "data": "synthetic (tip load on a clamped cantilever)",
+        "config": {"workload": "cantilever_48x24x24, preserved pad under the load, stress limit 1.5, deflection 0.2, "
+                               "modulus 1000, beta 1 -> 8, solve rtol 1e-6", "cells": 48 * 24 * 24},
+        "averaged_value": round(a["value"], 4),
+        "constraints_last": c["hist"][-1]["constraints"] if c["hist"] else None,
+        "cg_iters_per_step": [q.get("cgIterations") for q in c["hist"]],
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +298,9 @@ def main():
                          "the same studies one after another on the plain FP64 path")
     ap.add_argument("--loop", action="store_true",
                     help="instead: the production loop (simp.optimize's) on the same workload, mixed against FP64")
+    ap.add_argument("--constrained", action="store_true",
+                    help="instead: the constrained minimum-mass loop (MMA with stress and deflection limits) on the "
+                         "48x24x24 cantilever")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB elementwise copy (3 times): the known-size kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiled run")
@@ -238,6 +312,9 @@ def main():
     log = (lambda m: print(m, file=sys.stderr, flush=True)) if a.verbose else None
     if a.studies:
         print(json.dumps(run_studies_bench(log, loop=a.loop)))
+        return
+    if a.constrained:
+        print(json.dumps(run_constrained_bench(a.steps, a.warmup, a.no_cpu, log)))
         return
     nx, ny, nz = a.size or (WORKLOAD["nx"], WORKLOAD["ny"], WORKLOAD["nz"])
     if a.loop:
