@@ -289,6 +289,26 @@ int lp_stress_averaged(int nx, int ny, int nz, double pitch, double nu, int elem
                        double limit, const double* bond, double* kappa, int it, double* pm, double* smax,
                        double* rhs, double* gx);
 
+/* ---- the assembled route: fem.Stiffness (fem.py:377-526) ---------------- */
+/* The CSR stiffness of the voxel mesh of the present cells of an nx x ny x
+ * nz box, its layout the reference's (rows 3n+i over the mesh nodes in C
+ * order, for each neighbour in fem.OFFSETS order the columns 3m+j; Dirichlet
+ * rows and columns zeroed, fixed diagonal 1), built once on the host.
+ * present (cells) u8; fixed ((nx+1)(ny+1)(nz+1), 3) u8 node-major; ke the
+ * 24x24 element matrix at E = 1, pitch included. Out: n_dof, nnz. */
+typedef struct lp_stiff lp_stiff;
+int lp_stiffness_create(lp_stiff** out, int nx, int ny, int nz, const uint8_t* present, const uint8_t* fixed,
+                        const double* ke, long long* n_dof, long long* nnz);
+/* indptr (n_dof + 1) and indices (nnz), int64, host arrays. */
+int lp_stiffness_pattern(lp_stiff* s, long long* indptr, long long* indices);
+/* femops.scatter_gather (femops.py:99-278) and Stiffness.assemble: data (nnz)
+ * for per-cell moduli scale (box cells, C order), each value the sum of
+ * scale_c Ke blocks in the reference's order with separate roundings (equal
+ * to the reference's to the bit). device_ptr = 1: scale and data are device
+ * pointers. */
+int lp_stiffness_assemble(lp_stiff* s, const double* scale, double* data, int device_ptr);
+int lp_stiffness_destroy(lp_stiff* s);
+
 /* 26-connected components of a cell mask, scipy.ndimage.label(mask,
  * structure=ones((3,3,3))) as simp._Analysis._connected and _prune_dead use
  * it (simp.py:264-282, 1276-1282): labels (nx, ny, nz) int32, 0 off the

@@ -117,6 +117,11 @@ _SIGS = {
     "lp_label26": (_I, [_I, _I, _I, _P, _P, _P]),
     "lp_set_solve_tol": (_I, [_P, _D, _I]),
     "lp_material_average": (_I, [_I, _I, _I, _D, _D, _P, _P, _P]),
+    "lp_stiffness_create": (_I, [C.POINTER(_P), _I, _I, _I, _P, _P, _P, C.POINTER(C.c_longlong),
+                                 C.POINTER(C.c_longlong)]),
+    "lp_stiffness_pattern": (_I, [_P, _P, _P]),
+    "lp_stiffness_assemble": (_I, [_P, _P, _P, _I]),
+    "lp_stiffness_destroy": (_I, [_P]),
     "lp_stress_averaged": (_I, [_I, _I, _I, _D, _D, _I, _D, _I, _P, _P, _P, _D, _D, _D, _P, _P, _I, _P, _P,
                                 _P, _P]),
     "lp_project": (_I, [C.c_longlong, _P, _D, _D, _P, _P, _I]),

@@ -5,6 +5,9 @@
 - ``hex8_element()``: the LOADPATH_HEX8 switch (fem.py:151-160).
 - ``HexMesh``: fem.HexMesh (fem.py:317-374): node ids ascend in C order over
   the node grid, ``edof = 3 * node + component``.
+- ``Stiffness``: fem.Stiffness (fem.py:377-526), the assembled route's CSR
+  stiffness: its layout built once by liblpmi's host code, ``assemble(scale)``
+  on the GPU (lp_stiffness_*).
 """
 from __future__ import annotations
 
@@ -85,3 +88,76 @@ class HexMesh:
 def box_mesh(nx: int, ny: int, nz: int, pitch: float = 1.0) -> HexMesh:
     ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
     return HexMesh(np.stack([ii.ravel(), jj.ravel(), kk.ravel()], axis=1), pitch)
+
+
+class Stiffness:
+    """fem.Stiffness(mesh, nu, fixed, element) (fem.py:377-526): the global
+    stiffness of a voxel mesh in a fixed CSR layout, Dirichlet conditions as
+    identity rows. ``assemble(scale)`` (per-element moduli, mesh order)
+    returns a scipy CSR matrix whose values are assembled on the GPU, equal
+    to the reference's to the bit; ``indptr``, ``indices``, ``nnz``,
+    ``ke`` (pitch included), ``fixed``, ``free`` as the reference's."""
+
+    def __init__(self, mesh, nu, fixed, element: str | None = None):
+        _lib.require_gpu()
+        self.mesh = mesh
+        self.nu = nu
+        self.element = element or hex8_element()
+        self.ke = hex8_ke(nu, self.element) * mesh.pitch
+        fixed = np.asarray(fixed, bool)
+        if fixed.shape != (mesh.n_dof,):
+            raise ValueError("fixed must be a bool per dof")
+        self.fixed = fixed
+        self.free = ~fixed
+        lo = mesh.elem_ijk.min(axis=0)
+        dims = tuple(int(d) for d in mesh.elem_ijk.max(axis=0) - lo + 1)
+        self._dims = dims
+        e = mesh.elem_ijk - lo
+        self._cells = np.ravel_multi_index((e[:, 0], e[:, 1], e[:, 2]), dims)
+        present = np.zeros(int(np.prod(dims)), np.uint8)
+        present[self._cells] = 1
+        nd = mesh.node_ijk - lo
+        nodes = np.ravel_multi_index((nd[:, 0], nd[:, 1], nd[:, 2]), tuple(d + 1 for d in dims))
+        if np.any(np.diff(nodes) <= 0):
+            raise ValueError("mesh node ids must ascend in C order over the node grid")
+        fg = np.zeros((int(np.prod([d + 1 for d in dims])), 3), np.uint8)
+        fg[nodes] = fixed.reshape(-1, 3)
+        h = C.c_void_p()
+        n_dof, nnz = C.c_longlong(), C.c_longlong()
+        _lib.check(_lib.lib().lp_stiffness_create(C.byref(h), *dims, _lib.ptr(present), _lib.ptr(fg),
+                                                  _lib.ptr(np.ascontiguousarray(self.ke)), C.byref(n_dof),
+                                                  C.byref(nnz)), "stiffness_create")
+        self._h = h
+        if n_dof.value != mesh.n_dof:
+            raise ValueError("the mesh's nodes are not the nodes its cells use")
+        self.nnz = int(nnz.value)
+        self.indptr = np.empty(mesh.n_dof + 1, np.int64)
+        self.indices = np.empty(self.nnz, np.int64)
+        _lib.check(_lib.lib().lp_stiffness_pattern(h, _lib.ptr(self.indptr), _lib.ptr(self.indices)),
+                   "stiffness_pattern")
+        self._scale = np.zeros(int(np.prod(dims)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().lp_stiffness_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def assemble_data(self, scale) -> np.ndarray:
+        """The CSR values for per-element moduli scale (mesh order)."""
+        self._scale[self._cells] = np.asarray(scale, float)
+        data = np.empty(self.nnz)
+        _lib.check(_lib.lib().lp_stiffness_assemble(self._h, _lib.ptr(self._scale), _lib.ptr(data), 0),
+                   "stiffness_assemble")
+        return data
+
+    def assemble(self, scale):
+        import scipy.sparse as sp
+
+        k = sp.csr_matrix((self.assemble_data(scale), self.indices, self.indptr),
+                          shape=(self.mesh.n_dof, self.mesh.n_dof))
+        k.has_sorted_indices = True
+        return k

@@ -197,3 +197,53 @@ def von_mises_matrix() -> np.ndarray:
         v[i, i] = 1.0
         v[3 + i, 3 + i] = 3.0
     return v
+
+
+def stiffness_csr(mesh: Mesh, ke: np.ndarray, scale, fixed):
+    """fem.Stiffness(mesh, nu, fixed).assemble(scale) (fem.py:377-526), as its
+    numpy path builds it: (data, indices, indptr). Rows 3n+i; for each of
+    the node's neighbours (27 offsets, dx outermost, kept where a cell joins
+    the two) the columns 3m+j; values scale_e Ke[a, b] added pair by pair
+    (a, then b) over the elements; Dirichlet rows and columns zeroed and a
+    fixed diagonal set to 1."""
+    enode = mesh.enode
+    nn = mesh.n_nodes
+    fixed = np.asarray(fixed, bool)
+    corners = CORNERS
+
+    def off_index(d):
+        return (d[..., 0] + 1) * 9 + (d[..., 1] + 1) * 3 + (d[..., 2] + 1)
+
+    adj = np.zeros((nn, 27), bool)
+    pair_off = np.empty((8, 8), np.int64)
+    for a in range(8):
+        for b in range(8):
+            pair_off[a, b] = off_index(corners[b] - corners[a])
+            adj[enode[:, a], pair_off[a, b]] = True
+    cnt = adj.sum(axis=1)
+    rank = np.cumsum(adj, axis=1) - 1
+    row_len = np.repeat(3 * cnt, 3)
+    indptr = np.concatenate([[0], np.cumsum(row_len)]).astype(np.int64)
+    nnz = int(indptr[-1])
+    indices = np.empty(nnz, np.int64)
+    offsets = np.array([[dx, dy, dz] for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1)])
+    lookup = {tuple(ijk): i for i, ijk in enumerate(mesh.node_ijk)}
+    n_idx, o_idx = np.nonzero(adj)
+    nbr = np.array([lookup[tuple(v)] for v in mesh.node_ijk[n_idx] + offsets[o_idx]], np.int64)
+    for i in range(3):
+        base = indptr[3 * n_idx + i] + 3 * rank[n_idx, o_idx]
+        for j in range(3):
+            indices[base + j] = 3 * nbr + j
+    data = np.zeros(nnz)
+    s = np.asarray(scale, float)[:, None]
+    for a in range(8):
+        na = enode[:, a]
+        row = 3 * cnt[na]
+        for b in range(8):
+            base = indptr[3 * na] + 3 * rank[na, pair_off[a, b]]
+            slots = base[:, None] + np.repeat(np.arange(3), 3)[None, :] * row[:, None] + np.tile(np.arange(3), 3)[None]
+            data[slots] += s * ke[3 * a:3 * a + 3, 3 * b:3 * b + 3].ravel()[None]
+    row_of = np.repeat(np.arange(mesh.n_dof), row_len)
+    data *= ~(fixed[row_of] | fixed[indices])
+    data[np.nonzero((row_of == indices) & fixed[row_of])[0]] = 1.0
+    return data, indices, indptr
