@@ -234,7 +234,7 @@ def run_constrained_bench(steps, warmup, no_cpu=False, log=None):
     out = {}
     for name, radius in (("corner", None), ("averaged", 1.3)):
         r = optimize_constrained(p, 0.3, SimpOptions(solve_rtol=1e-6, **kw), Limits(radius=radius, **limits),
-                                 preserve=pres, precision="mixed", max_levels=4, coarse_cells=0)
+                                 preserve=pres, precision="mixed", autotune=True, max_levels=4, coarse_cells=0)
         h = r.history
         t_w = h[warmup - 1]["seconds"] if warmup > 0 else 0.0
         dt = h[warmup + steps - 1]["seconds"] - t_w

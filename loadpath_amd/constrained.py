@@ -60,7 +60,8 @@ class Limits:
 class _Run:
     """One constrained study's device state (grid, filter, aggregates)."""
 
-    def __init__(self, problem, nu, opts: SimpOptions, limits: Limits, preserve, precision, element, grid_kw):
+    def __init__(self, problem, nu, opts: SimpOptions, limits: Limits, preserve, precision, element, grid_kw,
+                 autotune=False):
         from .mg import GridElasticity
 
         if getattr(problem, "present", None) is not None:
@@ -108,6 +109,7 @@ class _Run:
             else:
                 self.agg = StressPNorm(self.grid, "corners", limit=limits.stress, counted=~contact)
         self.m = len(self.defl_cases) + len(self.stress_cases)
+        self.autotune = autotune
 
     def designs(self, x, beta):
         xt = (self.h @ x) / self.hs
@@ -127,6 +129,11 @@ class _Run:
         de = PENALTY * xe ** (PENALTY - 1) * (1 - E_MIN)
         g = self.grid
         g.set_moduli(e)
+        if self.autotune:
+            # the fine operator's x-chunk for this grid, timed once (a small
+            # grid wants more, shorter chunks than the default)
+            g.autotune(len(self.loads))
+            self.autotune = False
         us = g.solve_many(self.loads)
         lim = self.limits
 
@@ -172,7 +179,7 @@ class _Run:
 
 def optimize_constrained(problem, nu: float, opts: SimpOptions, limits: Limits, *, preserve=None,
                          precision: str = "fp64", element: str = "im", x0=None, on_iter=None,
-                         **grid_kw) -> SimpResult:
+                         autotune: bool = False, **grid_kw) -> SimpResult:
     """constrained.optimize_constrained(domain, nu, opts, limits, on_iter)
     (constrained.py:276-854) for a box problem (a BoxProblem with no present
     mask): the lightest design within the stress and deflection limits.
@@ -180,10 +187,11 @@ def optimize_constrained(problem, nu: float, opts: SimpOptions, limits: Limits, 
     records carry iteration, volume, beta, change, constraints, seconds,
     cgIterations, move and the per-case readings; ``.delivered`` is the
     record of the design delivered (None when no design at beta_max was
-    inside the limits)."""
+    inside the limits). autotune: time the fine operator's launch shape for
+    the grid once, at the first solve (a speed setting only)."""
     _lib.require_gpu()
     t0 = time.time()
-    R = _Run(problem, nu, opts, limits, preserve, precision, element, grid_kw)
+    R = _Run(problem, nu, opts, limits, preserve, precision, element, grid_kw, autotune)
     n, design, free = R.n, R.design, R.free
     x = np.full(n, opts.volume_fraction) if x0 is None else np.clip(np.asarray(x0, float).ravel(), 0.0, 1.0)
     x[R.pres] = 1.0
