@@ -356,7 +356,9 @@ int lp_mma_get(lp_mma* mma, int what, double* out);
 /* The autotuner (no reference counterpart): times the fine-level operator
  * of the V-cycle (FP16 MFMA on the mixed path, the vector kernel otherwise)
  * on nrhs right-hand sides for each x-chunk in {4, 8, 12, 16, 24, 32}, keeps the
- * fastest and returns it with its time per apply. */
+ * fastest and returns it with its time per apply; on the mixed path it then
+ * times the weights level's operator for 2, 4, 8 and 16 node planes per
+ * block and keeps the fastest (lp_tune param 8). */
 int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best);
 
 /* Launch-shape tuning (no reference counterpart: the autotuner's knob).
@@ -377,7 +379,8 @@ int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best);
  * kernel (node sums formed in registers across lanes) rather than the
  * block-tile kernel (node sums through LDS): the fine level (1, the
  * default; the faster choice on each level), both levels (2) or neither
- * (0). */
+ * (0). Param 8: node planes per block of the weights level's FP16-MFMA
+ * kernel (1-64; 8 by default); value 0 returns the current one unchanged. */
 int lp_tune(lp_grid* g, int param, int value);
 
 #ifdef __cplusplus

@@ -50,7 +50,7 @@ constexpr int PSTR = 292;           // component stride of a staged plane (float
 constexpr int NCELL = TC * TC;      // 256 cells = 8 tiles of 32
 constexpr int FSTR = 264;           // row stride of the product buffer (= 8 mod 16)
 constexpr int kThreads = 256;
-constexpr int kWeightsXChunk = 8;   // node planes per block on the weights level
+constexpr int kWeightsXChunk = 8;   // node planes per block on the weights level (default)
 static_assert(NCELL == 8 * 32, "cells per layer must fill the MFMA tiles");
 static_assert(2 * kThreads >= NSTAGE, "two staging entries per thread");
 
@@ -761,7 +761,8 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
 
 AGeom mfma_geom(const Grid& Gd, int level) {
     AGeom G = make_apply_geom(Gd, level);
-    if (Gd.lv[level].kind == LK_WEIGHTS) G.xchunk = kWeightsXChunk;
+    if (Gd.lv[level].kind == LK_WEIGHTS)
+        G.xchunk = Gd.tune_weights_xchunk > 0 ? Gd.tune_weights_xchunk : kWeightsXChunk;
     const int nty = (G.Ny + TN - 1) / TN, ntz = (G.Nz + TN - 1) / TN;
     G.ntz = ntz;
     G.nyzb = nty * ntz;
@@ -809,7 +810,8 @@ void launch_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int n
 // wave-tile kernel: 15 x 15 node tiles per wave, kWaveTiles waves per block
 AGeom wave_geom(const Grid& Gd, int level) {
     AGeom G = make_apply_geom(Gd, level);
-    if (Gd.lv[level].kind == LK_WEIGHTS) G.xchunk = kWeightsXChunk;
+    if (Gd.lv[level].kind == LK_WEIGHTS)
+        G.xchunk = Gd.tune_weights_xchunk > 0 ? Gd.tune_weights_xchunk : kWeightsXChunk;
     const int nty = (G.Ny + WN - 1) / WN, ntz = (G.Nz + WN - 1) / WN;
     G.ntz = ntz;
     G.nyzb = nty * ntz;

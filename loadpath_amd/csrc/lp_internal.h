@@ -150,6 +150,7 @@ struct Grid {
     double ke_orbit[32] = {};
     int tune_fine_ny = 1;       // fine-level nodes per thread along y (1 or 2)
     int tune_fine_xchunk = 16;  // fine-level nodes per thread along x
+    int tune_weights_xchunk = 8; // weights-level node planes per block (FP16-MFMA kernels)
     int tune_fused_cheb = 1;    // MFMA levels: Chebyshev vector passes fused into the operator
     int tune_graph = 1;         // PCG replays the V-cycle from a captured hipGraph
     int tune_wave = 1;          // FP16-MFMA levels: wave-tile kernel on the fine level (1), on both (2), on none (0)

@@ -925,6 +925,14 @@ int lp_tune(lp_grid* g, int param, int value) {
         ++g->epoch;
         return old;
     }
+    if (param == 8) {
+        if (value < 0 || value > 64) return LP_ERR_ARG;
+        const int old = g->tune_weights_xchunk;
+        if (value == 0) return old;
+        g->tune_weights_xchunk = value;
+        ++g->epoch;
+        return old;
+    }
     return LP_ERR_ARG;
 }
 
@@ -951,6 +959,27 @@ int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best) {
         ++G.epoch;
         if (xchunk) *xchunk = best;
         if (ms_best) *ms_best = tbest;
+        // the weights level's FP16-MFMA kernel: node planes per block (a
+        // coarse grid wants short chunks, or it runs a handful of blocks)
+        if (G.nlevels > 1 && G.lv[1].kind == LK_WEIGHTS && eb == 4 && G.fp16()) {
+            const int wcand[] = {2, 4, 8, 16};
+            int wbest = G.tune_weights_xchunk;
+            double wt = 1e300;
+            for (int c : wcand) {
+                if (c > G.lv[1].g.Nx) continue;
+                G.tune_weights_xchunk = c;
+                ++G.epoch;
+                double ms = 0.0;
+                const int rc = lp_time_apply(g, 1, eb, nrhs, reps, &ms);
+                if (rc != LP_OK) return rc;
+                if (ms < wt) {
+                    wt = ms;
+                    wbest = c;
+                }
+            }
+            G.tune_weights_xchunk = wbest;
+            ++G.epoch;
+        }
         return LP_OK;
     });
 }

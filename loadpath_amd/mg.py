@@ -339,12 +339,14 @@ class GridElasticity:
         key = (self.cells, self.precision, nrhs)
         lib = _lib.lib()
         if key in GridElasticity._TUNED:
-            lib.lp_tune(self._h, 1, GridElasticity._TUNED[key])
-            return GridElasticity._TUNED[key]
+            fine, weights = GridElasticity._TUNED[key]
+            lib.lp_tune(self._h, 1, fine)
+            lib.lp_tune(self._h, 8, weights)
+            return fine
         xc = C.c_int()
         ms = C.c_double()
         _lib.check(lib.lp_autotune(self._h, nrhs, int(reps), C.byref(xc), C.byref(ms)), "autotune")
-        GridElasticity._TUNED[key] = xc.value
+        GridElasticity._TUNED[key] = (xc.value, int(lib.lp_tune(self._h, 8, 0)))
         return xc.value
 
     def time_apply(self, level=0, dtype=np.float64, nrhs=1, reps=20) -> float:
