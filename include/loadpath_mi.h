@@ -128,10 +128,12 @@ int lp_element_product(lp_grid* g, int nrhs, const double* a, const double* b, d
 /* p-mean von Mises stress aggregate, constrained.py:607-664 (the
  * non-averaged branch), one load case per rhs on the fine grid:
  *   sigma_k = cb_k u_c (cb from lp_hex8_cb / pitch, npts 1 or 8),
- *   vm = sqrt(max(sigma^T V sigma, 1e-30)), s = x_c^q vm / lim (0 where
- *   counted == 0), pm = sc (sum (s / sc)^p / nc)^(1/p), sc = max(s_max, 1e-30).
- * u (nrhs, 3, nodes), x (nx, ny, nz) the material share, counted (nx, ny, nz)
- * uint8 or NULL (every cell), nc the number of counted samples. Outputs pm
+ *   vm = sqrt(max(sigma^T V sigma, 1e-30)), s = x_c^q vm / lim (0 at a
+ *   point not counted), pm = sc (sum (s / sc)^p / nc)^(1/p), sc = max(s_max,
+ *   1e-30). u (nrhs, 3, nodes), x (nx, ny, nz) the material share, counted
+ * (nx, ny, nz) uint8 with bit k = sample point k counted (0xff: the whole
+ * cell; per point for the contact band, constrained.py:349-358), or NULL
+ * (every point), nc the number of counted samples. Outputs pm
  * and s_max (host, nrhs); optional rhs (nrhs, 3, nodes) = d(kappa pm)/du,
  * zero on fixed dofs (the adjoint right-hand side), and gx (nrhs, nx, ny,
  * nz) = the explicit d(kappa pm)/dx at fixed u. The full gradient is

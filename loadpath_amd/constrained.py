@@ -47,7 +47,8 @@ class Limits:
     peak von Mises (MPa), deflection at the loads (mm), modulus (MPa) to
     turn unit-E displacements into mm, bond (ratio, layer normal, k) of a
     part built in layers, radius the material length of the averaged
-    reading. buckling and contact_band are not supported here."""
+    reading, contact_band the distance from a loaded or supported node
+    within which a corner reads no stress. buckling is not supported here."""
     stress: float | None
     deflection: float | None
     modulus: float
@@ -66,8 +67,8 @@ class _Run:
 
         if getattr(problem, "present", None) is not None:
             raise NotImplementedError("the constrained loop here runs on a full box")
-        if limits.buckling is not None or limits.contact_band:
-            raise NotImplementedError("buckling limits and the contact band are not supported")
+        if limits.buckling is not None:
+            raise NotImplementedError("buckling limits are not supported")
         if limits.bond is not None and not limits.radius:
             raise NotImplementedError("the layer bond is read on the averaged stress (limits.radius)")
         self.opts, self.limits = opts, limits
@@ -99,15 +100,31 @@ class _Run:
         for f in self.loads:
             loaded |= f.reshape(-1, 3).any(axis=1)
         contact = (support | loaded)[mesh.enode].any(axis=1)
+        # with a contact band: every corner within that distance of a loaded
+        # or supported node reads no stress (constrained.py:349-358)
+        pt_counted = None
+        if limits.contact_band and (support | loaded).any():
+            from scipy.spatial import cKDTree
+
+            xyz = mesh.node_ijk * mesh.pitch
+            near = cKDTree(xyz[support | loaded]).query(xyz, k=1, distance_upper_bound=float(limits.contact_band))[0] \
+                < np.inf
+            pt_counted = ~near[mesh.enode]
         self.agg = self.avg = None
         if self.stress_cases:
             if limits.radius:
-                readable = np.ones(mesh.n_nodes, bool)
-                readable[mesh.enode[contact].ravel()] = False
+                if pt_counted is not None:
+                    readable = np.zeros(mesh.n_nodes, bool)
+                    readable[mesh.enode[pt_counted]] = True
+                    readable[mesh.enode[~pt_counted]] = False
+                else:
+                    readable = np.ones(mesh.n_nodes, bool)
+                    readable[mesh.enode[contact].ravel()] = False
                 self.avg = StressAveraged(mesh, nu, float(limits.radius), readable, limit=limits.stress,
                                           bond=limits.bond, element=element, fixed=np.flatnonzero(fixed))
             else:
-                self.agg = StressPNorm(self.grid, "corners", limit=limits.stress, counted=~contact)
+                self.agg = StressPNorm(self.grid, "corners", limit=limits.stress,
+                                       counted=None if pt_counted is not None else ~contact, point_counted=pt_counted)
         self.m = len(self.defl_cases) + len(self.stress_cases)
         self.autotune = autotune
 

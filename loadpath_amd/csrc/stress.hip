@@ -31,7 +31,7 @@ struct PnArgs {
     const double* u;         // (nrhs, 3, Nx, Ny, Nz)
     size_t ustride;
     const double* x;         // (nx, ny, nz)
-    const uint8_t* counted;  // (nx, ny, nz) or null = every cell
+    const uint8_t* counted;  // (nx, ny, nz) bit k = sample point k counted, or null = every point
     const double* cb;        // (npts, 6, 24)
     int npts;
     double p, q, lim, kappa, nc;
@@ -96,11 +96,11 @@ __global__ void __launch_bounds__(kBlock) k_pnorm_scan(PnArgs A, Geom g, const d
         double ue[24];
         gather_cell(A.u + rhs * A.ustride, g, cell, ue);
         const double xc = A.x[cell];
-        const bool cnt = A.counted == nullptr || A.counted[cell] != 0;
+        const unsigned cbits = A.counted == nullptr ? 0xffu : A.counted[cell];
         for (int k = 0; k < A.npts; ++k) {
             double sv[6];
             const double vm = point_stress(A.cb + k * 144, ue, sv);
-            const double sr = srel_of(A, xc, vm, cnt);
+            const double sr = srel_of(A, xc, vm, (cbits >> k) & 1u);
             if (MODE == 0)
                 acc = fmax(acc, sr);
             else
@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(kBlock) k_pnorm_cell(PnArgs A, Geom g, int rhs
     double ue[24];
     gather_cell(A.u + rhs * A.ustride, g, cell, ue);
     const double xc = A.x[cell];
-    const bool cnt = A.counted == nullptr || A.counted[cell] != 0;
+    const unsigned cbits = A.counted == nullptr ? 0xffu : A.counted[cell];
     double r[24];
 #pragma unroll
     for (int j = 0; j < 24; ++j) r[j] = 0.0;
@@ -136,6 +136,7 @@ __global__ void __launch_bounds__(kBlock) k_pnorm_cell(PnArgs A, Geom g, int rhs
         const double* cbk = A.cb + k * 144;
         double sv[6];
         const double vm = point_stress(cbk, ue, sv);
+        const bool cnt = (cbits >> k) & 1u;
         const double sr = srel_of(A, xc, vm, cnt);
         const double dg = A.kappa * pow(sr / fmax(pm, 1e-30), A.p - 1.0) / A.nc;
         ex += dg * (A.q * pow(fmax(xc, 1e-12), A.q - 1.0) * vm / A.lim);

@@ -32,6 +32,9 @@ import numpy as np
 
 from . import _lib
 
+# the node (fem.CORNERS order) each corner sample sits on, samples x outermost
+# (constrained._CORNER_OF, constrained.py:106)
+CORNER_OF = (0, 4, 2, 6, 1, 5, 3, 7)
 P_DEFAULT = 8.0     # constrained._P
 Q_DEFAULT = 1.0     # constrained._Q
 _POINTS = {"center": 0, "corners": 1}
@@ -53,17 +56,32 @@ class StressPNorm:
     """The p-mean stress aggregate on a GridElasticity (constrained.py:607-664)."""
 
     def __init__(self, grid, points: str = "center", p: float = P_DEFAULT, q: float = Q_DEFAULT,
-                 limit: float = 1.0, counted=None):
+                 limit: float = 1.0, counted=None, point_counted=None):
         self.grid = grid
         self.points = points
         self.p, self.q, self.limit = float(p), float(q), float(limit)
         self.cb = np.ascontiguousarray(hex8_cb(grid.nu, points, grid.element, grid.mesh.pitch))
         ne = len(grid.elem_ijk)
-        cnt = np.ones(ne, bool) if counted is None else np.asarray(counted, bool)
-        if cnt.shape != (ne,):
-            raise ValueError("counted must be (n_elem,) bool")
-        self._counted = np.ascontiguousarray(grid.cell_grid(cnt.astype(float)) > 0.5).astype(np.uint8)
-        self.nc = float(max(int(cnt.sum()), 1) * len(self.cb))
+        npts = len(self.cb)
+        if point_counted is not None:
+            # per corner, fem.CORNERS order (constrained.py:349-358): bit k
+            # of a cell's mask is its k-th sample point (x outermost)
+            if points != "corners":
+                raise ValueError("point_counted needs the corner samples")
+            pc = np.asarray(point_counted, bool)
+            if pc.shape != (ne, 8):
+                raise ValueError("point_counted must be (n_elem, 8) bool")
+            bits = np.zeros(ne, np.int64)
+            for k, corner in enumerate(CORNER_OF):
+                bits |= pc[:, corner].astype(np.int64) << k
+            self.nc = float(max(int(pc.sum()), 1))
+        else:
+            cnt = np.ones(ne, bool) if counted is None else np.asarray(counted, bool)
+            if cnt.shape != (ne,):
+                raise ValueError("counted must be (n_elem,) bool")
+            bits = np.where(cnt, (1 << npts) - 1, 0)
+            self.nc = float(max(int(cnt.sum()), 1) * npts)
+        self._counted = np.ascontiguousarray(grid.cell_grid(bits.astype(float))).astype(np.uint8)
 
     def evaluate(self, us, x, kappa: float = 1.0, need_grad: bool = True) -> dict:
         us = [np.asarray(u, float) for u in us]

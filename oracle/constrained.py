@@ -56,6 +56,10 @@ class Limits:
     modulus: float
     bond: tuple | None = None
     radius: float | None = None
+    contact_band: float | None = None
+
+
+CORNER_OF = (0, 4, 2, 6, 1, 5, 3, 7)   # constrained._CORNER_OF (constrained.py:106)
 
 
 def corner_cbs(nu, element="im", pitch=1.0):
@@ -93,10 +97,26 @@ class Constrained:
             loaded |= f.reshape(-1, 3).any(axis=1)
         contact = (support | loaded)[mesh.enode].any(axis=1)
         self.counted = ~contact
+        # with a contact band: every corner within that distance of a loaded
+        # or supported node reads no stress (constrained.py:349-358)
+        self.pt_counted = None
+        if limits.contact_band and (support | loaded).any():
+            from scipy.spatial import cKDTree
+
+            xyz = mesh.node_ijk * mesh.pitch
+            near = cKDTree(xyz[support | loaded]).query(xyz, k=1, distance_upper_bound=float(limits.contact_band))[0] \
+                < np.inf
+            self.pt_counted = ~near[mesh.enode]
+            self.counted = self.pt_counted.any(axis=1)
         self.readable = None
         if limits.stress is not None and limits.radius:
-            rd = np.ones(mesh.n_nodes, bool)
-            rd[mesh.enode[contact].ravel()] = False
+            if self.pt_counted is not None:
+                rd = np.zeros(mesh.n_nodes, bool)
+                rd[mesh.enode[self.pt_counted]] = True
+                rd[mesh.enode[~self.pt_counted]] = False
+            else:
+                rd = np.ones(mesh.n_nodes, bool)
+                rd[mesh.enode[contact].ravel()] = False
             self.readable = rd
         elif limits.bond is not None:
             raise NotImplementedError("the layer bond is restated for the averaged reading only")
@@ -167,15 +187,20 @@ class Constrained:
                     parts.append((c, r["g"], r["kappa"], r["smax"], r["rhs"], r["gx"]))
                     rec[f"pm.{c}"] = r["pm"]
             else:
-                nc = max(int(self.counted.sum()), 1) * len(self.cbs)
+                if self.pt_counted is None:
+                    nc = max(int(self.counted.sum()), 1) * len(self.cbs)
+                else:
+                    nc = max(int(self.pt_counted.sum()), 1)
                 act = np.flatnonzero(self.counted)
                 for c in self.stress_cases:
                     moved = us[c].reshape(-1, 3).any(axis=1)[self.mesh.enode].any(axis=1)
                     a = act[moved[act]]
                     ue = us[c][edof[a]]
-                    _, pm, smax, _ = S.pnorm_forward(ue, xe[a], self.cbs, lim=lim.stress, nc=nc)
+                    pmask = None if self.pt_counted is None else self.pt_counted[a][:, list(CORNER_OF)]
+                    _, pm, smax, _ = S.pnorm_forward(ue, xe[a], self.cbs, lim=lim.stress, nc=nc, pmask=pmask)
                     k = S.normalize_kappa(kappa[c], smax, pm, it)
-                    g, pm, smax, gx_a, rhs_e = S.pnorm_partials(ue, xe[a], self.cbs, lim=lim.stress, kappa=k, nc=nc)
+                    g, pm, smax, gx_a, rhs_e = S.pnorm_partials(ue, xe[a], self.cbs, lim=lim.stress, kappa=k, nc=nc,
+                                                                pmask=pmask)
                     gx = np.zeros(self.n)
                     gx[a] = gx_a
                     rhs = S.assemble_rhs(edof[a], rhs_e, nd, self.ev.fixed)

@@ -75,17 +75,20 @@ def sample_cb(nu: float, points: str = "center", element: str = "im", pitch: flo
     raise ValueError(points)
 
 
-def pnorm_forward(ue, x, cbs, p=P_DEFAULT, q=Q_DEFAULT, lim=1.0, kappa=1.0, nc=None):
-    """ue (ne, 24) element displacements, x (ne,) material share.
-    Returns (g, pm, smax, parts) with parts per sample point
-    (cb, sv, vm, s_rel) (constrained.py:618-641)."""
+def pnorm_forward(ue, x, cbs, p=P_DEFAULT, q=Q_DEFAULT, lim=1.0, kappa=1.0, nc=None, pmask=None):
+    """ue (ne, 24) element displacements, x (ne,) material share, pmask
+    (ne, npts) the sample points read (None: all; a point not read has zero
+    stress, constrained.py:622-625). Returns (g, pm, smax, parts) with parts
+    per sample point (cb, sv, vm, s_rel) (constrained.py:618-641)."""
     ue = np.asarray(ue, float)
     x = np.asarray(x, float)
     if nc is None:
         nc = len(x) * len(cbs)
     parts = []
-    for cb in cbs:
+    for k, cb in enumerate(cbs):
         sig = ue @ cb.T
+        if pmask is not None:
+            sig = sig * np.asarray(pmask)[:, k][:, None]
         sv = sig @ VM
         vm = np.sqrt(np.maximum(np.einsum("ij,ij->i", sv, sig), 1e-30))
         parts.append((cb, sv, vm, x ** q * vm / lim))
@@ -95,14 +98,14 @@ def pnorm_forward(ue, x, cbs, p=P_DEFAULT, q=Q_DEFAULT, lim=1.0, kappa=1.0, nc=N
     return kappa * pm, pm, smax, parts
 
 
-def pnorm_partials(ue, x, cbs, p=P_DEFAULT, q=Q_DEFAULT, lim=1.0, kappa=1.0, nc=None):
+def pnorm_partials(ue, x, cbs, p=P_DEFAULT, q=Q_DEFAULT, lim=1.0, kappa=1.0, nc=None, pmask=None):
     """(g, pm, smax, g_explicit (ne,), rhs_e (ne, 24)): the explicit
     derivative in x at fixed u and the element adjoint right-hand sides
     dg/du_e (constrained.py:643-652, dmat = 1)."""
     x = np.asarray(x, float)
     if nc is None:
         nc = len(x) * len(cbs)
-    g, pm, smax, parts = pnorm_forward(ue, x, cbs, p, q, lim, kappa, nc)
+    g, pm, smax, parts = pnorm_forward(ue, x, cbs, p, q, lim, kappa, nc, pmask)
     gx = np.zeros(len(x))
     rhs_e = np.zeros((len(x), 24))
     for cb, sv, vm, sr in parts:

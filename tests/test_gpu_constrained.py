@@ -2,7 +2,8 @@
 constrained.optimize_constrained) against the oracle's restatement of
 constrained._constrained_run (oracle/constrained.py): the same volumes,
 constraint readings, move limits and beta steps iteration by iteration,
-with the corner-sampled stress, the averaged stress and the layer bond."""
+with the corner-sampled stress, the averaged stress, the layer bond and
+the contact band (corners near the supports and loads not read)."""
 import numpy as np
 import pytest
 
@@ -21,8 +22,10 @@ def _oracle_problem(p):
     return os_.Problem(p.name, p.dims, p.volfrac, p.rmin, list(p.fixed), [list(c) for c in p.cases])
 
 
-@pytest.mark.parametrize("radius,bond", [(None, None), (1.3, None), (2.2, (0.6, (0.0, 0.0, 1.0), 0.8))])
-def test_constrained_loop_matches_oracle(gpu, radius, bond):
+@pytest.mark.parametrize("radius,bond,band", [(None, None, None), (1.3, None, None),
+                                              (2.2, (0.6, (0.0, 0.0, 1.0), 0.8), None), (None, None, 1.5),
+                                              (1.3, None, 1.5)])
+def test_constrained_loop_matches_oracle(gpu, radius, bond, band):
     from loadpath_amd.constrained import Limits, optimize_constrained
     from loadpath_amd.simp import SimpOptions
     from oracle.constrained import Limits as OLimits
@@ -35,10 +38,12 @@ def test_constrained_loop_matches_oracle(gpu, radius, bond):
     n_it = 24
     kw = dict(volume_fraction=0.5, filter_radius=1.5, max_iterations=n_it, beta_every=6, beta_max=8.0)
     r = optimize_constrained(p, 0.3, SimpOptions(solve_rtol=1e-11, **kw),
-                             Limits(stress=80.0, deflection=5.0, modulus=1000.0, radius=radius, bond=bond),
+                             Limits(stress=80.0, deflection=5.0, modulus=1000.0, radius=radius, bond=bond,
+                                    contact_band=band),
                              preserve=pres, precision="fp64", maxiter=4000, coarse_cells=50)
     ro = oracle_loop(_oracle_problem(p), LoopOptions(**kw),
-                     OLimits(stress=80.0, deflection=5.0, modulus=1000.0, radius=radius, bond=bond),
+                     OLimits(stress=80.0, deflection=5.0, modulus=1000.0, radius=radius, bond=bond,
+                             contact_band=band),
                      preserve=pres.ravel())
     assert len(r.history) == len(ro["history"])
     for a, b in zip(r.history, ro["history"]):

@@ -9,18 +9,20 @@ from oracle.loop import LoopOptions
 from oracle.simp import cantilever_3d
 
 
-def _setup(radius=None, bond=None):
+def _setup(radius=None, bond=None, band=None):
     p = cantilever_3d(8, 2, 4)
     opts = LoopOptions(volume_fraction=0.5, filter_radius=1.5, max_iterations=50, beta_every=6, beta_max=8.0)
-    return p, opts, Limits(stress=80.0, deflection=5.0, modulus=1000.0, radius=radius, bond=bond)
+    return p, opts, Limits(stress=80.0, deflection=5.0, modulus=1000.0, radius=radius, bond=bond, contact_band=band)
 
 
-@pytest.mark.parametrize("radius,bond", [(None, None), (1.3, None), (2.2, (0.6, (0.0, 0.0, 1.0), 0.8))])
-def test_constraint_gradients_against_central_differences(radius, bond):
+@pytest.mark.parametrize("radius,bond,band", [(None, None, None), (1.3, None, None),
+                                              (2.2, (0.6, (0.0, 0.0, 1.0), 0.8), None), (None, None, 1.5),
+                                              (1.3, None, 1.5)])
+def test_constraint_gradients_against_central_differences(radius, bond, band):
     """Deflection: d(defl / limit)/dx; stress: kappa dpm/dx (kappa is held
     as the loop holds it), each through filter, projection, erosion and the
     solve, along random directions on the design cells."""
-    p, opts, lim = _setup(radius, bond)
+    p, opts, lim = _setup(radius, bond, band)
     pres = np.zeros(p.dims, bool)
     pres[-1, :, :2] = True
     L = Constrained(p, opts, lim, preserve=pres.ravel())
@@ -55,3 +57,14 @@ def test_loop_invariants():
     sharp = [q for q in h if q["beta"] >= opts.beta_max and max(q["constraints"]) <= 0.02]
     assert d["volume"] == min(q["volume"] for q in sharp)
     assert 0.0 < d["volume"] < 1.0
+
+
+def test_contact_band_masks_corners():
+    """The band drops the corners near the supports and the load, not whole
+    cells: some cells keep part of their corners."""
+    p, opts, lim = _setup(band=1.5)
+    L = Constrained(p, opts, lim)
+    pc = L.pt_counted
+    partial = pc.any(axis=1) & ~pc.all(axis=1)
+    assert partial.any() and (~pc).any() and pc.any()
+    assert np.array_equal(L.counted, pc.any(axis=1))
