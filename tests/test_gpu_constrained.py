@@ -65,3 +65,29 @@ def test_constrained_rejects_what_it_does_not_build(gpu):
         optimize_constrained(p, 0.3, o, Limits(stress=80.0, deflection=None, modulus=1.0, buckling=2.0))
     with pytest.raises(ValueError):
         optimize_constrained(p, 0.3, o, Limits(stress=None, deflection=None, modulus=1.0))
+
+
+def test_constrained_ties_match_oracle(gpu):
+    """A y mirror and extrusion along y (constrained.py:775-780, 796-799):
+    the device loop's iterates equal the oracle's."""
+    from loadpath_amd.constrained import Limits, box_mirror, optimize_constrained
+    from loadpath_amd.simp import SimpOptions
+    from oracle.constrained import Limits as OLimits
+    from oracle.constrained import box_mirror as obox_mirror
+    from oracle.constrained import optimize_constrained as oracle_loop
+    from oracle.loop import LoopOptions
+
+    p = _problem()
+    kw = dict(volume_fraction=0.5, filter_radius=1.5, max_iterations=16, beta_every=6, beta_max=8.0)
+    r = optimize_constrained(p, 0.3, SimpOptions(solve_rtol=1e-11, **kw),
+                             Limits(stress=80.0, deflection=5.0, modulus=1000.0), precision="fp64",
+                             mirror=box_mirror(p.dims, "y"), extrusion=1, maxiter=4000, coarse_cells=50)
+    ro = oracle_loop(_oracle_problem(p), LoopOptions(**kw), OLimits(stress=80.0, deflection=5.0, modulus=1000.0),
+                     mirror=obox_mirror(p.dims, "y"), extrusion=1)
+    assert len(r.history) == len(ro["history"])
+    for a, b in zip(r.history, ro["history"]):
+        assert a["volume"] == pytest.approx(b["volume"], abs=1e-6)
+        assert np.allclose(a["constraints"], b["constraints"], rtol=0, atol=1e-5)
+    x = r.x.reshape(p.dims)
+    assert np.array_equal(x, x[:, ::-1, :]) and np.allclose(x, x[:, :1, :])
+    assert np.abs(r.x - ro["x"]).max() < 1e-5

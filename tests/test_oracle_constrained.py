@@ -68,3 +68,19 @@ def test_contact_band_masks_corners():
     partial = pc.any(axis=1) & ~pc.all(axis=1)
     assert partial.any() and (~pc).any() and pc.any()
     assert np.array_equal(L.counted, pc.any(axis=1))
+
+
+def test_mirror_and_extrusion_ties_hold():
+    """With a y mirror and extrusion along y every iterate is symmetric and
+    constant along each column (the ties the reference applies)."""
+    from oracle.constrained import box_mirror
+
+    p, opts, lim = _setup()
+    opts.max_iterations = 12
+    mir = box_mirror(p.dims, "y")
+    seen = []
+    r = optimize_constrained(p, opts, lim, mirror=mir, extrusion=1, on_iter=lambda rec, xp: seen.append(xp.copy()))
+    x = r["x"].reshape(p.dims)
+    assert np.array_equal(x, x[:, ::-1, :])
+    assert np.allclose(x, x[:, :1, :])
+    assert len(seen) == 12
