@@ -65,3 +65,21 @@ def test_box_problems_are_consistent():
     assert fx[:, :, :, 0].all() and not fx[:, :, :, 1:].any()
     c1 = P.cantilever_48x24x24()
     assert c1.load_grids()[0, 2, 48, 0, 12] == -1.0
+
+
+def test_product_package_never_imports_the_oracle():
+    """The oracle is test infrastructure: no module of the package imports
+    it (the product path has no CPU fallback through it)."""
+    import ast
+    import pathlib
+
+    root = pathlib.Path(__file__).resolve().parents[1] / "loadpath_amd"
+    for path in root.rglob("*.py"):
+        tree = ast.parse(path.read_text(), str(path))
+        for node in ast.walk(tree):
+            names = []
+            if isinstance(node, ast.Import):
+                names = [a.name for a in node.names]
+            elif isinstance(node, ast.ImportFrom) and node.module and node.level == 0:
+                names = [node.module]
+            assert not any(n == "oracle" or n.startswith("oracle.") for n in names), f"{path} imports {names}"
