@@ -36,9 +36,11 @@ HBM_PEAK_GBS = 8000.0
 # profiles/counters_multiload_256x128x128.txt
 FINE_MFMA_TRAFFIC_BYTES = 2.9107e9
 WORKLOAD = dict(nx=256, ny=128, nz=128, cases=8)
+FINE_MFMA_KERNEL = ("k_apply_wave<K, false> (V-cycle fine-level operator: FP16 MFMA contraction, node sums "
+                    "across lanes in registers, fused Chebyshev / residual epilogues in FP32)")
 
 
-def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None):
+def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None, keep_fields=False):
     import ctypes as C
 
     import numpy as np
@@ -84,16 +86,33 @@ def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None):
         prof = dict(launches=n.value, ms=ms.value, bytes=by.value, complete=(rc == 0))
         lib.lp_profile(s.grid._h, 0, eb, 0)
     levels = [lv.dims for lv in s.grid.levels]
+    out = dict(seconds=dt, iters_per_s=steps / dt, hist=hist + timed, timed=timed, prof=prof, levels=levels,
+               fp32_repeats=int(lib.lp_tune(s.grid._h, 6, 0)), strict_switches=int(lib.lp_tune(s.grid._h, 11, 0)),
+               strict=bool(s.grid.strict))
+    if keep_fields:
+        out.update(xphys=s.xphys, dc=s.dc, u=s.displacements())
     del s
-    return dict(seconds=dt, iters_per_s=steps / dt, hist=hist + timed, timed=timed, prof=prof, levels=levels)
+    return out
 
 
-def cpu_baseline(gpu_timed, log=None):
-    """The FP64 numpy oracle on 1/8 of the cells (128x64x64, same supports
-    and load construction): hierarchy build + 2 PCG iterations of one load
-    case, timed; a SIMP iteration at 256x128x128 is extrapolated as
-    8 x (build + PCG-iteration time x the CG iterations all 8 cases took in
-    the GPU run's timed iterations)."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(gpu_timed, nx, ny, nz, log=None):
+    """The FP64 numpy oracle on the headline grid itself (same supports,
+    loads and 6-level hierarchy): one hierarchy build and one PCG iteration
+    of one load case, timed; a SIMP iteration is estimated as the build +
+    that PCG iteration's time x the CG iterations the 8 cases took per SIMP
+    iteration in the GPU run's timed iterations (the oracle runs the same
+    algorithm, so its counts match to a few iterations)."""
     import numpy as np
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
@@ -101,14 +120,14 @@ def cpu_baseline(gpu_timed, log=None):
     from oracle.fem import box_mesh
     from oracle.mg import GridOracle
 
-    nx, ny, nz = 128, 64, 64
     p = multiload_256x128x128(nx, ny, nz)
     mesh = box_mesh(nx, ny, nz)
     fg = p.fixed_grid()
     fdof = np.empty(mesh.n_dof, bool)
     for i in range(3):
         fdof[i::3] = fg[i].ravel()
-    g = GridOracle(mesh, 0.3, fdof, rtol=1e-8, maxiter=2, coarse_cells=200)
+    cc = max(nx * ny * nz // 2 ** 15, 8)       # 6 levels at 256x128x128 (coarsest 8x4x4), as the GPU runs
+    g = GridOracle(mesh, 0.3, fdof, rtol=1e-8, maxiter=1, coarse_cells=cc)
     e = np.full(mesh.n_elem, 1e-9 + 0.15 ** 3 * (1 - 1e-9))
     t0 = time.perf_counter()
     g.set_moduli(e)
@@ -118,13 +137,74 @@ def cpu_baseline(gpu_timed, log=None):
     _, it, _ = g._pcg(b, None)
     t_it = (time.perf_counter() - t0) / max(it, 1)
     cg_total = float(np.mean([st["cg_iters_total"] for st in gpu_timed]))
-    t_simp = 8.0 * (t_build + t_it * cg_total)
+    t_simp = t_build + t_it * cg_total
     if log:
-        log(f"  cpu: build {t_build:.2f}s, {t_it:.3f}s per PCG iteration at 1/8 scale, {cg_total:.0f} CG its/SIMP it")
-    return dict(value=1.0 / t_simp, unit="SIMP iter/s", cores=_CORES, kind="port",
-                sample=f"oracle (numpy FP64 CG + FP32 V-cycle) at 128x64x64 (1/8 of the cells): hierarchy build "
-                       f"{t_build:.1f}s + {it} PCG iterations ({t_it:.2f}s each) of one load case, extrapolated x8 "
-                       f"by cells and by the {cg_total:.0f} CG iterations per SIMP iteration of the 8 cases")
+        log(f"  cpu: build {t_build:.2f}s ({len(g.levels)} levels), {t_it:.3f}s per PCG iteration, "
+            f"{cg_total:.0f} CG its/SIMP it")
+    return dict(value=1.0 / t_simp, unit="SIMP iter/s", cores=_CORES, kind="port", cpu_model=_cpu_model(),
+                sample=f"oracle (numpy FP64 CG + FP32 V-cycle) on the {nx}x{ny}x{nz} grid, {len(g.levels)} levels: "
+                       f"hierarchy build {t_build:.1f}s + {it} PCG iteration ({t_it:.2f}s) of one load case, timed; "
+                       f"a SIMP iteration = build + {cg_total:.0f} PCG iterations (the 8 cases' count per SIMP "
+                       f"iteration in the GPU run), extrapolated by that count only")
+
+
+def config1_bench(iterations, cpu_iterations, log=None):
+    """The 48x24x24 cantilever (27,648 cells, x = 0 clamped, unit load at
+    node (48, 0, 12), vf 0.3, rmin 1.5, rtol 1e-8, 3-level MG) for
+    `iterations` top3d OC iterations on the device (mixed and FP64) and the
+    oracle's loop with its multigrid solves on the host cores for
+    `cpu_iterations`, each end to end from rho = 0.3."""
+    import numpy as np
+    import torch
+
+    from loadpath_amd.problems import cantilever_48x24x24
+    from loadpath_amd.simp import GridSimp
+
+    p = cantilever_48x24x24()
+    res = {}
+    for prec in ("mixed", "fp64"):
+        s = GridSimp(p, precision=prec, smoother="chebyshev", rtol=1e-8, maxiter=500, element="im",
+                     filter="top3d")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h = s.iterate(iterations)
+        torch.cuda.synchronize()
+        res[prec] = dict(seconds=time.perf_counter() - t0, c=[q["compliance"] for q in h],
+                         cg=[q["cg_iters"] for q in h])
+        del s
+    out = dict(workload="cantilever_48x24x24, top3d OC, 3-level MG, Chebyshev(2), rtol 1e-8",
+               iterations=iterations,
+               gpu_mixed_iter_s=round(iterations / res["mixed"]["seconds"], 3),
+               gpu_fp64_iter_s=round(iterations / res["fp64"]["seconds"], 3),
+               compliance_last=res["fp64"]["c"][-1],
+               compliance_rel_err=max(abs(a - b) / abs(b) for a, b in zip(res["mixed"]["c"], res["fp64"]["c"])),
+               cg_iters_mixed=res["mixed"]["cg"], cg_iters_fp64=res["fp64"]["cg"])
+    if cpu_iterations > 0:
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        from oracle import simp as osimp
+        from oracle.mg import GridOracle
+
+        q = osimp.Problem(p.name, p.dims, p.volfrac, p.rmin, p.fixed, p.cases)
+        ev = osimp.Evaluator(q, "im")
+        g = GridOracle(ev.mesh, 0.3, ev.fixed, rtol=1e-8, maxiter=500)
+        prev = [None]
+
+        def solve(e):
+            g.set_moduli(e)
+            us = g.solve_many(ev.fs, prev[0])
+            prev[0] = us
+            return us
+
+        t0 = time.perf_counter()
+        r = osimp.optimize_top3d(q, solve=solve, iterations=cpu_iterations, element="im")
+        dt = time.perf_counter() - t0
+        n = min(cpu_iterations, iterations)
+        out.update(cpu_iter_s=round(cpu_iterations / dt, 4), cpu_iterations=cpu_iterations, cpu_cores=_CORES,
+                   cpu_compliance_rel_err=abs(r["history"][n - 1][1] - res["fp64"]["c"][n - 1])
+                   / abs(res["fp64"]["c"][n - 1]))
+    if log:
+        log(f"  config 1: {out}")
+    return out
 
 
 def run_studies_bench(log=None, max_iterations=40, loop=False):
@@ -287,9 +367,12 @@ def run_constrained_bench(steps, warmup, no_cpu=False, log=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-config1", action="store_true")
+    ap.add_argument("--config1-cpu-iters", type=int, default=50,
+                    help="oracle iterations of the 48x24x24 cantilever timed on the host cores (0: none)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--size", type=int, nargs=3, default=None, help="override the grid (testing)")
     ap.add_argument("-v", "--verbose", action="store_true")
@@ -332,24 +415,30 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    mixed = run_gpu("mixed", a.steps, a.warmup, nx, ny, nz, profile=True, log=log)
-    fp64 = None if a.no_fp64 else run_gpu("fp64", a.steps, a.warmup, nx, ny, nz, log=log)
-    err = None
+    mixed = run_gpu("mixed", a.steps, a.warmup, nx, ny, nz, profile=True, log=log, keep_fields=not a.no_fp64)
+    fp64 = None if a.no_fp64 else run_gpu("fp64", a.steps, a.warmup, nx, ny, nz, log=log, keep_fields=True)
+    errs = {}
     if fp64 is not None:
-        err = max(abs(m["compliance"] - f["compliance"]) / abs(f["compliance"])
-                  for m, f in zip(mixed["hist"], fp64["hist"]))
+        import numpy as np
+
+        errs["compliance_rel_err"] = max(abs(m["compliance"] - f["compliance"]) / abs(f["compliance"])
+                                         for m, f in zip(mixed["hist"], fp64["hist"]))
+        # the fields after the last iteration, both paths from the same start
+        errs["displacement_rel_err"] = float(np.linalg.norm(mixed["u"] - fp64["u"]) / np.linalg.norm(fp64["u"]))
+        errs["sensitivity_rel_err"] = float(np.abs(mixed["dc"] - fp64["dc"]).max() / np.abs(fp64["dc"]).max())
+        errs["density_max_abs_err"] = float(np.abs(mixed["xphys"] - fp64["xphys"]).max())
+        errs["volume_fraction"] = [round(float(mixed["xphys"].mean()), 9), round(float(fp64["xphys"].mean()), 9)]
     prof = mixed["prof"]
     roof = None
     if prof and prof["launches"]:
         gbs = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9
         roof = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
                     traffic=FINE_MFMA_TRAFFIC_BYTES if (nx, ny, nz) == (WORKLOAD["nx"], WORKLOAD["ny"], WORKLOAD["nz"])
-                    else None, kernel="k_apply_wave<K, false> (V-cycle fine-level operator: FP16 MFMA contraction, "
-                                         "node sums across lanes in registers, fused Chebyshev / residual "
-                                         "epilogues in FP32)",
+                    else None, kernel=FINE_MFMA_KERNEL,
                     launches=prof["launches"], avg_us=round(prof["ms"] * 1e3 / prof["launches"], 2),
                     avg_bytes=round(prof["bytes"] / prof["launches"]))
-    cpu = None if a.no_cpu else cpu_baseline(mixed["timed"], log)
+    cfg1 = None if a.no_config1 else config1_bench(50, 0 if a.no_cpu else a.config1_cpu_iters, log)
+    cpu = None if a.no_cpu else cpu_baseline(mixed["timed"], nx, ny, nz, log)
     ms = 1e3 * mixed["seconds"] / a.steps
     out = {
         "metric": "SIMP iter/s at 256x128x128 hex8, mixed vs FP64 GPU, <=1e-6 rel compliance err",
@@ -370,10 +459,14 @@ def main():
                    "cells": nx * ny * nz, "load_cases": 8, "levels": mixed["levels"]},
         "fp64_value": round(fp64["iters_per_s"], 4) if fp64 else None,
         "speedup_vs_fp64": round(mixed["iters_per_s"] / fp64["iters_per_s"], 3) if fp64 else None,
-        "compliance_rel_err": err,
+        **errs,
         "cg_iters_per_step": [st["cg_iters_total"] for st in mixed["timed"]],
+        "fp64_cg_iters_per_step": [st["cg_iters_total"] for st in fp64["timed"]] if fp64 else None,
+        "fp32_repeats": mixed["fp32_repeats"],
+        "strict_switches": [mixed["strict_switches"], fp64["strict_switches"] if fp64 else None],
         "roofline": roof,
         "cpu_baseline": cpu,
+        "config1": cfg1,
     }
     print(json.dumps(out))
 

@@ -382,7 +382,17 @@ int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best);
  * block-tile kernel (node sums through LDS): the fine level (1, the
  * default; the faster choice on each level), both levels (2) or neither
  * (0). Param 8: node planes per block of the weights level's FP16-MFMA
- * kernel (1-64; 8 by default); value 0 returns the current one unchanged. */
+ * kernel (1-64; 8 by default); value 0 returns the current one unchanged.
+ * Param 9: the STALL fallback of GridElasticity._solve (mg.py:950-965) in
+ * lp_solve and the device SIMP loops: the plain hierarchy gets 100 CG
+ * iterations, then the strict one is built and every right-hand side solved
+ * again from its start (1, the default), or the solve runs to maxiter on
+ * the hierarchy as built (0: a caller that runs the fallback itself, as the
+ * reference's GridElasticity does around a backend's solve); 2 caps the
+ * plain hierarchy's attempt at one iteration (for tests). Param 10: the
+ * strict flag, GridElasticity.strict (mg.py:561-586); the next
+ * lp_set_moduli builds the hierarchy it names. Param 11 returns the number
+ * of solves that moved to the strict hierarchy (value ignored). */
 int lp_tune(lp_grid* g, int param, int value);
 
 #ifdef __cplusplus

@@ -135,6 +135,13 @@ struct Grid {
     DBuf cwork;                 // factor workspace
     // CG vectors (FP64), max_rhs each
     DBuf x, r, p, ap, b;        // FP64 CG state (z = M r is never stored in FP64)
+    DBuf x0;                    // the warm start kept for a repeat in the strict hierarchy
+    // the STALL fallback of GridElasticity._solve inside lp_solve and the
+    // device loops (1); 0 when the caller runs it around the solve, as the
+    // reference's GridElasticity does around a backend's solve; 2 caps the
+    // first attempt at one iteration (exercises the strict repeat)
+    int stall = 1;
+    int strict_switches = 0;    // solves that moved to the strict hierarchy
     // reductions
     DBuf partials;              // doubles
     size_t partials_cap = 0;
@@ -320,5 +327,11 @@ void launch_dense_matvec(const double* ainv, int n, const void* b, void* x, int 
 // ---- solver.hip -----------------------------------------------------------
 void vcycle(Grid& G, int level, int nrhs);           // on lv[level].vb -> lv[level].vx
 void pcg(Grid& G, int nrhs, bool warm, int* iters, int* converged);
+// CG iterations the plain hierarchy gets before the strict one takes over (mg.py:928-930)
+constexpr int kStall = 100;
+// pcg with the STALL -> strict fallback (mg.py:950-965); LP_OK or a rebuild's status
+int solve_stall(Grid& G, int nrhs, bool warm, int* iters, int* converged);
+// set_moduli's rebuild from the fine moduli in lv[0].e64, lambda_max included
+int rebuild_hierarchy(Grid& G);
 
 }  // namespace lp

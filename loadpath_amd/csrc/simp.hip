@@ -505,7 +505,8 @@ int loop_iterate(lp_simp* S, lp_simp_stats& st) {
                        G.lv[0].mask.as<uint8_t>(), G.b.as<double>(), g0.nn, vn);
     LP_HIP(hipGetLastError());
     std::vector<int> iters(S->nrhs), ok(S->nrhs);
-    pcg(G, S->nrhs, S->warm && S->have_u, iters.data(), ok.data());
+    rc = solve_stall(G, S->nrhs, S->warm && S->have_u, iters.data(), ok.data());
+    if (rc != LP_OK) return rc;
     for (int k = 0; k < S->nrhs; ++k)
         if (!ok[k]) {
             set_error("linear solve did not converge; the structure may be a mechanism (check supports)");
@@ -802,7 +803,8 @@ int lp_simp_iterate(lp_simp* S, int niter, lp_simp_stats* stats) {
             LP_HIP(hipGetLastError());
             std::vector<int> iters(S->nrhs), ok(S->nrhs);
             const bool warm = S->warm && S->have_u;
-            pcg(G, S->nrhs, warm, iters.data(), ok.data());
+            rc = solve_stall(G, S->nrhs, warm, iters.data(), ok.data());
+            if (rc != LP_OK) return rc;
             bool all = true;
             for (int k = 0; k < S->nrhs; ++k) all &= ok[k] != 0;
             if (!all) {

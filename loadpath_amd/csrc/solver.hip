@@ -473,6 +473,52 @@ static int set_moduli_impl(Grid& G, bool defer) {
     return LP_OK;
 }
 
+int rebuild_hierarchy(Grid& G) { return set_moduli_impl(G, false); }
+
+// GridElasticity._solve (mg.py:950-965) on the batch in G.b / G.x: the plain
+// hierarchy gets at most kStall CG iterations; if a right-hand side has not
+// converged by then, the strict hierarchy is built from the same moduli (and
+// stays: G.strict is sticky, as the reference's attribute is) and every
+// right-hand side is solved again from its original start. The iteration
+// counts add up. Returns LP_OK or the rebuild's status; converged[] tells
+// whether each right-hand side met the tolerance.
+int solve_stall(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
+    const bool stall = G.stall != 0 && !G.strict && G.nlevels >= 2;
+    if (!stall) {
+        pcg(G, nrhs, warm, iters, converged);
+        return LP_OK;
+    }
+    const size_t bytes = 3 * G.g0.nn * (size_t)nrhs * 8;
+    if (warm) {
+        G.x0.alloc(3 * G.g0.nn * (size_t)G.max_rhs * 8);
+        LP_HIP(hipMemcpyAsync(G.x0.p, G.x.p, bytes, hipMemcpyDeviceToDevice, G.stream));
+    }
+    const int full = G.opt.maxiter;
+    G.opt.maxiter = std::min(full, G.stall == 2 ? 1 : kStall);
+    try {
+        pcg(G, nrhs, warm, iters, converged);
+    } catch (...) {
+        G.opt.maxiter = full;
+        throw;
+    }
+    G.opt.maxiter = full;
+    bool ok = true;
+    for (int k = 0; k < nrhs; ++k) ok &= converged[k] != 0;
+    if (ok) return LP_OK;
+    G.strict = true;
+    ++G.strict_switches;
+    const int rc = set_moduli_impl(G, false);
+    if (rc != LP_OK) return rc;
+    if (warm) LP_HIP(hipMemcpyAsync(G.x.p, G.x0.p, bytes, hipMemcpyDeviceToDevice, G.stream));
+    std::vector<int> it2(nrhs), ok2(nrhs);
+    pcg(G, nrhs, warm, it2.data(), ok2.data());
+    for (int k = 0; k < nrhs; ++k) {
+        iters[k] += it2[k];
+        converged[k] = ok2[k];
+    }
+    return LP_OK;
+}
+
 }  // namespace lp
 
 using namespace lp;
@@ -705,26 +751,9 @@ int lp_solve(lp_grid* g, int nrhs, const double* b, double* x, int warm, int dev
         };
         load();
         std::vector<int> it1(nrhs), ok1(nrhs);
-        const bool stall = !G.strict && G.nlevels >= 2;
-        const int full = G.opt.maxiter;
-        if (stall) G.opt.maxiter = std::min(full, 100);   // mg.py:928-965 STALL
-        pcg(G, nrhs, warm != 0, it1.data(), ok1.data());
-        G.opt.maxiter = full;
+        const int rc = solve_stall(G, nrhs, warm != 0, it1.data(), ok1.data());
+        if (rc != LP_OK) return rc;
         bool all_ok = true;
-        for (int k = 0; k < nrhs; ++k) all_ok &= ok1[k] != 0;
-        if (!all_ok && stall) {
-            G.strict = true;
-            const int rc = set_moduli_impl(G, false);
-            if (rc != LP_OK) return rc;
-            load();
-            std::vector<int> it2(nrhs), ok2(nrhs);
-            pcg(G, nrhs, warm != 0, it2.data(), ok2.data());
-            for (int k = 0; k < nrhs; ++k) {
-                it1[k] += it2[k];
-                ok1[k] = ok2[k];
-            }
-        }
-        all_ok = true;
         for (int k = 0; k < nrhs; ++k) {
             iters[k] = it1[k];
             converged[k] = ok1[k];
@@ -933,6 +962,19 @@ int lp_tune(lp_grid* g, int param, int value) {
         ++g->epoch;
         return old;
     }
+    if (param == 9) {
+        if (value < 0 || value > 2) return LP_ERR_ARG;
+        const int old = g->stall;
+        g->stall = value;
+        return old;
+    }
+    if (param == 10) {
+        if (value != 0 && value != 1) return LP_ERR_ARG;
+        const int old = g->strict ? 1 : 0;
+        g->strict = value != 0;
+        return old;
+    }
+    if (param == 11) return g->strict_switches;
     return LP_ERR_ARG;
 }
 

@@ -70,6 +70,9 @@ class HipBackend:
                                              _lib.ptr(np.ascontiguousarray(grid.ke, np.float64)), C.byref(opt)),
                    "lp_grid_create")
         self._h = h
+        # the host grid runs the STALL -> strict fallback around solve()
+        # (mg.py:950-965): the device solves to the cap it is given
+        _lib.lib().lp_tune(h, 9, 0)
         self.last_iters = 0
 
     def __del__(self):
@@ -90,17 +93,25 @@ class HipBackend:
         return np.ascontiguousarray(e, np.float64)
 
     def load(self):
-        """After each set_moduli of the host grid: rebuild on the device."""
+        """After each set_moduli of the host grid: rebuild on the device, in
+        the plain or the strict hierarchy as the host grid's ``strict`` says
+        (the host sets it before the rebuild that follows a stall,
+        mg.py:960-962)."""
         self.opt_sync()
+        _lib.lib().lp_tune(self._h, 10, int(bool(getattr(self.g, "strict", False))))
         _lib.check(_lib.lib().lp_set_moduli(self._h, _lib.ptr(self._moduli()), 0), "set_moduli")
 
     def opt_sync(self):
-        # the host toggles maxiter around the STALL fallback (mg.py:947-956)
+        """The host grid's rtol and maxiter onto the device handle: callers set
+        ``GridElasticity.rtol`` between solves (simp.py:800) and ``_solve``
+        caps maxiter at STALL around the first attempt (mg.py:953-956)."""
         self._opt.maxiter = int(self.g.maxiter)
         self._opt.rtol = float(self.g.rtol)
+        _lib.check(_lib.lib().lp_set_solve_tol(self._h, self._opt.rtol, self._opt.maxiter), "set_solve_tol")
 
     def solve(self, b, x0):
         """b, x0: float64 node grids (masked). Returns (x, iterations, converged)."""
+        self.opt_sync()
         b = np.ascontiguousarray(b, np.float64)[None]
         x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, np.float64)[None].copy()
         its = np.zeros(1, np.int32)

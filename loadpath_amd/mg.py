@@ -113,7 +113,7 @@ class GridElasticity:
             fg[i, self.node_flat] = fixed[i::3]
         self.fixed_grid = fg.reshape((3,) + self.node_shape)
         self.levels = []
-        self.strict = False
+        self._strict = False
         self.last_iters = 0
         self._lmax_started = set()
         self._mask = None
@@ -186,7 +186,7 @@ class GridElasticity:
                 else:
                     _lib.check(lib.lp_estimate_lmax(self._h, i, None), "lmax")
             _lib.check(lib.lp_grid_info(self._h, C.byref(nl), _lib.ptr(dims), C.byref(strict), _lib.ptr(lm)), "info")
-        self.strict = bool(strict.value)
+        self._strict = bool(strict.value)
         self.levels = [_LevelInfo(dims[3 * i:3 * i + 3], lm[i]) for i in range(n)]
         self._mask = None
 
@@ -254,6 +254,18 @@ class GridElasticity:
             _lib.check(rc, "solve")
         self._refresh(lmax=False)
         return x, its, ok
+
+    # strict: the hierarchy the next set_moduli builds (mg.py:561-586); a
+    # solve that stalls in the plain one sets it, and it stays set
+    @property
+    def strict(self):
+        return self._strict
+
+    @strict.setter
+    def strict(self, v):
+        self._strict = bool(v)
+        if getattr(self, "_h", None) is not None:
+            _lib.lib().lp_tune(self._h, 10, int(self._strict))
 
     # rtol / maxiter: the reference's callers set them before a solve
     # (simp.py:734-746); they reach the device handle at once
