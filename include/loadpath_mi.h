@@ -318,6 +318,21 @@ int lp_stiffness_destroy(lp_stiff* s);
  * Host arrays. */
 int lp_label26(int nx, int ny, int nz, const uint8_t* mask, int* labels, int* count);
 
+/* lp_label26 without a device allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). mask
+ * (nx, ny, nz) uint8 and labels (nx, ny, nz) int32 are DEVICE pointers
+ * (labels also holds the parent links while it runs); count is a host
+ * pointer. The round flag and the numbering's per-block root counts live
+ * in a static device array of 1 MiB per GPU, held until the process ends,
+ * so nx * ny * nz is limited to LP_LABEL26_NOALLOC_MAX_CELLS (larger grids
+ * are rejected with LP_ERR_ARG). Runs on the null stream of the current
+ * device and returns when done; every call on one GPU shares that array,
+ * so calls that overlap on one GPU (several host threads) race. Same
+ * labels and count as lp_label26. */
+#define LP_LABEL26_NOALLOC_MAX_CELLS (1 << 26)
+int lp_label26_noalloc(int nx, int ny, int nz, const uint8_t* mask, int* labels, int* count);
+
 /* ---- diagnostics -------------------------------------------------------- */
 const char* lp_last_error(void);
 int lp_version(void);
