@@ -407,7 +407,10 @@ int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best);
  * plain hierarchy's attempt at one iteration (for tests). Param 10: the
  * strict flag, GridElasticity.strict (mg.py:561-586); the next
  * lp_set_moduli builds the hierarchy it names. Param 11 returns the number
- * of solves that moved to the strict hierarchy (value ignored). */
+ * of solves that moved to the strict hierarchy (value ignored). Param 12:
+ * the all-FP64 path's first coarse level is applied as its cell matrices
+ * sum_q w_q M_q, formed at each hierarchy build (1, the default), or
+ * matrix-free as sum_q w_q (M_q u) (0); from the next lp_set_moduli. */
 int lp_tune(lp_grid* g, int param, int value);
 
 #ifdef __cplusplus

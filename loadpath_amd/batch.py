@@ -59,7 +59,8 @@ def _run_one(problem, precision, max_iterations, tol, graph, autotune, out, k, s
         if (st["converged"] if loop else st["change"] <= tol):
             break
     out[k] = dict(name=problem.name, seconds=time.perf_counter() - t0, iterations=len(hist),
-                  compliance=hist[-1]["compliance"], history=hist, xphys=s.xphys)
+                  compliance=hist[-1]["compliance"], history=hist, xphys=s.xphys, strict=bool(s.grid.strict),
+                  fp32_repeats=int(lib.lp_tune(s.grid._h, 6, 0)), strict_switches=int(lib.lp_tune(s.grid._h, 11, 0)))
 
 
 def run_studies(problems, precision="mixed", max_iterations=40, tol=0.01, concurrent=True, graph=True,

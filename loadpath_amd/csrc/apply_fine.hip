@@ -42,7 +42,7 @@ AGeom make_apply_geom(const Grid& Gd, int level) {
 int apply_blocks(const Grid& Gd, int level, int eb) {
     if (weights_mfma_path(Gd, level, eb)) return mfma_blocks(Gd, level);
     if (Gd.lv[level].kind == LK_FINE && eb == 4 && Gd.fp16()) return mfma_blocks(Gd, level);
-    if (Gd.lv[level].kind == LK_EXPLICIT && eb == 4) return explicit_blocks(Gd, level);
+    if (Gd.lv[level].kind == LK_EXPLICIT || explicit_weights(Gd, level, eb)) return explicit_blocks(Gd, level);
     const AGeom G = make_apply_geom(Gd, level);
     return Gd.lv[level].kind == LK_FINE ? 8 * G.per : G.nblk;
 }
@@ -91,6 +91,8 @@ int launch_apply(const Grid& G, int level, int eb, const void* u, size_t vstride
         case LK_WEIGHTS:
             if (weights_mfma_path(G, level, eb))
                 launch_apply_weights_mfma(G, level, u, vstride, nrhs, E, s);
+            else if (explicit_weights(G, level, eb))
+                launch_apply_explicit(G, level, eb, u, vstride, nrhs, E, s);
             else
                 launch_apply_weights(G, level, eb, u, vstride, nrhs, E, s);
             break;

@@ -492,6 +492,7 @@ int build_hierarchy(Grid& G) {
         for (int d = 0; d < 3; ++d) h[d] = h[d] > 1 ? h[d] + h[d] % 2 : h[d];
         C.g.set(h[0], h[1], h[2]);
         C.fine = F.g;
+        C.k64_apply = false;
         dtmp.alloc(3 * C.g.nn * 8);
         LP_HIP(hipMemsetAsync(counter.p, 0, 8, s));
         LP_HIP(hipMemsetAsync(counter2.p, 0, 8, s));
@@ -526,6 +527,17 @@ int build_hierarchy(Grid& G) {
                                G.mq64.as<double>(), C.patch_idx.as<int>(), C.patch64.as<double>(), F.g, C.g,
                                dtmp.as<double>());
             LP_HIP(hipGetLastError());
+            // the FP64 path applies this level as formed cell matrices
+            // sum_q w_q M_q (patch cells: the patch matrix), 1/8 of the
+            // matrix-free form's flops for 4.6 KB per cell read once per apply
+            C.k64_apply = need64 && G.tune_l1_explicit != 0;
+            if (C.k64_apply) {
+                C.k64.alloc(576 * C.g.ncell * 8);
+                hipLaunchKernelGGL(k_weights_explicit, dim3((unsigned)C.g.ncell), dim3(576), 0, s, F.e64.as<double>(),
+                                   G.mq64.as<double>(), C.patch_idx.as<int>(), C.patch64.as<double>(), F.g, C.g,
+                                   C.k64.as<double>());
+                LP_HIP(hipGetLastError());
+            }
             LP_HIP(hipStreamSynchronize(s));   // ctr goes out of scope
         } else {
             C.kind = LK_EXPLICIT;
@@ -591,7 +603,7 @@ int build_hierarchy(Grid& G) {
         G.ncoarse = (int)ndof;
         G.cinv.alloc(ndof * ndof * 8);
         LP_HIP(hipMemsetAsync(G.cinv.p, 0, ndof * ndof * 8, s));
-        if (C.kind == LK_WEIGHTS) {
+        if (C.kind == LK_WEIGHTS && !C.k64_apply) {
             C.k64.alloc(576 * C.g.ncell * 8);
             hipLaunchKernelGGL(k_weights_explicit, dim3((unsigned)C.g.ncell), dim3(576), 0, s,
                                G.lv[nl - 2].e64.as<double>(), G.mq64.as<double>(), C.patch_idx.as<int>(),

@@ -97,6 +97,7 @@ struct Level {
     DBuf patch_prod;            // weights, mixed path: FP32 (max_rhs, npatch, 24) patch products
     int npatch = 0;
     DBuf k64, k32;              // explicit: (576, ncell)
+    bool k64_apply = false;     // weights: k64 holds sum_q w_q M_q and the FP64 operator applies it
     double lmax = 0.0;
     long long n_active = 0;
     // V-cycle work vectors (elem size 8 or 4) for max_rhs right-hand sides
@@ -161,6 +162,7 @@ struct Grid {
     int tune_fused_cheb = 1;    // MFMA levels: Chebyshev vector passes fused into the operator
     int tune_graph = 1;         // PCG replays the V-cycle from a captured hipGraph
     int tune_wave = 1;          // FP16-MFMA levels: wave-tile kernel on the fine level (1), on both (2), on none (0)
+    int tune_l1_explicit = 1;   // FP64 path: first coarse level as formed cell matrices (1) or matrix-free weights (0)
     int tune_flex = -1;         // flexible (Polak-Ribiere) beta: 1 on, 0 off, -1 on for the mixed path only
     bool flex() const { return tune_flex < 0 ? opt.precision == LP_PREC_MIXED : tune_flex != 0; }
     // mixed path: a solve the FP16 levels fail is repeated with FP32
@@ -252,6 +254,8 @@ int apply_blocks(const Grid& G, int level, int eb);
 int mfma_blocks(const Grid& G, int level);
 // blocks (= partials per rhs) of an FP32 explicit-level launch
 int explicit_blocks(const Grid& G, int level);
+// the FP64 operator of a weights level applies its formed cell matrices (k64)
+bool explicit_weights(const Grid& G, int level, int eb);
 void launch_apply_fine(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs,
                        const EpiArgs& E, hipStream_t s);
 void launch_apply_weights(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs,

@@ -489,10 +489,7 @@ int solve_stall(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
         return LP_OK;
     }
     const size_t bytes = 3 * G.g0.nn * (size_t)nrhs * 8;
-    if (warm) {
-        G.x0.alloc(3 * G.g0.nn * (size_t)G.max_rhs * 8);
-        LP_HIP(hipMemcpyAsync(G.x0.p, G.x.p, bytes, hipMemcpyDeviceToDevice, G.stream));
-    }
+    if (warm) LP_HIP(hipMemcpyAsync(G.x0.p, G.x.p, bytes, hipMemcpyDeviceToDevice, G.stream));
     const int full = G.opt.maxiter;
     G.opt.maxiter = std::min(full, G.stall == 2 ? 1 : kStall);
     try {
@@ -637,6 +634,7 @@ int lp_grid_create(lp_grid** out, int nx, int ny, int nz, const uint8_t* fixed_g
             G->p.alloc(vb);
             G->ap.alloc(vb);
             G->b.alloc(vb);
+            G->x0.alloc(vb);
             G->scal.alloc(G->max_rhs * sizeof(CgScal));
             LP_HIP(hipMemset(G->scal.p, 0, G->max_rhs * sizeof(CgScal)));
             G->host_pinned_bytes = G->max_rhs * sizeof(CgScal) + 64;
@@ -975,6 +973,12 @@ int lp_tune(lp_grid* g, int param, int value) {
         return old;
     }
     if (param == 11) return g->strict_switches;
+    if (param == 12) {
+        if (value != 0 && value != 1) return LP_ERR_ARG;
+        const int old = g->tune_l1_explicit;
+        g->tune_l1_explicit = value;
+        return old;
+    }
     return LP_ERR_ARG;
 }
 

@@ -39,31 +39,55 @@ def hex8_ke(nu: float, element: str | None = None) -> np.ndarray:
 
 
 class HexMesh:
-    """Active voxels of a grid and the nodes they use (fem.py:317-374)."""
+    """Active voxels of a grid and the nodes they use (fem.py:317-374).
+
+    The contract is loadpath's numbering: node ids ascend with the node's
+    C-order index on the (max + 2)-node grid, and ``edof = 3 * node +
+    component``. Two constructions give it: a full box of cells starting at
+    (0, 0, 0) numbers every grid node, so ids and corner ids are closed
+    forms; any other set takes the sorted distinct corner keys, where a
+    node's id is its rank among them (``np.unique``'s inverse) and
+    ``node_id`` is a binary search."""
 
     def __init__(self, elem_ijk: np.ndarray, pitch: float, origin=(0.0, 0.0, 0.0)):
         self.elem_ijk = np.asarray(elem_ijk, np.int64).reshape(-1, 3)
         self.pitch = float(pitch)
         self.origin = np.asarray(origin, float)
-        ne = len(self.elem_ijk)
-        if ne == 0:
+        if len(self.elem_ijk) == 0:
             raise ValueError("no active elements")
-        dims = self.elem_ijk.max(axis=0) + 2
-        self.node_dims = dims
-        e_flat = (self.elem_ijk[:, 0] * dims[1] + self.elem_ijk[:, 1]) * dims[2] + self.elem_ijk[:, 2]
-        offs = (CORNERS[:, 0] * dims[1] + CORNERS[:, 1]) * dims[2] + CORNERS[:, 2]
-        used_mark = np.zeros(int(np.prod(dims)), bool)
-        for o in offs:
-            used_mark[e_flat + o] = True
-        used = np.flatnonzero(used_mark)
-        self._lookup = np.full(int(np.prod(dims)), -1, np.int64)
-        self._lookup[used] = np.arange(len(used))
-        self.enode = self._lookup[e_flat[:, None] + offs[None, :]]
-        self.node_ijk = np.stack(np.unravel_index(used, tuple(dims)), axis=1).astype(np.int64)
-        self.n_nodes = len(used)
+        if self.elem_ijk.min() < 0:
+            raise ValueError("cell indices must be non-negative")
+        cells = self.elem_ijk.max(axis=0) + 1
+        self.node_dims = cells + 1
+        nd = tuple(int(d) for d in self.node_dims)
+        stride = np.array([nd[1] * nd[2], nd[2], 1], np.int64)
+        corner_step = CORNERS @ stride                        # (8,) key offsets of the corners
+        cell_key = self.elem_ijk @ stride                     # key of each cell's corner 0
+        self._box = len(self.elem_ijk) == int(np.prod(cells)) and self._is_c_order_box(cells)
+        if self._box:
+            # every node of the grid is used: id == key
+            self._keys = None
+            self.enode = cell_key[:, None] + corner_step[None, :]
+            self.n_nodes = int(np.prod(nd))
+            ids = np.arange(self.n_nodes, dtype=np.int64)
+        else:
+            keys, inv = np.unique((cell_key[:, None] + corner_step[None, :]).ravel(), return_inverse=True)
+            self._keys = keys
+            self.enode = inv.reshape(-1, 8).astype(np.int64)
+            self.n_nodes = len(keys)
+            ids = keys
+        self.node_ijk = np.stack([ids // stride[0], (ids // stride[1]) % nd[1], ids % nd[2]], axis=1)
         self.node_xyz = self.origin + self.node_ijk * self.pitch
         self.n_dof = 3 * self.n_nodes
+        self._stride = stride
         self._edof = None
+
+    def _is_c_order_box(self, cells) -> bool:
+        """The cells are exactly the box [0, cells) listed in C order."""
+        ijk = self.elem_ijk
+        want = np.arange(len(ijk), dtype=np.int64)
+        got = (ijk[:, 0] * cells[1] + ijk[:, 1]) * cells[2] + ijk[:, 2]
+        return bool(np.array_equal(got, want))
 
     @property
     def n_elem(self):
@@ -75,14 +99,20 @@ class HexMesh:
             self._edof = (3 * self.enode[:, :, None] + np.arange(3)).reshape(self.n_elem, 24)
         return self._edof
 
+    def elem_centers(self):
+        return self.origin + (self.elem_ijk + 0.5) * self.pitch
+
     def node_id(self, ijk) -> np.ndarray:
+        """Node id at integer node coordinates, -1 when absent or outside."""
         ijk = np.asarray(ijk, np.int64)
-        d = self.node_dims
-        ok = np.all((ijk >= 0) & (ijk < d), axis=-1)
-        flat = (ijk[..., 0] * d[1] + ijk[..., 1]) * d[2] + ijk[..., 2]
-        out = np.full(ijk.shape[:-1], -1, np.int64)
-        out[ok] = self._lookup[flat[ok]]
-        return out
+        inside = np.all((ijk >= 0) & (ijk < self.node_dims), axis=-1)
+        key = np.where(inside, ijk @ self._stride, 0)
+        if self._box:
+            return np.where(inside, key, -1)
+        pos = np.searchsorted(self._keys, key)
+        pos_c = np.minimum(pos, self.n_nodes - 1)
+        hit = inside & (self._keys[pos_c] == key)
+        return np.where(hit, pos_c, -1).astype(np.int64)
 
 
 def box_mesh(nx: int, ny: int, nz: int, pitch: float = 1.0) -> HexMesh:

@@ -35,9 +35,12 @@ def test_concurrent_studies_match_sequential(gpu):
     a = run_studies(ps, precision="mixed", max_iterations=3, concurrent=True, autotune=False)
     b = run_studies(ps, precision="mixed", max_iterations=3, concurrent=False, autotune=False)
     for sa, sb in zip(a["studies"], b["studies"]):
-        assert sa["iterations"] == sb["iterations"]
-        assert [h["compliance"] for h in sa["history"]] == [h["compliance"] for h in sb["history"]]
-        assert np.array_equal(sa["xphys"], sb["xphys"])
+        info = {k: (sa[k], sb[k]) for k in ("name", "strict", "fp32_repeats", "strict_switches")}
+        info["cg"] = ([h["cg_iters"] for h in sa["history"]], [h["cg_iters"] for h in sb["history"]])
+        info["oc"] = ([h["oc_evals"] for h in sa["history"]], [h["oc_evals"] for h in sb["history"]])
+        assert sa["iterations"] == sb["iterations"], info
+        assert [h["compliance"] for h in sa["history"]] == [h["compliance"] for h in sb["history"]], info
+        assert np.array_equal(sa["xphys"], sb["xphys"]), info
 
 
 def test_concurrent_production_loops_match_sequential(gpu):
