@@ -40,7 +40,7 @@ FINE_MFMA_KERNEL = ("k_apply_wave<K, false> (V-cycle fine-level operator: FP16 M
                     "across lanes in registers, fused Chebyshev / residual epilogues in FP32)")
 
 
-def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None, keep_fields=False):
+def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None, keep_fields=False, cg32=None):
     import ctypes as C
 
     import numpy as np
@@ -52,6 +52,11 @@ def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None, keep_
     p = multiload_256x128x128(nx, ny, nz)
     s = GridSimp(p, precision=precision, smoother="chebyshev", rtol=1e-8, maxiter=500, max_levels=6,
                  coarse_cells=0, element="im", filter="top3d")
+    lib = _lib.lib()
+    if cg32 is not None and precision == "mixed":
+        _lib.check(lib.lp_tune(s.grid._h, 13, int(cg32)), "tune")
+        if os.environ.get("LP_CG32_DELTA"):
+            lib.lp_tune(s.grid._h, 15, int(os.environ["LP_CG32_DELTA"]))
     hist = []
     for k in range(warmup):
         hist += s.iterate(1)
@@ -88,6 +93,7 @@ def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None, keep_
     levels = [lv.dims for lv in s.grid.levels]
     out = dict(seconds=dt, iters_per_s=steps / dt, hist=hist + timed, timed=timed, prof=prof, levels=levels,
                fp32_repeats=int(lib.lp_tune(s.grid._h, 6, 0)), strict_switches=int(lib.lp_tune(s.grid._h, 11, 0)),
+               reliable_updates=int(lib.lp_tune(s.grid._h, 14, 0)),
                strict=bool(s.grid.strict))
     if keep_fields:
         out.update(xphys=s.xphys, dc=s.dc, u=s.displacements())
@@ -371,6 +377,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config1", action="store_true")
+    ap.add_argument("--cg32", type=int, default=None, choices=(0, 1),
+                    help="the mixed path's CG in FP32 with reliable FP64 updates (1) or in FP64 (0); default: the "
+                         "library's")
     ap.add_argument("--config1-cpu-iters", type=int, default=50,
                     help="oracle iterations of the 48x24x24 cantilever timed on the host cores (0: none)")
     ap.add_argument("--no-fp64", action="store_true")
@@ -415,7 +424,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    mixed = run_gpu("mixed", a.steps, a.warmup, nx, ny, nz, profile=True, log=log, keep_fields=not a.no_fp64)
+    mixed = run_gpu("mixed", a.steps, a.warmup, nx, ny, nz, profile=True, log=log, keep_fields=not a.no_fp64,
+                    cg32=a.cg32)
     fp64 = None if a.no_fp64 else run_gpu("fp64", a.steps, a.warmup, nx, ny, nz, log=log, keep_fields=True)
     errs = {}
     if fp64 is not None:
@@ -463,6 +473,7 @@ def main():
         "cg_iters_per_step": [st["cg_iters_total"] for st in mixed["timed"]],
         "fp64_cg_iters_per_step": [st["cg_iters_total"] for st in fp64["timed"]] if fp64 else None,
         "fp32_repeats": mixed["fp32_repeats"],
+        "reliable_updates": mixed["reliable_updates"],
         "strict_switches": [mixed["strict_switches"], fp64["strict_switches"] if fp64 else None],
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -39,6 +39,8 @@ AGeom make_apply_geom(const Grid& Gd, int level) {
     return G;
 }
 
+int fine_vector_blocks(const Grid& Gd) { return 8 * make_apply_geom(Gd, 0).per; }
+
 int apply_blocks(const Grid& Gd, int level, int eb) {
     if (weights_mfma_path(Gd, level, eb)) return mfma_blocks(Gd, level);
     if (Gd.lv[level].kind == LK_FINE && eb == 4 && Gd.fp16()) return mfma_blocks(Gd, level);

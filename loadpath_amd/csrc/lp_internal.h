@@ -137,6 +137,13 @@ struct Grid {
     // CG vectors (FP64), max_rhs each
     DBuf x, r, p, ap, b;        // FP64 CG state (z = M r is never stored in FP64)
     DBuf x0;                    // the warm start kept for a repeat in the strict hierarchy
+    // mixed path, FP32 CG with reliable FP64 updates: direction, its product
+    // and the solution since the last update (FP32); G.x holds the FP64 sum
+    DBuf p32, ap32, xlo32;
+    int tune_cg32 = 0;          // the mixed path's CG in FP32 with reliable updates (1) or in FP64 (0)
+    double cg32_delta = 0.1;    // a reliable update when |r| falls below delta x its largest since the last
+    bool cg32_flex_replaced = false;   // the flexible beta also at a replaced step
+    long long cg32_updates = 0; // reliable updates so far
     // the STALL fallback of GridElasticity._solve inside lp_solve and the
     // device loops (1); 0 when the caller runs it around the solve, as the
     // reference's GridElasticity does around a backend's solve; 2 caps the
@@ -252,6 +259,8 @@ int launch_apply(const Grid& G, int level, int elem_bytes, const void* u, size_t
 // blocks (= reduction partials per rhs) of one operator launch at element size eb
 int apply_blocks(const Grid& G, int level, int eb);
 int mfma_blocks(const Grid& G, int level);
+// blocks (= partials per rhs) of the vector fine-level kernel (launch_apply_fine) at level 0
+int fine_vector_blocks(const Grid& G);
 // blocks (= partials per rhs) of an FP32 explicit-level launch
 int explicit_blocks(const Grid& G, int level);
 // the FP64 operator of a weights level applies its formed cell matrices (k64)
@@ -300,6 +309,16 @@ void launch_cg_update(Grid& G, int nrhs, const void* z, int zb, bool xonly, hipS
 void launch_cg_finalize(Grid& G, int nrhs, int stage, int nb, hipStream_t s);
 // warm start: which 0 = partials of |b - s Ax|^2 (Ax in G.ap), 1 = apply the decision
 void launch_warm(Grid& G, int nrhs, int which, hipStream_t s);
+// FP32 CG with reliable FP64 updates: xlo += alpha p, r -= alpha ap (|r|^2 partials)
+void launch_cg32_step(Grid& G, int nrhs, float* r, hipStream_t s);
+// partials of r . z and (ap != null) z . ap, in the flexible beta's layout
+void launch_dot2_f32(Grid& G, int nrhs, const float* r, const float* z, const float* ap, hipStream_t s);
+// p = z + beta p (first: p = z)
+void launch_cg32_update(Grid& G, int nrhs, const float* z, bool first, hipStream_t s);
+// x += xlo, xlo = 0
+void launch_cg32_fold(Grid& G, int nrhs, hipStream_t s);
+// rv = (float) r (active rhs)
+void launch_cast_active(Grid& G, int nrhs, const double* r, float* rv, hipStream_t s);
 
 // ---- transfer.hip --------------------------------------------------------
 void launch_restrict_masked(const void* rf, const Geom& fine, const uint8_t* cmask, void* rc,

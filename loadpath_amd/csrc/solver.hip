@@ -430,6 +430,129 @@ static void pcg_run(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
     }
 }
 
+// The mixed path's CG in FP32 with reliable FP64 updates (iterative
+// refinement inside one Krylov run): the direction p, its product A p (the
+// FP32 vector operator), the residual r (the V-cycle's input) and the
+// solution since the last update xlo are FP32; the FP64 solution x is
+// updated, and the residual recomputed in FP64 as b - A x, whenever a
+// right-hand side's FP32 residual falls below delta times its largest since
+// the last update or below the tolerance. The recomputed residual replaces
+// the FP32 one and the iteration goes on with the same direction (a
+// Fletcher-Reeves beta at that step); convergence is decided on the FP64
+// residual only, so the answer meets the tolerance as the FP64 CG's does.
+// The warm start and the stopping rule are the FP64 CG's (mg.py:872-926).
+static void pcg32_run(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
+    hipStream_t s = G.stream;
+    const size_t n = 3 * G.g0.nn;
+    const int nbd = dot_blocks(n);
+    const int nba64 = apply_blocks(G, 0, 8);
+    const int nba32 = fine_vector_blocks(G);   // the FP32 vector operator's partials per rhs
+    ensure_partials(G, (size_t)nrhs * std::max({2 * nbd, nba64, nba32}));
+    double* P = G.partials.as<double>();
+    CgScal* S = G.scal.as<CgScal>();
+    Level& L0 = G.lv[0];
+    float* r32 = L0.vb.as<float>();
+    const float* z32 = L0.vx.as<float>();
+    const bool flex = G.flex();
+    launch_dot(G.b.p, G.b.p, 8, n, n, nrhs, P, nullptr, s);
+    launch_cg_finalize(G, nrhs, 3, nbd, s);
+    if (warm) {
+        EpiArgs E;
+        E.kind = EPI_STORE_DOT;
+        E.out = G.ap.p;
+        E.partials = P;
+        launch_apply(G, 0, 8, G.x.p, n, nrhs, E, s);
+        launch_cg_finalize(G, nrhs, 5, nba64, s);
+        launch_dot(G.b.p, G.x.p, 8, n, n, nrhs, P, nullptr, s);
+        launch_cg_finalize(G, nrhs, 6, nbd, s);
+        launch_warm(G, nrhs, 0, s);
+        launch_cg_finalize(G, nrhs, 7, nbd, s);
+        launch_warm(G, nrhs, 1, s);
+        launch_cast(G.r.p, 8, r32, 4, n, n, nrhs, s);
+    } else {
+        launch_fill(G.x.p, 8, n * nrhs, 0.0, s);
+        launch_cast(G.b.p, 8, r32, 4, n, n, nrhs, s);
+    }
+    launch_fill(G.xlo32.p, 4, n * nrhs, 0.0, s);
+    std::vector<CgScal> h;
+    read_scal(G, nrhs, h);
+    // per rhs: the largest FP32 residual norm since the last update
+    std::vector<double> rmax(nrhs);
+    for (int k = 0; k < nrhs; ++k) rmax[k] = h[k].bnorm;
+    if (warm) {
+        // the warm start's residual norm: |b - s A x| when kept, else |b|
+        launch_dot(G.r.p, G.r.p, 8, n, n, nrhs, P, nullptr, s);
+        launch_cg_finalize(G, nrhs, 5, nbd, s);
+        read_scal(G, nrhs, h);
+        for (int k = 0; k < nrhs; ++k) rmax[k] = std::sqrt(h[k].tmp);
+    }
+    vcycle_run(G, nrhs);
+    launch_dot2_f32(G, nrhs, r32, z32, nullptr, s);
+    launch_cg_finalize(G, nrhs, 4, nbd, s);
+    launch_cg32_update(G, nrhs, z32, true, s);
+    for (int it = 1; it <= G.opt.maxiter; ++it) {
+        EpiArgs E;
+        E.kind = EPI_STORE_DOT;
+        E.out = G.ap32.p;
+        E.partials = P;
+        E.scal = S;
+        launch_apply_fine(G, 0, 4, G.p32.p, n, nrhs, E, s);
+        launch_cg_finalize(G, nrhs, 0, nba32, s);
+        launch_cg32_step(G, nrhs, r32, s);
+        launch_cg_finalize(G, nrhs, 8, nbd, s);
+        read_scal(G, nrhs, h);
+        bool any = false, check = false;
+        for (int k = 0; k < nrhs; ++k) {
+            if (!h[k].active) continue;
+            any = true;
+            const double rn = std::sqrt(h[k].rr);
+            if (!(rn > h[k].tol) || rn < G.cg32_delta * rmax[k] || it == G.opt.maxiter) check = true;
+            rmax[k] = std::max(rmax[k], rn);
+        }
+        if (!any) break;
+        bool replaced = false;
+        if (check) {
+            // the reliable update: x += xlo; r = b - A x in FP64; r32 = r
+            launch_cg32_fold(G, nrhs, s);
+            EpiArgs R;
+            R.kind = EPI_RESID;
+            R.b = G.b.p;
+            R.out = G.r.p;
+            R.partials = P;
+            R.scal = S;
+            launch_apply(G, 0, 8, G.x.p, n, nrhs, R, s);
+            launch_cg_finalize(G, nrhs, 9, nba64, s);
+            launch_cast_active(G, nrhs, G.r.as<double>(), r32, s);
+            read_scal(G, nrhs, h);
+            ++G.cg32_updates;
+            replaced = true;
+            any = false;
+            for (int k = 0; k < nrhs; ++k) {
+                if (!h[k].active) continue;
+                any = true;
+                rmax[k] = std::sqrt(h[k].rr);
+            }
+            if (!any) break;
+        }
+        vcycle_run(G, nrhs);
+        if (flex && (!replaced || G.cg32_flex_replaced)) {
+            launch_dot2_f32(G, nrhs, r32, z32, G.ap32.as<float>(), s);
+            launch_cg_beta_flex(G, nrhs, nbd, s);
+        } else {
+            launch_dot2_f32(G, nrhs, r32, z32, nullptr, s);
+            launch_cg_finalize(G, nrhs, 2, nbd, s);
+        }
+        launch_cg32_update(G, nrhs, z32, false, s);
+    }
+    // rhs still active at the cap: their last steps are in xlo
+    launch_cg32_fold(G, nrhs, s);
+    read_scal(G, nrhs, h);
+    for (int k = 0; k < nrhs; ++k) {
+        iters[k] = h[k].iters + (h[k].failed ? 1 : 0);
+        converged[k] = h[k].converged;
+    }
+}
+
 // PCG with the mixed path's safety net: when a right-hand side breaks down
 // or reaches the cap with the FP16 levels in the V-cycle, the solve is
 // repeated with FP32 operators on every level, warm from the iterates (the
@@ -439,7 +562,10 @@ void pcg(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
     const bool net = G.fp16() && G.tune_fallback != 0;
     const int full = G.opt.maxiter;
     if (net && G.tune_fallback == 2) G.opt.maxiter = 1;
-    pcg_run(G, nrhs, warm, iters, converged);
+    // the first attempt of the mixed path may run the CG in FP32 with
+    // reliable FP64 updates; the repeat is the FP64 CG
+    if (G.fp16() && G.tune_cg32 != 0) pcg32_run(G, nrhs, warm, iters, converged);
+    else pcg_run(G, nrhs, warm, iters, converged);
     G.opt.maxiter = full;
     if (!net) return;
     bool ok = true;
@@ -635,6 +761,11 @@ int lp_grid_create(lp_grid** out, int nx, int ny, int nz, const uint8_t* fixed_g
             G->ap.alloc(vb);
             G->b.alloc(vb);
             G->x0.alloc(vb);
+            if (opt->precision == LP_PREC_MIXED) {
+                G->p32.alloc(vb / 2);
+                G->ap32.alloc(vb / 2);
+                G->xlo32.alloc(vb / 2);
+            }
             G->scal.alloc(G->max_rhs * sizeof(CgScal));
             LP_HIP(hipMemset(G->scal.p, 0, G->max_rhs * sizeof(CgScal)));
             G->host_pinned_bytes = G->max_rhs * sizeof(CgScal) + 64;
@@ -973,6 +1104,24 @@ int lp_tune(lp_grid* g, int param, int value) {
         return old;
     }
     if (param == 11) return g->strict_switches;
+    if (param == 13) {
+        if (value != 0 && value != 1) return LP_ERR_ARG;
+        if (value && !g->xlo32.p) return LP_ERR_ARG;   // mixed-precision grids only
+        const int old = g->tune_cg32;
+        g->tune_cg32 = value;
+        return old;
+    }
+    if (param == 14) return (int)std::min<long long>(g->cg32_updates, 0x7fffffff);
+    if (param == 15) {
+        // reliable-update threshold delta in thousandths (1..999); the sign
+        // selects the beta at a replaced step: > 0 Fletcher-Reeves, < 0
+        // the flexible one from the FP32 recursion's A p
+        if (value == 0 || value > 999 || value < -999) return LP_ERR_ARG;
+        const int old = (int)std::lround(g->cg32_delta * 1000.0) * (g->cg32_flex_replaced ? -1 : 1);
+        g->cg32_delta = std::abs(value) / 1000.0;
+        g->cg32_flex_replaced = value < 0;
+        return old;
+    }
     if (param == 12) {
         if (value != 0 && value != 1) return LP_ERR_ARG;
         const int old = g->tune_l1_explicit;

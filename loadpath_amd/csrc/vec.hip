@@ -213,6 +213,19 @@ __global__ void k_cg_finalize(const double* __restrict__ partials, int nb, CgSca
         case 7:  // |b - s Ax|^2 -> keep the scaled start (beta = 1) or drop it (beta = 0)
             c.beta = (c.alpha > 0 && sqrt(s) < c.bnorm) ? 1.0 : 0.0;
             break;
+        case 8:  // the FP32 recurrence's |r|^2: counted, convergence left to the true residual
+            if (!c.active) break;
+            c.rr = s;
+            c.iters += 1;
+            break;
+        case 9:  // the true residual |b - A x|^2 (FP64) of a reliable update -> convergence
+            if (!c.active) break;
+            c.rr = s;
+            if (sqrt(s) <= c.tol) {
+                c.converged = 1;
+                c.active = 0;
+            }
+            break;
         default: break;
     }
 }
@@ -298,6 +311,87 @@ __global__ void k_cg_beta_flex(const double* __restrict__ partials, int nb, int 
     if (!c.active) return;
     c.beta = -c.alpha * zap / c.rz;
     c.rz = rz1;
+}
+
+// ---- FP32 CG with reliable FP64 updates (the mixed path) ----------------
+
+// xlo += alpha p (rhs whose alpha is pending); r -= alpha ap; partial |r|^2.
+// r is the V-cycle's input vector itself (FP32)
+__global__ void k_cg32_step(float* __restrict__ r, const float* __restrict__ ap, float* __restrict__ xlo,
+                            const float* __restrict__ p, size_t n, size_t stride, const CgScal* scal,
+                            double* partials) {
+    __shared__ double sred[4];
+    const int rhs = blockIdx.y;
+    double acc = 0;
+    if (scal[rhs].active) {
+        const float a = (float)scal[rhs].alpha;
+        const bool upd = scal[rhs].xpend != 0;
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+            const size_t k = rhs * stride + i;
+            if (upd) xlo[k] += a * p[k];
+            const float ri = r[k] - a * ap[k];
+            r[k] = ri;
+            acc += (double)ri * (double)ri;
+        }
+    }
+    const double t = block_sum(acc, sred);
+    if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = t;
+}
+
+// partials of r . z (rows 0..nrhs-1) and z . ap (rows nrhs..2 nrhs-1), FP32
+// fields summed in double, active rhs only
+__global__ void k_dot2_f32(const float* __restrict__ r, const float* __restrict__ z, const float* __restrict__ ap,
+                           size_t n, size_t stride, double* partials, const CgScal* scal) {
+    __shared__ double sred[4];
+    const int rhs = blockIdx.y;
+    double acc = 0, acc2 = 0;
+    if (scal[rhs].active) {
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+            const size_t k = rhs * stride + i;
+            const double zi = (double)z[k];
+            acc += (double)r[k] * zi;
+            if (ap) acc2 += (double)ap[k] * zi;
+        }
+    }
+    const double t = block_sum(acc, sred);
+    if (threadIdx.x == 0) partials[(size_t)rhs * gridDim.x + blockIdx.x] = t;
+    if (ap) {
+        __syncthreads();
+        const double t2 = block_sum(acc2, sred);
+        if (threadIdx.x == 0) partials[((size_t)gridDim.y + rhs) * gridDim.x + blockIdx.x] = t2;
+    }
+}
+
+// p = z + beta p for the active rhs (first: p = z)
+__global__ void k_cg32_update(float* __restrict__ p, const float* __restrict__ z, size_t n, size_t stride,
+                              const CgScal* scal, int first) {
+    const int rhs = blockIdx.y;
+    if (!scal[rhs].active) return;
+    const float beta = first ? 0.0f : (float)scal[rhs].beta;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = rhs * stride + i;
+        p[k] = first ? z[k] : z[k] + beta * p[k];
+    }
+}
+
+// the reliable update: x += (double) xlo; xlo = 0 (every rhs: a converged
+// one's xlo is already 0, a broken-down one's holds its last steps)
+__global__ void k_cg32_fold(double* __restrict__ x, float* __restrict__ xlo, size_t n, size_t stride) {
+    const int rhs = blockIdx.y;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = rhs * stride + i;
+        x[k] += (double)xlo[k];
+        xlo[k] = 0.0f;
+    }
+}
+
+// rv = (float) r for the active rhs
+__global__ void k_cast_active(const double* __restrict__ r, float* __restrict__ rv, size_t n, size_t stride,
+                              const CgScal* scal) {
+    const int rhs = blockIdx.y;
+    if (!scal[rhs].active) return;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        rv[rhs * stride + i] = (float)r[rhs * stride + i];
 }
 
 inline int grid_for(size_t n) {
@@ -439,6 +533,40 @@ void launch_cg_update(Grid& G, int nrhs, const void* z, int zb, bool xonly, hipS
     else
         hipLaunchKernelGGL(k_cg_update<double>, dim3(grid_for(n), nrhs), dim3(kBlock), 0, s, G.x.as<double>(),
                            G.p.as<double>(), static_cast<const double*>(z), n, n, G.scal.as<CgScal>(), xonly ? 1 : 0);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_cg32_step(Grid& G, int nrhs, float* r, hipStream_t s) {
+    const size_t n = 3 * G.g0.nn;
+    hipLaunchKernelGGL(k_cg32_step, dim3(dot_blocks(n), nrhs), dim3(kBlock), 0, s, r, G.ap32.as<float>(),
+                       G.xlo32.as<float>(), G.p32.as<float>(), n, n, G.scal.as<CgScal>(), G.partials.as<double>());
+    LP_HIP(hipGetLastError());
+}
+
+void launch_dot2_f32(Grid& G, int nrhs, const float* r, const float* z, const float* ap, hipStream_t s) {
+    const size_t n = 3 * G.g0.nn;
+    hipLaunchKernelGGL(k_dot2_f32, dim3(dot_blocks(n), nrhs), dim3(kBlock), 0, s, r, z, ap, n, n,
+                       G.partials.as<double>(), G.scal.as<CgScal>());
+    LP_HIP(hipGetLastError());
+}
+
+void launch_cg32_update(Grid& G, int nrhs, const float* z, bool first, hipStream_t s) {
+    const size_t n = 3 * G.g0.nn;
+    hipLaunchKernelGGL(k_cg32_update, dim3(grid_for(n), nrhs), dim3(kBlock), 0, s, G.p32.as<float>(), z, n, n,
+                       G.scal.as<CgScal>(), first ? 1 : 0);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_cg32_fold(Grid& G, int nrhs, hipStream_t s) {
+    const size_t n = 3 * G.g0.nn;
+    hipLaunchKernelGGL(k_cg32_fold, dim3(grid_for(n), nrhs), dim3(kBlock), 0, s, G.x.as<double>(),
+                       G.xlo32.as<float>(), n, n);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_cast_active(Grid& G, int nrhs, const double* r, float* rv, hipStream_t s) {
+    const size_t n = 3 * G.g0.nn;
+    hipLaunchKernelGGL(k_cast_active, dim3(grid_for(n), nrhs), dim3(kBlock), 0, s, r, rv, n, n, G.scal.as<CgScal>());
     LP_HIP(hipGetLastError());
 }
 

@@ -241,3 +241,56 @@ def test_mixed_simp_tracks_fp64():
     for sa, sb in zip(ha, hb):
         assert sa["compliance"] == pytest.approx(sb["compliance"], rel=1e-7)
     assert np.abs(a.xphys - b.xphys).max() < 1e-6
+
+
+@pytest.mark.parametrize("rtol", [1e-8, 1e-11])
+def test_fp32_cg_reliable_updates_match_direct(rtol):
+    """The mixed path's CG in FP32 with reliable FP64 updates (lp_tune param
+    13): the answer meets the tolerance in FP64 (it is checked on the FP64
+    residual), warm starts included, in about the FP64 CG's iterations."""
+    from oracle.fem import assemble, hex8_ke
+    from loadpath_amd import _lib
+    from loadpath_amd.mg import GridElasticity
+
+    mesh, fixed, f, e = cantilever_bar(16, 8, 8, seed=2)
+    k = assemble(mesh, hex8_ke(0.3), e)
+    free = ~fixed
+    ref = np.zeros(mesh.n_dof)
+    ref[free] = spla.spsolve(k[free][:, free].tocsc(), f[free])
+    lib = _lib.lib()
+    res = {}
+    for cg32 in (0, 1):
+        g = GridElasticity(mesh, 0.3, fixed, precision="mixed", rtol=rtol, coarse_cells=100, maxiter=2000)
+        lib.lp_tune(g._h, 13, cg32)
+        g.set_moduli(e)
+        u = g.solve_many([f, 3.0 * f])
+        it = g.last_iters
+        # the true residual of the answer, in FP64
+        for j, s in enumerate((1.0, 3.0)):
+            r = (s * f - k @ u[j])[free]
+            # FP32 CG: stopped on its FP64 residual; FP64 CG: on its recursion
+            bound = 1.05 if cg32 else 10.0
+            assert np.linalg.norm(r) <= bound * rtol * np.linalg.norm(s * f[free]), (cg32, j)
+        assert np.linalg.norm(u[0] - ref) <= max(1e3 * rtol, 1e-7) * np.linalg.norm(ref)
+        g.set_moduli(e * 1.1)
+        w = g.solve_many([f], [u[0]])[0]
+        assert np.linalg.norm(w - ref / 1.1) <= max(1e3 * rtol, 1e-7) * np.linalg.norm(ref / 1.1)
+        res[cg32] = (it, int(lib.lp_tune(g._h, 14, 0)), int(lib.lp_tune(g._h, 6, 0)))
+    (i64, _, _), (i32, upd, rep) = res[0], res[1]
+    assert upd >= 1 and rep == 0, res
+    assert i32 <= 1.15 * i64 + 3, res
+
+
+def test_fp32_cg_simp_tracks_fp64():
+    from loadpath_amd import _lib
+    from loadpath_amd.problems import multiload_256x128x128
+    from loadpath_amd.simp import GridSimp
+
+    p = multiload_256x128x128(32, 16, 16)
+    a = GridSimp(p, precision="mixed", rtol=1e-9, coarse_cells=0, max_levels=4)
+    _lib.lib().lp_tune(a.grid._h, 13, 1)
+    b = GridSimp(p, precision="fp64", rtol=1e-9, coarse_cells=0, max_levels=4)
+    ha, hb = a.iterate(3), b.iterate(3)
+    for sa, sb in zip(ha, hb):
+        assert sa["compliance"] == pytest.approx(sb["compliance"], rel=1e-7)
+    assert np.abs(a.xphys - b.xphys).max() < 1e-6
