@@ -57,6 +57,8 @@ def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None, keep_
         _lib.check(lib.lp_tune(s.grid._h, 13, int(cg32)), "tune")
         if os.environ.get("LP_CG32_DELTA"):
             lib.lp_tune(s.grid._h, 15, int(os.environ["LP_CG32_DELTA"]))
+    if precision == "mixed" and os.environ.get("LP_BF16"):
+        _lib.check(lib.lp_tune(s.grid._h, 16, int(os.environ["LP_BF16"])), "tune")
     hist = []
     for k in range(warmup):
         hist += s.iterate(1)

@@ -42,7 +42,26 @@ struct AGeom {
     int ny_per;             // nodes per thread along y (fine kernel)
     int ntz;                // node tiles along z (fine MFMA kernel)
     int nwt;                // wave tiles (wave-tile MFMA kernel)
+    // MFMA kernels: > 0 = the right-hand sides of a tile run in consecutive
+    // blocks of one XCD (their shared moduli, masks and dinv come from its
+    // L2 once), the grid one-dimensional; 0 = one grid z-slice per rhs
+    int rhs_inner;
 };
+
+// block -> (rhs, L) of the MFMA kernels: L is the block's index within its
+// right-hand side's grid (the partials slot); with rhs_inner the 8*per
+// blocks of each rhs interleave as (tile slot, rhs, XCD)
+__device__ __forceinline__ void mfma_block_index(const AGeom& G, int& rhs, int& L) {
+    if (G.rhs_inner > 0) {
+        const int b = (int)blockIdx.x;
+        const int j = b >> 3;
+        rhs = j % G.rhs_inner;
+        L = ((j / G.rhs_inner) << 3) | (b & 7);
+    } else {
+        rhs = (int)blockIdx.z;
+        L = (int)blockIdx.x;
+    }
+}
 
 // Ke as its KE_NV orbit values (see ke_pattern.h): kernel arguments, so
 // they sit in SGPRs for the whole launch
@@ -70,7 +89,7 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-__device__ __forceinline__ void block_partial(double part, EpiArgs& E, int rhs, int L) {
+__device__ __forceinline__ void block_partial(double part, EpiArgs& E, int rhs, int L, int nb = -1) {
     __shared__ double sred[8];
     part = wave_sum(part);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -79,7 +98,7 @@ __device__ __forceinline__ void block_partial(double part, EpiArgs& E, int rhs, 
     if (threadIdx.x == 0) {
         double s = 0;
         for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sred[i];
-        E.partials[(size_t)rhs * gridDim.x + L] = s;
+        E.partials[(size_t)rhs * (nb < 0 ? (int)gridDim.x : nb) + L] = s;
     }
 }
 

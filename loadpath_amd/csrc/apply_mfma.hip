@@ -41,6 +41,8 @@ namespace {
 
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef short short4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 constexpr int TN = 15;              // tile nodes per side (y, z)
 constexpr int TC = TN + 1;          // cells per side
@@ -81,8 +83,8 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
     __shared__ float fsc[NCELL];        // per cell: its products' common factor
     __shared__ __align__(8) _Float16 alds[WTS ? 8 * 3 * 64 * 4 : 4];
     using namespace apply_detail;
-    const int rhs = blockIdx.z;
-    const int L = blockIdx.x;
+    int rhs, L;
+    mfma_block_index(G, rhs, L);
     const int logical = (L & 7) * G.per + (L >> 3);
     double part = 0.0;
     const bool skip = (E.scal != nullptr && !E.scal[rhs].active) || logical >= G.nblk;
@@ -364,7 +366,7 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
             bl = bh;
         }
     }
-    if (E.partials != nullptr) block_partial(part, E, rhs, L);
+    if (E.partials != nullptr) block_partial(part, E, rhs, L, 8 * G.per);
 }
 
 // patch cells of the weights level: (patch, row) per thread, the explicit
@@ -435,6 +437,16 @@ constexpr int WPS = 3 * WCS;            // plane stride
 constexpr int kWaveTiles = kThreads / 64;
 constexpr int WSTAGE = (WPL + 63) / 64; // staging entries per lane
 
+// two FP32 values to a packed pair of BF16 (round to nearest even), lo in
+// the low half: a 2-vector conversion is one v_cvt_pk_bf16_f32 (a 4-vector
+// one is lowered value by value with permutes)
+typedef float float2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned pk_bf16(float lo, float hi) {
+    const float2v v = {lo, hi};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+}
+
 __device__ __forceinline__ float dpp_shr1(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
 }
@@ -453,15 +465,21 @@ __device__ __forceinline__ void swap32(float& a, float& b) {
     b = __uint_as_float(r[1]);
 }
 
-template <int EK, bool WTS>
+// BF (fine level only): BF16 operands instead of the scaled FP16 ones. BF16
+// has FP32's exponent range, so a cell needs no scale of its own: the
+// operand is the corner differences as they are and the products times the
+// cell's modulus are its outputs (8 mantissa bits instead of 11, and none of
+// the per-cell maximum and power-of-two scale instructions).
+template <int EK, bool WTS, bool BF = false>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 4)))
 k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __restrict__ mask, AGeom G, MfmaSrc S,
              EpiArgs E) {
+    static_assert(!(BF && WTS), "BF16 operands on the fine level only");
     __shared__ float stage[kWaveTiles * 2 * WPS];
     __shared__ __align__(8) _Float16 alds[WTS ? 8 * 3 * 64 * 4 : 4];
     using namespace apply_detail;
-    const int rhs = blockIdx.z;
-    const int L = blockIdx.x;
+    int rhs, L;
+    mfma_block_index(G, rhs, L);
     const int logical = (L & 7) * G.per + (L >> 3);
     const int tid = (int)threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -472,6 +490,7 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
     constexpr bool STAGE_B = EK == EPI_CHEB_FIRST || EK == EPI_CHEB_ONLY;
     const int jc = lane & 31, hh = lane >> 5, r = (lane >> 4) & 1, c = lane & 15;
     half4 A[3];
+    short4v Ab[3];
     if (WTS) {
         if (!bskip)
             for (int k = tid; k < 8 * 3 * 64 * 4; k += kThreads) {
@@ -480,6 +499,15 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
                 alds[k] = row < 24 ? (_Float16)S.a[q * 576 + ke_row_of(row) * 24 + ke_row_of(col)] : (_Float16)0.0f;
             }
         __syncthreads();
+    } else if (BF) {
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            bf16x4 t;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                t[i] = (__bf16)(jc < 24 ? S.a[ke_row_of(jc) * 24 + ke_row_of(8 * s + 4 * hh + i)] : 0.0f);
+            Ab[s] = __builtin_bit_cast(short4v, t);
+        }
     } else {
 #pragma unroll
         for (int s = 0; s < 3; ++s)
@@ -566,8 +594,22 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
             for (int q = 0; q < 12; ++q) {
                 const int b4 = q / 3, cc = q % 3;
                 d[q] = PO[cc * WCS + s0 + (b4 & 1) * WS + (b4 >> 1)] - u0v[cc];
-                dmax = fmaxf(dmax, fabsf(d[q]));
+                if (!BF) dmax = fmaxf(dmax, fabsf(d[q]));
             }
+            float y[12];
+            if (BF) {
+                floatx16 acc = {};
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    typedef unsigned uint2v __attribute__((ext_vector_type(2)));
+                    uint2v bt;
+                    bt[0] = pk_bf16(d[4 * s], d[4 * s + 1]);
+                    bt[1] = pk_bf16(d[4 * s + 2], d[4 * s + 3]);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(Ab[s], __builtin_bit_cast(short4v, bt), acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int q = 0; q < 12; ++q) y[q] = acc[q] * ecell;
+            } else {
             {
                 // a cell's 24 operands sit in lanes l and l + 32: one scale
                 float lo = dmax, hi = dmax;
@@ -583,7 +625,6 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
             for (int s = 0; s < 3; ++s)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) B[s][i] = (_Float16)(d[4 * s + i] * sc);
-            float y[12];
             if (!WTS) {
                 floatx16 acc = {};
 #pragma unroll
@@ -634,6 +675,7 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
 #pragma unroll
                     for (int q = 0; q < 12; ++q) y[q] *= f;
                 }
+            }
             }
             // slot q = 3 (by + 2 bz) + comp; z: own bz = 0 + left neighbour's bz = 1
 #pragma unroll
@@ -756,7 +798,7 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
             if (more) store(bl);
         }
     }
-    if (E.partials != nullptr) block_partial(part, E, rhs, L);
+    if (E.partials != nullptr) block_partial(part, E, rhs, L, 8 * G.per);
 }
 
 AGeom mfma_geom(const Grid& Gd, int level) {
@@ -776,12 +818,13 @@ template <bool WTS>
 void launch_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E, const MfmaSrc& S,
                  hipStream_t s) {
     const Level& lv = Gd.lv[level];
-    const AGeom G = mfma_geom(Gd, level);
+    AGeom G = mfma_geom(Gd, level);
+    G.rhs_inner = Gd.tune_rhs_inner ? nrhs : 0;
     if ((size_t)G.Nx * G.Ny * G.Nz * 3 * sizeof(float) > (1ull << 31)) {
         set_error("level operator: too many nodes for 32-bit buffer offsets");
         throw ArgFailure();
     }
-    dim3 grid(8 * G.per, 1, nrhs), block(kThreads);
+    const dim3 grid = G.rhs_inner ? dim3(8 * G.per * nrhs, 1, 1) : dim3(8 * G.per, 1, nrhs), block(kThreads);
     const float* uf = static_cast<const float*>(u);
     const uint8_t* m = lv.mask.as<uint8_t>();
 #define LP_MFMA_CASE(K) \
@@ -822,20 +865,21 @@ AGeom wave_geom(const Grid& Gd, int level) {
     return G;
 }
 
-template <bool WTS>
+template <bool WTS, bool BF = false>
 void launch_wave(const Grid& Gd, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,
                  const MfmaSrc& S, hipStream_t s) {
     const Level& lv = Gd.lv[level];
-    const AGeom G = wave_geom(Gd, level);
+    AGeom G = wave_geom(Gd, level);
+    G.rhs_inner = Gd.tune_rhs_inner ? nrhs : 0;
     if ((size_t)G.Nx * G.Ny * G.Nz * 3 * sizeof(float) > (1ull << 31)) {
         set_error("level operator: too many nodes for 32-bit buffer offsets");
         throw ArgFailure();
     }
-    dim3 grid(8 * G.per, 1, nrhs), block(kThreads);
+    const dim3 grid = G.rhs_inner ? dim3(8 * G.per * nrhs, 1, 1) : dim3(8 * G.per, 1, nrhs), block(kThreads);
     const float* uf = static_cast<const float*>(u);
     const uint8_t* m = lv.mask.as<uint8_t>();
 #define LP_WAVE_CASE(K) \
-    case K: hipLaunchKernelGGL((k_apply_wave<K, WTS>), grid, block, 0, s, uf, vstride, m, G, S, E); break;
+    case K: hipLaunchKernelGGL((k_apply_wave<K, WTS, BF>), grid, block, 0, s, uf, vstride, m, G, S, E); break;
     switch (E.kind) {
         LP_WAVE_CASE(EPI_STORE)
         LP_WAVE_CASE(EPI_STORE_DOT)
@@ -865,7 +909,8 @@ static bool use_wave(const Grid& Gd, int level) {
 }
 
 int mfma_blocks(const Grid& Gd, int level) {
-    return 8 * (use_wave(Gd, level) ? wave_geom(Gd, level).per : mfma_geom(Gd, level).per);
+    const bool wave = use_wave(Gd, level) || (Gd.tune_bf16 && Gd.lv[level].kind == LK_FINE);
+    return 8 * (wave ? wave_geom(Gd, level).per : mfma_geom(Gd, level).per);
 }
 
 bool mfma_level(const Grid& G, int level, int eb) {
@@ -881,7 +926,9 @@ void launch_apply_fine_mfma(const Grid& Gd, int level, const void* u, size_t vst
     MfmaSrc S{};
     S.e = Gd.lv[level].e32.as<float>();
     S.a = Gd.ke32.as<float>();
-    if (use_wave(Gd, level))
+    if (Gd.tune_bf16 && Gd.lv[level].kind == LK_FINE)
+        launch_wave<false, true>(Gd, level, u, vstride, nrhs, E, S, s);
+    else if (use_wave(Gd, level))
         launch_wave<false>(Gd, level, u, vstride, nrhs, E, S, s);
     else
         launch_mfma<false>(Gd, level, u, vstride, nrhs, E, S, s);

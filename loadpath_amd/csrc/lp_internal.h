@@ -169,6 +169,8 @@ struct Grid {
     int tune_fused_cheb = 1;    // MFMA levels: Chebyshev vector passes fused into the operator
     int tune_graph = 1;         // PCG replays the V-cycle from a captured hipGraph
     int tune_wave = 1;          // FP16-MFMA levels: wave-tile kernel on the fine level (1), on both (2), on none (0)
+    int tune_rhs_inner = 1;     // FP16-MFMA kernels: a tile's right-hand sides in consecutive blocks of one XCD
+    int tune_bf16 = 0;          // mixed path, fine level: BF16 operands (1) or scaled FP16 ones (0)
     int tune_l1_explicit = 1;   // FP64 path: first coarse level as formed cell matrices (1) or matrix-free weights (0)
     int tune_flex = -1;         // flexible (Polak-Ribiere) beta: 1 on, 0 off, -1 on for the mixed path only
     bool flex() const { return tune_flex < 0 ? opt.precision == LP_PREC_MIXED : tune_flex != 0; }

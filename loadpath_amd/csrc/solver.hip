@@ -1122,6 +1122,20 @@ int lp_tune(lp_grid* g, int param, int value) {
         g->cg32_flex_replaced = value < 0;
         return old;
     }
+    if (param == 17) {
+        if (value != 0 && value != 1) return LP_ERR_ARG;
+        const int old = g->tune_rhs_inner;
+        g->tune_rhs_inner = value;
+        ++g->epoch;
+        return old;
+    }
+    if (param == 16) {
+        if (value != 0 && value != 1) return LP_ERR_ARG;
+        const int old = g->tune_bf16;
+        g->tune_bf16 = value;
+        ++g->epoch;
+        return old;
+    }
     if (param == 12) {
         if (value != 0 && value != 1) return LP_ERR_ARG;
         const int old = g->tune_l1_explicit;

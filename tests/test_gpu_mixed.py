@@ -294,3 +294,74 @@ def test_fp32_cg_simp_tracks_fp64():
     for sa, sb in zip(ha, hb):
         assert sa["compliance"] == pytest.approx(sb["compliance"], rel=1e-7)
     assert np.abs(a.xphys - b.xphys).max() < 1e-6
+
+
+# BF16 operands (8-bit significand), FP32 accumulation
+TOL_BF16 = 3e-2
+
+
+def _bf16(g):
+    from loadpath_amd import _lib
+
+    assert _lib.lib().lp_tune(g._h, 16, 1) == 0
+    return g
+
+
+def test_bf16_operator_matches_fp64():
+    mesh, fixed, f, e = cantilever_bar(12, 10, 9, seed=11)
+    g, o = _grids(mesh, fixed, e, coarse_cells=20)
+    _bf16(g)
+    rng = np.random.default_rng(1)
+    shape = o.levels[0].node_shape
+    u = rng.standard_normal((2,) + shape).astype(np.float32)
+    got = g.apply(u, level=0, dtype=np.float32)
+    want = np.stack([o.levels[0].apply(v.astype(np.float64)) for v in u])
+    err = np.abs(got - want).max() / np.abs(want).max()
+    assert err <= TOL_BF16, err
+    # the rounding is the operand's, not a scale's: FP16 is tighter
+    from loadpath_amd import _lib
+
+    _lib.lib().lp_tune(g._h, 16, 0)
+    got16 = g.apply(u, level=0, dtype=np.float32)
+    assert np.abs(got16 - want).max() / np.abs(want).max() < err
+
+
+def test_bf16_operator_cancels_rigid_offset_and_tiny_values():
+    mesh, fixed, f, e = cantilever_bar(12, 10, 8, seed=12)
+    free = np.zeros_like(fixed)
+    g, o = _grids(mesh, free, e, coarse_cells=10 ** 9)
+    _bf16(g)
+    rng = np.random.default_rng(2)
+    shape = o.levels[0].node_shape
+    d = rng.standard_normal(shape)
+    for u in ((1000.0 + d), d * 2e-36):
+        u = u.astype(np.float32)
+        got = g.apply(u[None], level=0, dtype=np.float32)[0]
+        want = o.levels[0].apply(u.astype(np.float64))
+        assert np.isfinite(got).all()
+        assert np.abs(got - want).max() <= TOL_BF16 * np.abs(want).max()
+
+
+def test_bf16_solve_and_simp_track_fp64():
+    from oracle.fem import assemble, hex8_ke
+    from loadpath_amd.mg import GridElasticity
+    from loadpath_amd.problems import multiload_256x128x128
+    from loadpath_amd.simp import GridSimp
+
+    mesh, fixed, f, e = cantilever_bar(16, 8, 8, seed=2)
+    k = assemble(mesh, hex8_ke(0.3), e)
+    free = ~fixed
+    ref = np.zeros(mesh.n_dof)
+    ref[free] = spla.spsolve(k[free][:, free].tocsc(), f[free])
+    g = _bf16(GridElasticity(mesh, 0.3, fixed, precision="mixed", rtol=1e-10, coarse_cells=100, maxiter=2000))
+    g.set_moduli(e)
+    u = g.solve_many([f])[0]
+    assert np.linalg.norm(u - ref) <= 1e-7 * np.linalg.norm(ref)
+    p = multiload_256x128x128(32, 16, 16)
+    a = GridSimp(p, precision="mixed", rtol=1e-9, coarse_cells=0, max_levels=4)
+    _bf16(a.grid)
+    b = GridSimp(p, precision="fp64", rtol=1e-9, coarse_cells=0, max_levels=4)
+    ha, hb = a.iterate(3), b.iterate(3)
+    for sa, sb in zip(ha, hb):
+        assert sa["compliance"] == pytest.approx(sb["compliance"], rel=1e-7)
+    assert np.abs(a.xphys - b.xphys).max() < 1e-6
