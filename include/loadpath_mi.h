@@ -233,6 +233,33 @@ typedef struct lp_loop_opts {
  * the iteration after which the loop stops. */
 int lp_simp_set_loop(lp_simp* s, const lp_loop_opts* o, const uint8_t* preserve, const double* case_weights);
 
+/* The production loop's design ties (simp.py:1650, 2223-2226, 2288-2289,
+ * 2296-2304): mirror planes through the box centre across the given axes
+ * (0 x, 1 y, 2 z; domain.mirror, applied in this order as
+ * simp._symmetrize does, simp.py:1476-1482) and extrusion columns along
+ * extrusion_axis (-1: none; manufacture.tie_groups(domain, extrusion),
+ * manufacture.py:413-427, and simp._extrude, simp.py:1453-1463: each
+ * column's design cells take their mean). The sensitivities dc and dv are
+ * tied after the filter, every trial design of the volume root before its
+ * volume is read (void cells emptied again after the columns). */
+int lp_simp_set_ties(lp_simp* s, const int* mirror_axes, int n_mirror, int extrusion_axis);
+
+/* The multigrid path's live-cell reduction of simp._Analysis._solve_mg
+ * (simp.py:795-826): alive = xe > ALIVE (1e-3) or an anchor cell; while
+ * fewer than REDUCE_BELOW (0.8) of the cells are alive, the cells not alive
+ * or not 26-connected to a live cell that touches a support (_connected,
+ * simp.py:264-282) get modulus 0; a load on a dof no kept cell holds, or a
+ * reduced solve that fails, sends the iteration to the whole grid (the
+ * latter from a zero start). support_touch, anchor: (nx, ny, nz) uint8 per
+ * cell (the cell touches a supported node; the cell touches a supported or
+ * loaded node or is preserved, simp.py:154-155), host arrays; NULL for
+ * either switches the reduction off. */
+int lp_simp_set_reduction(lp_simp* s, const uint8_t* support_touch, const uint8_t* anchor);
+/* Counters of the reduction since lp_simp_set_reduction: iterations solved
+ * on the reduced system, whole-grid repeats after a failed reduced solve or
+ * build, and iterations sent to the whole grid by a load on a dead dof. */
+int lp_simp_loop_info(lp_simp* s, int* reduced_iters, int* full_retries, int* full_for_loads);
+
 /* Heaviside projection simp._project / _dproject (simp.py:1443-1450):
  * out = (tanh(b e) + tanh(b (x - e))) / (tanh(b e) + tanh(b (1 - e))), dout
  * its derivative in x; either output may be NULL. */

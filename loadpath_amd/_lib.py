@@ -116,6 +116,9 @@ _SIGS = {
     "lp_nodal_stress": (_I, [_I, _I, _I, _D, _D, _I, _P, _P, _D, _P, _P, _P, _P, _P, _I]),
     "lp_label26": (_I, [_I, _I, _I, _P, _P, _P]),
     "lp_label26_noalloc": (_I, [_I, _I, _I, _P, _P, _P]),
+    "lp_simp_set_ties": (_I, [_P, _P, _I, _I]),
+    "lp_simp_set_reduction": (_I, [_P, _P, _P]),
+    "lp_simp_loop_info": (_I, [_P, _P, _P, _P]),
     "lp_set_solve_tol": (_I, [_P, _D, _I]),
     "lp_material_average": (_I, [_I, _I, _I, _D, _D, _P, _P, _P]),
     "lp_stiffness_create": (_I, [C.POINTER(_P), _I, _I, _I, _P, _P, _P, C.POINTER(C.c_longlong),

@@ -35,6 +35,7 @@ from . import _lib
 from .mma import MMA
 from .simp import E_MIN, PENALTY, SimpOptions, SimpResult, density_filter, dproject, project
 from .stress import StressAveraged, StressPNorm
+from .ties import tie
 
 STALL_GAIN = 5e-3     # constrained.STALL_GAIN (constrained.py:68)
 HELD_SIZES = 1e3      # constrained.HELD_SIZES (constrained.py:93)
@@ -56,52 +57,6 @@ class Limits:
     bond: tuple | None = None
     contact_band: float | None = None
     radius: float | None = None
-
-
-def box_mirror(dims, axes) -> dict:
-    """Domain.mirror for a full box (domain.py:58): per axis ('x', 'y',
-    'z') the C-order index of each cell's mirror image across the box's
-    centre plane."""
-    ijk = np.stack(np.unravel_index(np.arange(int(np.prod(dims))), dims), axis=1)
-    out = {}
-    for ax in axes:
-        a = "xyz".index(ax)
-        m = ijk.copy()
-        m[:, a] = dims[a] - 1 - m[:, a]
-        out[ax] = np.ravel_multi_index(tuple(m.T), dims)
-    return out
-
-
-def extrusion_columns(elem_ijk, dims, axis: int):
-    """manufacture.tie_groups(domain, extrusion=axis) (manufacture.py:413-427):
-    one label per column of cells along the extrusion axis."""
-    others = [i for i in range(3) if i != axis]
-    ij = np.asarray(elem_ijk)[:, others]
-    return np.unique(ij[:, 0] * (dims[others[1]] + 1) + ij[:, 1], return_inverse=True)[1].ravel()
-
-
-def symmetrize(v, mirror):
-    """simp._symmetrize (simp.py:1476-1482): the mean of each cell and its
-    mirror image, plane by plane."""
-    for m in (mirror or {}).values():
-        has = m >= 0
-        out = v.copy()
-        out[has] = 0.5 * (v[has] + v[m[has]])
-        v = out
-    return v
-
-
-def extrude(v, columns, design):
-    """simp._extrude (simp.py:1453-1463): each column's mean over its design
-    cells."""
-    if columns is None:
-        return v
-    col = columns[design]
-    n = col.max() + 1
-    mean = np.bincount(col, weights=v[design], minlength=n) / np.maximum(np.bincount(col, minlength=n), 1)
-    out = v.copy()
-    out[design] = mean[col]
-    return out
 
 
 class _Run:
@@ -251,15 +206,15 @@ def optimize_constrained(problem, nu: float, opts: SimpOptions, limits: Limits, 
     cgIterations, move and the per-case readings; ``.delivered`` is the
     record of the design delivered (None when no design at beta_max was
     inside the limits). autotune: time the fine operator's launch shape for
-    the grid once, at the first solve (a speed setting only). mirror:
-    Domain.mirror, {axis: mirror cell index} (``box_mirror``); extrusion:
-    the axis of a 2.5D part (SimpOptions.extrusion), both tied as the
-    reference ties them (constrained.py:775-780, 796-799)."""
+    the grid once, at the first solve (a speed setting only). mirror: the
+    mirror planes through the box centre, as axes ("y", "xz"; domain.mirror
+    on a box); extrusion: the axis of a 2.5D part (SimpOptions.extrusion);
+    both tied as the reference ties them (constrained.py:775-780, 796-799;
+    loadpath_amd.ties)."""
     _lib.require_gpu()
     t0 = time.time()
     R = _Run(problem, nu, opts, limits, preserve, precision, element, grid_kw, autotune)
     n, design, free = R.n, R.design, R.free
-    columns = None if extrusion is None else extrusion_columns(R.mesh.elem_ijk, R.dims, extrusion)
     x = np.full(n, opts.volume_fraction) if x0 is None else np.clip(np.asarray(x0, float).ravel(), 0.0, 1.0)
     x[R.pres] = 1.0
     mma = MMA(len(free), R.m, xmin=0.0, xmax=1.0, move=min(opts.move, 0.1))
@@ -278,11 +233,8 @@ def optimize_constrained(problem, nu: float, opts: SimpOptions, limits: Limits, 
         cons, grads, rec, kappa, xp, dpb = R.constraints(x, beta, kappa, it)
         f0 = 10.0 * float(xp[design].mean())
         df0 = R.h.T @ (g0 * dpb / R.hs)
-        if mirror:
-            df0 = symmetrize(df0, mirror)
-            grads = [symmetrize(gr, mirror) for gr in grads]
-        df0 = extrude(df0, columns, design)
-        grads = [extrude(gr, columns, design) for gr in grads]
+        df0 = tie(df0, R.dims, mirror, extrusion, design)
+        grads = [tie(gr, R.dims, mirror, extrusion, design) for gr in grads]
         worst = max(cons)
         if worst > FEASIBLE and worst > worst_prev:
             mma.move = max(0.02, 0.5 * mma.move)
@@ -292,9 +244,7 @@ def optimize_constrained(problem, nu: float, opts: SimpOptions, limits: Limits, 
         worst_prev = worst
         xnew = x.copy()
         xnew[free] = mma.update(x[free], f0, df0[free], np.array(cons), np.array([gr[free] for gr in grads]))
-        if mirror:
-            xnew = symmetrize(xnew, mirror)
-        xnew = extrude(xnew, columns, design)
+        xnew = tie(xnew, R.dims, mirror, extrusion, design)
         change = float(np.abs(xnew - x)[design].max())
         x = xnew
         since_beta += 1

@@ -174,6 +174,30 @@ __global__ void k_decode_roots(int n, int* lab) {
 }  // namespace
 }  // namespace lp
 
+namespace lp {
+
+// components of a device cell mask on stream s: labels = each cell's root
+// (its component's smallest cell index), -1 off the mask; flag is a device
+// int for the rounds' changed bit (the host reads it after each round)
+void label26_roots(int nx, int ny, int nz, const uint8_t* mask, int* labels, int* flag, hipStream_t s) {
+    const int n = nx * ny * nz;
+    const unsigned nb = (unsigned)((n + kB - 1) / kB);
+    hipLaunchKernelGGL(k_label_init, dim3(nb), dim3(kB), 0, s, mask, n, labels);
+    LP_HIP(hipGetLastError());
+    for (int round = 0; round < n + 1; ++round) {
+        hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(64), 0, s, flag);
+        hipLaunchKernelGGL(k_label_hook, dim3(nb), dim3(kB), 0, s, nx, ny, nz, labels, flag);
+        hipLaunchKernelGGL(k_label_jump, dim3(nb), dim3(kB), 0, s, n, labels);
+        LP_HIP(hipGetLastError());
+        int changed = 0;
+        LP_HIP(hipMemcpyAsync(&changed, flag, 4, hipMemcpyDeviceToHost, s));
+        LP_HIP(hipStreamSynchronize(s));
+        if (!changed) break;
+    }
+}
+
+}  // namespace lp
+
 using namespace lp;
 
 extern "C" {

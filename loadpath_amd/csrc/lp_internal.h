@@ -349,6 +349,11 @@ bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s);
 void launch_dense_matvec(const double* ainv, int n, const void* b, void* x, int elem_bytes, int nrhs,
                          size_t stride, hipStream_t s);
 
+// ---- label.hip ------------------------------------------------------------
+// 26-connected components of a device cell mask: labels = the root (smallest
+// cell index) of each cell's component, -1 off the mask
+void label26_roots(int nx, int ny, int nz, const uint8_t* mask, int* labels, int* flag, hipStream_t s);
+
 // ---- solver.hip -----------------------------------------------------------
 void vcycle(Grid& G, int level, int nrhs);           // on lv[level].vb -> lv[level].vx
 void pcg(Grid& G, int nrhs, bool warm, int* iters, int* converged);

@@ -77,6 +77,127 @@ __global__ void k_mask_loads(const double* __restrict__ f, const uint8_t* __rest
     b[rhs * stride + i] = ((mask[i % nn] >> c) & 1) ? v : 0.0;
 }
 
+// ---- the production loop's ties and live-cell reduction -----------------
+
+constexpr double kAliveMin = 1e-3;      // simp.ALIVE (simp.py:59)
+constexpr double kReduceBelow = 0.8;    // simp.REDUCE_BELOW (simp.py:61)
+
+__global__ void k_fill_u8(uint8_t* v, size_t n, uint8_t x) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = x;
+}
+
+// mirror plane through the box centre across axis a (domain.mirror,
+// simp._symmetrize, simp.py:1476-1482): a cell and its image both take
+// 0.5 (v + v_image); one thread per pair, the centre plane of an odd axis
+// unchanged (0.5 (v + v) = v)
+__global__ void k_tie_mirror(double* v, int nx, int ny, int nz, int axis) {
+    const int d[3] = {nx, ny, nz};
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c[3];
+    c[2] = (int)(i % nz);
+    c[1] = (int)((i / nz) % ny);
+    c[0] = (int)(i / ((size_t)nz * ny));
+    const int m = d[axis] - 1 - c[axis];
+    if (m <= c[axis]) return;               // the lower cell of each pair acts
+    int cm[3] = {c[0], c[1], c[2]};
+    cm[axis] = m;
+    const size_t j = ((size_t)cm[0] * ny + cm[1]) * nz + cm[2];
+    const double t = 0.5 * (v[i] + v[j]);
+    v[i] = t;
+    v[j] = t;
+}
+
+// extrusion columns along axis a (manufacture.tie_groups, simp._extrude,
+// simp.py:1453-1463): every design cell of a column takes the column's mean
+// over its design cells, summed in increasing order along the axis; one
+// thread per column
+__global__ void k_tie_extrude(double* v, const uint8_t* __restrict__ pres, int nx, int ny, int nz, int axis) {
+    const int d[3] = {nx, ny, nz};
+    const int o0 = axis == 0 ? 1 : 0, o1 = axis == 2 ? 1 : 2;
+    const size_t ncol = (size_t)d[o0] * d[o1];
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncol) return;
+    int c[3];
+    c[o0] = (int)(t / d[o1]);
+    c[o1] = (int)(t % d[o1]);
+    double sum = 0.0;
+    int cnt = 0;
+    for (int k = 0; k < d[axis]; ++k) {
+        c[axis] = k;
+        const size_t i = ((size_t)c[0] * ny + c[1]) * nz + c[2];
+        if (!pres[i]) {
+            sum += v[i];
+            ++cnt;
+        }
+    }
+    if (cnt == 0) return;
+    const double mean = sum / (double)cnt;
+    for (int k = 0; k < d[axis]; ++k) {
+        c[axis] = k;
+        const size_t i = ((size_t)c[0] * ny + c[1]) * nz + c[2];
+        if (!pres[i]) v[i] = mean;
+    }
+}
+
+__global__ void k_zero_passive(double* v, const uint8_t* __restrict__ passive, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && passive[i]) v[i] = 0.0;
+}
+
+// alive = xe > ALIVE or an anchor cell (simp.py:726); partial count
+__global__ void k_alive(const double* __restrict__ xe, const uint8_t* __restrict__ anchor, size_t n, double alive_min,
+                        uint8_t* alive, double* partials) {
+    __shared__ double sred[4];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double part = 0.0;
+    if (i < n) {
+        const uint8_t a = (xe[i] > alive_min || anchor[i]) ? 1 : 0;
+        alive[i] = a;
+        part = a;
+    }
+    const double r = bsum(part, sred);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// the components holding a live cell that touches a support (_connected,
+// simp.py:264-282): their roots are flagged
+__global__ void k_keep_roots(const uint8_t* __restrict__ alive, const uint8_t* __restrict__ support,
+                             const int* __restrict__ labels, size_t n, uint8_t* rootflag) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && alive[i] && support[i]) rootflag[labels[i]] = 1;
+}
+
+// moduli of the reduced system: E where the cell is live and in a kept
+// component, 0 elsewhere (_solve_mg, simp.py:806-807); partial kept count
+__global__ void k_keep_moduli(double* e, const uint8_t* __restrict__ alive, const int* __restrict__ labels,
+                              const uint8_t* __restrict__ rootflag, size_t n, double* partials) {
+    __shared__ double sred[4];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double part = 0.0;
+    if (i < n) {
+        const bool keep = alive[i] && labels[i] >= 0 && rootflag[labels[i]];
+        if (!keep) e[i] = 0.0;
+        part = keep ? 1.0 : 0.0;
+    }
+    const double r = bsum(part, sred);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+// a load on a dof no live cell holds (simp.py:808): flag set
+__global__ void k_dead_loads(const double* __restrict__ f, const uint8_t* __restrict__ mask,
+                             const uint8_t* __restrict__ held, size_t nn, size_t stride, int* flag) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int rhs = blockIdx.y;
+    if (i >= 3 * nn) return;
+    const int c = (int)(i / nn);
+    const size_t node = i % nn;
+    const bool live = (((mask[node] | held[node]) >> c) & 1) != 0;
+    if (!live && f[rhs * stride + i] != 0.0) *flag = 1;
+}
+
 // ce = sum_rhs u_c^T Ke u_c; partial of sum E ce; dc = -p (1-emin) x^(p-1) ce
 __global__ void k_sens(const double* __restrict__ u, const double* __restrict__ ke, Geom g, int nrhs, size_t vstride,
                        const double* __restrict__ xphys, const double* __restrict__ e, double penal, double emin,
@@ -292,6 +413,13 @@ __global__ void k_loop_chain(const double* __restrict__ dcx, const double* __res
 // dc <= 0, dv >= 1e-12 in place (simp.py:2191, 2204), ratio = sqrt(-dc / dv);
 // with a previous step, partials of a = gp . dx and b = (dc + lp dv) . dx
 // (simp.py:2237-2238) in rows 0 and 1
+__global__ void k_clamp_only(double* __restrict__ dc, double* __restrict__ dv, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dc[i] = fmin(dc[i], 0.0);
+    dv[i] = fmax(dv[i], 1e-12);
+}
+
 __global__ void k_loop_clamp(double* __restrict__ dc, double* __restrict__ dv, double* __restrict__ ratio, size_t n,
                              const double* __restrict__ dxp, const double* __restrict__ gp, double lam_p,
                              double* partials) {
@@ -437,6 +565,13 @@ struct lp_simp {
         bool have_prev = false, converged = false, weighted = false;
         std::vector<double> comp_hist;
         DBuf pres, wdes, xt, xe, g, gv, ratio, dxp, gp, wcase;
+        // ties: mirror axes in order, extrusion axis (-1 none)
+        std::vector<int> mirror;
+        int extrusion = -1;
+        // the multigrid path's live-cell reduction (simp.py:795-826)
+        bool reduce = false;
+        DBuf anchor, support, alive, labels, rootflag, e_full, flag;
+        int reduced_iters = 0, full_retries = 0, full_for_loads = 0;
     } L;
     ~lp_simp() {
         if (mma) lp_mma_destroy(mma);
@@ -461,6 +596,25 @@ double reduce_host(lp_simp* S, const double* partials, int nb, int slot) {
     LP_HIP(hipStreamSynchronize(s));
     return v;
 }
+
+// the design ties of the loop on a cell field (mirror planes in order, then
+// the extrusion columns)
+void loop_ties(lp_simp* S, double* v) {
+    const lp_simp::Loop& L = S->L;
+    const Geom& g0 = S->g->g0;
+    hipStream_t s = S->g->stream;
+    for (int a : L.mirror)
+        hipLaunchKernelGGL(k_tie_mirror, dim3(nblk(S->ncell)), dim3(kBlock), 0, s, v, g0.nx, g0.ny, g0.nz, a);
+    if (L.extrusion >= 0) {
+        const int d[3] = {g0.nx, g0.ny, g0.nz};
+        const size_t ncol = S->ncell / (size_t)d[L.extrusion];
+        hipLaunchKernelGGL(k_tie_extrude, dim3(nblk(ncol)), dim3(kBlock), 0, s, v, L.pres.as<uint8_t>(), g0.nx, g0.ny,
+                           g0.nz, L.extrusion);
+    }
+    LP_HIP(hipGetLastError());
+}
+
+bool loop_tied(const lp_simp* S) { return !S->L.mirror.empty() || S->L.extrusion >= 0; }
 
 void loop_filter(lp_simp* S, const double* v, const double* post, double* out) {
     const Geom& g0 = S->g->g0;
@@ -492,26 +646,103 @@ int loop_iterate(lp_simp* S, lp_simp_stats& st) {
                        L.xe.as<double>(), G.lv[0].e64.as<double>(), S->partials.as<double>());
     LP_HIP(hipGetLastError());
     const double vol_now = reduce_host(S, S->partials.as<double>(), (int)nb, 2);
-    int rc = build_hierarchy(G);
-    if (rc == LP_SINGULAR && !G.strict) {
-        G.strict = true;
-        rc = build_hierarchy(G);
+    double* e64 = G.lv[0].e64.as<double>();
+    // the multigrid path's reduction (_solve_mg, simp.py:795-826): under
+    // REDUCE_BELOW of the cells live, the cells that are not live or not
+    // joined to a support get modulus 0
+    bool reduced = false;
+    if (L.reduce) {
+        hipLaunchKernelGGL(k_alive, dim3(nb), dim3(kBlock), 0, s, L.xe.as<double>(), L.anchor.as<uint8_t>(), n,
+                           kAliveMin, L.alive.as<uint8_t>(), S->partials.as<double>());
+        LP_HIP(hipGetLastError());
+        const double n_alive = reduce_host(S, S->partials.as<double>(), (int)nb, 5);
+        if (n_alive < kReduceBelow * (double)n) {
+            LP_HIP(hipMemcpyAsync(L.e_full.p, e64, n * 8, hipMemcpyDeviceToDevice, s));
+            label26_roots(g0.nx, g0.ny, g0.nz, L.alive.as<uint8_t>(), L.labels.as<int>(), L.flag.as<int>(), s);
+            hipLaunchKernelGGL(k_fill_u8, dim3(nb), dim3(kBlock), 0, s, L.rootflag.as<uint8_t>(), n, (uint8_t)0);
+            hipLaunchKernelGGL(k_keep_roots, dim3(nb), dim3(kBlock), 0, s, L.alive.as<uint8_t>(),
+                               L.support.as<uint8_t>(), L.labels.as<int>(), n, L.rootflag.as<uint8_t>());
+            hipLaunchKernelGGL(k_keep_moduli, dim3(nb), dim3(kBlock), 0, s, e64, L.alive.as<uint8_t>(),
+                               L.labels.as<int>(), L.rootflag.as<uint8_t>(), n, S->partials.as<double>());
+            LP_HIP(hipGetLastError());
+            const double n_keep = reduce_host(S, S->partials.as<double>(), (int)nb, 6);
+            reduced = n_keep < (double)n;
+            if (!reduced) LP_HIP(hipMemcpyAsync(e64, L.e_full.p, n * 8, hipMemcpyDeviceToDevice, s));
+        }
+    }
+    auto rebuild = [&]() -> int {
+        int r = build_hierarchy(G);
+        if (r == LP_SINGULAR && !G.strict) {
+            G.strict = true;
+            r = build_hierarchy(G);
+        }
+        if (r != LP_OK) return r;
+        G.built = true;
+        if (G.opt.smoother == LP_SMOOTH_CHEBYSHEV)
+            for (int i = 0; i < G.nlevels; ++i) lp_estimate_lmax(S->g, i, nullptr);
+        return LP_OK;
+    };
+    auto full_grid = [&]() -> int {
+        LP_HIP(hipMemcpyAsync(e64, L.e_full.p, n * 8, hipMemcpyDeviceToDevice, s));
+        reduced = false;
+        return rebuild();
+    };
+    int rc = rebuild();
+    if (reduced && rc == LP_OK) {
+        // a load on a dof the kept cells do not hold: the whole grid
+        LP_HIP(hipMemsetAsync(L.flag.p, 0, 4, s));
+        hipLaunchKernelGGL(k_dead_loads, dim3(nblk(vn), S->nrhs), dim3(kBlock), 0, s, S->loads.as<double>(),
+                           G.lv[0].mask.as<uint8_t>(), G.lv[0].held.as<uint8_t>(), g0.nn, vn, L.flag.as<int>());
+        LP_HIP(hipGetLastError());
+        int dead = 0;
+        LP_HIP(hipMemcpyAsync(&dead, L.flag.p, 4, hipMemcpyDeviceToHost, s));
+        LP_HIP(hipStreamSynchronize(s));
+        if (dead) {
+            ++L.full_for_loads;
+            rc = full_grid();
+        }
+    }
+    if (rc != LP_OK && reduced) {
+        ++L.full_retries;
+        rc = full_grid();
     }
     if (rc != LP_OK) return rc;
-    G.built = true;
-    if (G.opt.smoother == LP_SMOOTH_CHEBYSHEV)
-        for (int i = 0; i < G.nlevels; ++i) lp_estimate_lmax(S->g, i, nullptr);
-    hipLaunchKernelGGL(k_mask_loads, dim3(nblk(vn), S->nrhs), dim3(kBlock), 0, s, S->loads.as<double>(),
-                       G.lv[0].mask.as<uint8_t>(), G.b.as<double>(), g0.nn, vn);
-    LP_HIP(hipGetLastError());
     std::vector<int> iters(S->nrhs), ok(S->nrhs);
-    rc = solve_stall(G, S->nrhs, S->warm && S->have_u, iters.data(), ok.data());
+    auto solve_now = [&](bool warm) -> int {
+        hipLaunchKernelGGL(k_mask_loads, dim3(nblk(vn), S->nrhs), dim3(kBlock), 0, s, S->loads.as<double>(),
+                           G.lv[0].mask.as<uint8_t>(), G.b.as<double>(), g0.nn, vn);
+        LP_HIP(hipGetLastError());
+        return solve_stall(G, S->nrhs, warm, iters.data(), ok.data());
+    };
+    auto all_ok = [&]() {
+        for (int k = 0; k < S->nrhs; ++k)
+            if (!ok[k]) return false;
+        return true;
+    };
+    const bool warm = S->warm && S->have_u;
+    if (warm && L.reduce) {
+        // the last displacements, masked to this system's live dofs, start
+        // the solve (mg.py:941: x0 * mask)
+        hipLaunchKernelGGL(k_mask_loads, dim3(nblk(vn), S->nrhs), dim3(kBlock), 0, s, G.x.as<double>(),
+                           G.lv[0].mask.as<uint8_t>(), G.x.as<double>(), g0.nn, vn);
+        LP_HIP(hipGetLastError());
+    }
+    rc = solve_now(warm);
+    if (reduced && (rc != LP_OK || !all_ok())) {
+        // the kept cells hang on the rest by an edge or a corner, or a
+        // coarse level lost a thin support: the whole grid, from zero
+        // (simp.py:817-822)
+        ++L.full_retries;
+        rc = full_grid();
+        if (rc != LP_OK) return rc;
+        rc = solve_now(false);
+    }
     if (rc != LP_OK) return rc;
-    for (int k = 0; k < S->nrhs; ++k)
-        if (!ok[k]) {
-            set_error("linear solve did not converge; the structure may be a mechanism (check supports)");
-            return LP_NOT_CONVERGED;
-        }
+    if (!all_ok()) {
+        set_error("linear solve did not converge; the structure may be a mechanism (check supports)");
+        return LP_NOT_CONVERGED;
+    }
+    if (reduced) ++L.reduced_iters;
     S->have_u = true;
     // compliance and dc_xp on the analysed design (simp.py:1897-1904)
     hipLaunchKernelGGL(k_sens, dim3(nb), dim3(kBlock), 0, s, G.x.as<double>(), G.ke64.as<double>(), g0, S->nrhs, vn,
@@ -528,6 +759,14 @@ int loop_iterate(lp_simp* S, lp_simp_stats& st) {
     LP_HIP(hipGetLastError());
     loop_filter(S, L.g.as<double>(), nullptr, S->dc.as<double>());
     loop_filter(S, L.gv.as<double>(), nullptr, S->dv.as<double>());
+    if (loop_tied(S)) {
+        // the clamped sensitivities of tied cells are averaged like the
+        // cells (simp.py:2191-2226); the clamp below then leaves them
+        hipLaunchKernelGGL(k_clamp_only, dim3(nb), dim3(kBlock), 0, s, S->dc.as<double>(), S->dv.as<double>(), n);
+        LP_HIP(hipGetLastError());
+        loop_ties(S, S->dc.as<double>());
+        loop_ties(S, S->dv.as<double>());
+    }
     // clamps, ratio and the measured step (simp.py:2228-2245)
     double sec = -1.0;
     double move;
@@ -556,6 +795,13 @@ int loop_iterate(lp_simp* S, lp_simp_stats& st) {
         hipLaunchKernelGGL(k_loop_step, dim3(nb), dim3(kBlock), 0, s, S->x.as<double>(), L.ratio.as<double>(), pres,
                            pas, n, move, std::sqrt(lm), S->xnew.as<double>());
         LP_HIP(hipGetLastError());
+        if (loop_tied(S)) {
+            // each trial design is tied, then its void cells emptied again
+            // (simp.py:2296-2304)
+            loop_ties(S, S->xnew.as<double>());
+            if (pas) hipLaunchKernelGGL(k_zero_passive, dim3(nb), dim3(kBlock), 0, s, S->xnew.as<double>(), pas, n);
+            LP_HIP(hipGetLastError());
+        }
         loop_filter(S, S->xnew.as<double>(), S->hs_inv.as<double>(), L.g.as<double>());
         hipLaunchKernelGGL(k_loop_volume, dim3(nb), dim3(kBlock), 0, s, L.g.as<double>(), pres, pas,
                            L.wdes.as<double>(), n, beta, o.eta, S->partials.as<double>());
@@ -1013,6 +1259,61 @@ int lp_simp_set_loop(lp_simp* S, const lp_loop_opts* o, const uint8_t* preserve,
         LP_HIP(hipStreamSynchronize(s));
         return LP_OK;
     });
+}
+
+int lp_simp_set_ties(lp_simp* S, const int* mirror_axes, int n_mirror, int extrusion_axis) {
+    return guarded_s([&]() -> int {
+        if (!S || n_mirror < 0 || n_mirror > 3 || (n_mirror > 0 && !mirror_axes) || extrusion_axis < -1 ||
+            extrusion_axis > 2) {
+            set_error("lp_simp_set_ties: bad argument");
+            return LP_ERR_ARG;
+        }
+        if (!S->L.on) {
+            set_error("lp_simp_set_ties: the production loop is not set (lp_simp_set_loop first)");
+            return LP_ERR_STATE;
+        }
+        std::vector<int> m;
+        for (int k = 0; k < n_mirror; ++k) {
+            if (mirror_axes[k] < 0 || mirror_axes[k] > 2) {
+                set_error("lp_simp_set_ties: mirror axis out of range");
+                return LP_ERR_ARG;
+            }
+            m.push_back(mirror_axes[k]);
+        }
+        S->L.mirror = m;
+        S->L.extrusion = extrusion_axis;
+        return LP_OK;
+    });
+}
+
+int lp_simp_set_reduction(lp_simp* S, const uint8_t* support_touch, const uint8_t* anchor) {
+    return guarded_s([&]() -> int {
+        if (!S) return LP_ERR_ARG;
+        lp_simp::Loop& L = S->L;
+        if (!support_touch || !anchor) {
+            L.reduce = false;
+            return LP_OK;
+        }
+        const size_t n = S->ncell;
+        for (DBuf* b : {&L.anchor, &L.support, &L.alive, &L.rootflag})
+            if (!b->p) b->alloc(n);
+        if (!L.labels.p) L.labels.alloc(n * 4);
+        if (!L.e_full.p) L.e_full.alloc(n * 8);
+        if (!L.flag.p) L.flag.alloc(4);
+        LP_HIP(hipMemcpy(L.support.p, support_touch, n, hipMemcpyHostToDevice));
+        LP_HIP(hipMemcpy(L.anchor.p, anchor, n, hipMemcpyHostToDevice));
+        L.reduce = true;
+        L.reduced_iters = L.full_retries = L.full_for_loads = 0;
+        return LP_OK;
+    });
+}
+
+int lp_simp_loop_info(lp_simp* S, int* reduced_iters, int* full_retries, int* full_for_loads) {
+    if (!S) return LP_ERR_ARG;
+    if (reduced_iters) *reduced_iters = S->L.reduced_iters;
+    if (full_retries) *full_retries = S->L.full_retries;
+    if (full_for_loads) *full_for_loads = S->L.full_for_loads;
+    return LP_OK;
 }
 
 int lp_simp_set_optimizer(lp_simp* S, int optimizer) {

@@ -162,9 +162,13 @@ class GridSimp:
             out.append(d)
         return out
 
-    def set_loop(self, opts: "SimpOptions", preserve=None, case_weights=None):
+    def set_loop(self, opts: "SimpOptions", preserve=None, case_weights=None, mirror=None, reduce=True):
         """Switch to the production loop (lp_simp_set_loop) and restart it
-        from x = volume fraction (preserved cells 1)."""
+        from x = volume fraction (preserved cells 1). mirror: mirror planes
+        through the box centre as axes ("y", "xz"; domain.mirror), tied with
+        opts.extrusion's columns (lp_simp_set_ties); reduce: the multigrid
+        path's live-cell reduction of simp._Analysis (lp_simp_set_reduction,
+        the reference's behaviour)."""
         sched = None if opts.beta_schedule is None else np.ascontiguousarray(opts.beta_schedule, np.int32)
         o = _lib.LoopOpts(beta_start=opts.beta_start, beta_max=opts.beta_max, beta_every=opts.beta_every,
                           eta=opts.eta, erode=opts.erode, move=opts.move, adaptive_move=int(opts.adaptive_move),
@@ -179,9 +183,49 @@ class GridSimp:
             raise ValueError(f"case_weights: {self.nrhs} load cases, {len(self._cw)} weights")
         _lib.check(_lib.lib().lp_simp_set_loop(self._h, C.byref(o), None if self._pres is None else _lib.ptr(self._pres),
                                                None if self._cw is None else _lib.ptr(self._cw)), "simp_set_loop")
+        axes = np.ascontiguousarray([_AXIS[a] if isinstance(a, str) else int(a) for a in (mirror or "")], np.int32)
+        ext = -1 if opts.extrusion is None else (_AXIS[opts.extrusion] if isinstance(opts.extrusion, str)
+                                                 else int(opts.extrusion))
+        _lib.check(_lib.lib().lp_simp_set_ties(self._h, _lib.ptr(axes) if len(axes) else None, len(axes), ext),
+                   "simp_set_ties")
+        if reduce:
+            self._support, self._anchor = self._anchor_cells(self._pres)
+            _lib.check(_lib.lib().lp_simp_set_reduction(self._h, _lib.ptr(self._support), _lib.ptr(self._anchor)),
+                       "simp_set_reduction")
+        else:
+            _lib.check(_lib.lib().lp_simp_set_reduction(self._h, None, None), "simp_set_reduction")
         self.optimizer = "loop"
         self.loop_opts = opts
         self.iterations = 0
+
+    def _anchor_cells(self, pres):
+        """(support_touch, anchor) per cell, uint8: the cell touches a node
+        with a held dof; it does, or touches a loaded node, or is preserved
+        (simp._Analysis, simp.py:154-155)."""
+        held = self.problem.fixed_grid().any(axis=0)
+        loaded = (self._loads != 0).any(axis=(0, 1))
+
+        def touch(nodes):
+            c = np.zeros(self.problem.dims, bool)
+            for a in (0, 1):
+                for b in (0, 1):
+                    for d in (0, 1):
+                        c |= nodes[a:a + c.shape[0], b:b + c.shape[1], d:d + c.shape[2]]
+            return c
+
+        sup = touch(held)
+        anc = sup | touch(loaded)
+        if pres is not None:
+            anc |= pres.astype(bool)
+        return np.ascontiguousarray(sup, np.uint8), np.ascontiguousarray(anc, np.uint8)
+
+    def loop_info(self):
+        """The reduction's counters: iterations solved reduced, whole-grid
+        repeats after a failed reduced solve, iterations sent to the whole
+        grid by a load on a dead dof."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _lib.check(_lib.lib().lp_simp_loop_info(self._h, C.byref(a), C.byref(b), C.byref(c)), "simp_loop_info")
+        return dict(reduced=a.value, full_retries=b.value, full_for_loads=c.value)
 
     def _get(self, what):
         out = np.empty(self.problem.dims)
@@ -211,6 +255,7 @@ class GridSimp:
 
 
 E_MIN = 1e-6       # simp.py:42 (the production loop's void modulus)
+_AXIS = {"x": 0, "y": 1, "z": 2}
 PENALTY = 3.0      # simp.py:55
 
 
@@ -235,6 +280,7 @@ class SimpOptions:
     erode: float = 0.15
     beta_start: float = 1.0
     prune: bool = True
+    extrusion: int | str | None = None   # the axis of a 2.5D part (SimpOptions.extrusion)
 
 
 @dataclass
@@ -258,7 +304,7 @@ class SimpResult:
 
 
 def optimize(problem, opts: SimpOptions, *, preserve=None, void=None, case_weights=None, precision="mixed", pad=True,
-             on_iter=None, delivered=True, **kw) -> SimpResult:
+             on_iter=None, delivered=True, mirror=None, reduce=True, **kw) -> SimpResult:
     """simp.optimize (simp.py:1570-1640) for a box problem with the plain
     compliance objective, every iteration on the device (loop
     simp.py:1840-2449). preserve: cells held solid (domain.preserve);
@@ -266,12 +312,15 @@ def optimize(problem, opts: SimpOptions, *, preserve=None, void=None, case_weigh
     target; case_weights: the load cases' weights (domain.case_weights).
     delivered: hand over the reference's delivered design (the opening of
     the eroded design with its cores kept, pruned when opts.prune:
-    deliver()); False: the blueprint at the loop's end. kw go to GridSimp (precision,
+    deliver()); False: the blueprint at the loop's end. mirror: mirror
+    planes through the box centre as axes (domain.mirror), tied with
+    opts.extrusion's columns; reduce: the multigrid path's live-cell
+    reduction (simp.py:795-826). kw go to GridSimp (precision,
     coarse_cells, element, ...)."""
     kw.setdefault("rtol", opts.solve_rtol)
     s = GridSimp(problem, precision=precision, filter="padded" if pad else "top3d", volfrac=opts.volume_fraction,
                  rmin=opts.filter_radius, active=None if void is None else ~np.asarray(void, bool), **kw)
-    s.set_loop(opts, preserve=preserve, case_weights=case_weights)
+    s.set_loop(opts, preserve=preserve, case_weights=case_weights, mirror=mirror, reduce=reduce)
     t0 = time.time()
     history = []
     converged = False

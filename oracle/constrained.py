@@ -39,7 +39,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import stress as S
-from .loop import E_MIN, PENALTY, LoopOptions
+from .loop import E_MIN, PENALTY, LoopOptions, column_average, mirror_average
 from .mma import MMA
 from .simp import Evaluator, GridFilter, dproject, element_energy, project
 
@@ -65,52 +65,6 @@ CORNER_OF = (0, 4, 2, 6, 1, 5, 3, 7)   # constrained._CORNER_OF (constrained.py:
 def corner_cbs(nu, element="im", pitch=1.0):
     """C B / pitch at the 8 corners, x outermost (constrained.py:383-385)."""
     return S.sample_cb(nu, "corners", element, pitch)
-
-
-def box_mirror(dims, axes) -> dict:
-    """Domain.mirror for a full box (domain.py:58): per axis ('x', 'y',
-    'z') the C-order index of each cell's mirror image across the box's
-    centre plane."""
-    ijk = np.stack(np.unravel_index(np.arange(int(np.prod(dims))), dims), axis=1)
-    out = {}
-    for ax in axes:
-        a = "xyz".index(ax)
-        m = ijk.copy()
-        m[:, a] = dims[a] - 1 - m[:, a]
-        out[ax] = np.ravel_multi_index(tuple(m.T), dims)
-    return out
-
-
-def extrusion_columns(elem_ijk, dims, axis: int):
-    """manufacture.tie_groups(domain, extrusion=axis) (manufacture.py:413-427):
-    one label per column of cells along the extrusion axis."""
-    others = [i for i in range(3) if i != axis]
-    ij = np.asarray(elem_ijk)[:, others]
-    return np.unique(ij[:, 0] * (dims[others[1]] + 1) + ij[:, 1], return_inverse=True)[1].ravel()
-
-
-def symmetrize(v, mirror):
-    """simp._symmetrize (simp.py:1476-1482): the mean of each cell and its
-    mirror image, plane by plane."""
-    for m in (mirror or {}).values():
-        has = m >= 0
-        out = v.copy()
-        out[has] = 0.5 * (v[has] + v[m[has]])
-        v = out
-    return v
-
-
-def extrude(v, columns, design):
-    """simp._extrude (simp.py:1453-1463): each column's mean over its design
-    cells."""
-    if columns is None:
-        return v
-    col = columns[design]
-    n = col.max() + 1
-    mean = np.bincount(col, weights=v[design], minlength=n) / np.maximum(np.bincount(col, minlength=n), 1)
-    out = v.copy()
-    out[design] = mean[col]
-    return out
 
 
 class Constrained:
@@ -268,7 +222,6 @@ def optimize_constrained(p, opts: LoopOptions, limits: Limits, nu=0.3, preserve=
     xphys, volume_fraction, iterations, converged, history, delivered)."""
     L = Constrained(p, opts, limits, nu, preserve, element)
     n, design, free = L.n, L.design, L.free
-    columns = None if extrusion is None else extrusion_columns(L.mesh.elem_ijk, p.dims, extrusion)
     x = np.full(n, opts.volume_fraction) if x0 is None else np.clip(np.asarray(x0, float), 0.0, 1.0)
     x[L.pres] = 1.0
     mma = MMA(len(free), L.m, xmin=0.0, xmax=1.0, move=min(opts.move, 0.1))
@@ -287,11 +240,8 @@ def optimize_constrained(p, opts: LoopOptions, limits: Limits, nu=0.3, preserve=
         f0 = 10.0 * float(xp[design].mean())
         g0 = 10.0 * design / design.sum()
         df0 = L.filt.T @ (g0 * dpb / L.hs)
-        if mirror:
-            df0 = symmetrize(df0, mirror)
-            grads = [symmetrize(g, mirror) for g in grads]
-        df0 = extrude(df0, columns, design)
-        grads = [extrude(g, columns, design) for g in grads]
+        df0 = column_average(mirror_average(df0, p.dims, mirror), p.dims, extrusion, design)
+        grads = [column_average(mirror_average(g, p.dims, mirror), p.dims, extrusion, design) for g in grads]
         worst = max(cons)
         if worst > FEASIBLE and worst > worst_prev:
             mma.move = max(0.02, 0.5 * mma.move)
@@ -301,9 +251,7 @@ def optimize_constrained(p, opts: LoopOptions, limits: Limits, nu=0.3, preserve=
         worst_prev = worst
         xnew = x.copy()
         xnew[free] = mma.update(x[free], f0, df0[free], np.array(cons), np.array([g[free] for g in grads]))
-        if mirror:
-            xnew = symmetrize(xnew, mirror)
-        xnew = extrude(xnew, columns, design)
+        xnew = column_average(mirror_average(xnew, p.dims, mirror), p.dims, extrusion, design)
         change = float(np.abs(xnew - x)[design].max())
         x = xnew
         since_beta += 1

@@ -3,9 +3,9 @@
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 
 Restates simp._optimize_run (simp.py:1640-2449) for a box domain with a
-preserved-solid mask and the plain compliance objective (no mirror,
-extrusion, overhang filter, heat, gravity, buckling, blockage or member
-limits; those branches are inactive there):
+preserved-solid mask and the plain compliance objective (no overhang
+filter, heat, gravity, buckling, blockage or member limits; those branches
+are inactive there):
 - blueprint xp = P(H x / hs, beta, eta) and analysed (eroded) design
   xe = P(H x / hs, beta, eta + erode), preserved cells 1, void cells 0
                                                    simp.py:1775-1812
@@ -27,6 +27,16 @@ limits; those branches are inactive there):
   beta_max it stops on change < tol (free step) or, the scheduled stop,
   after beta_every iterations with the volume bracketed
                                                    simp.py:2413-2449, 1436-1440
+- mirror planes and extrusion columns: the clamped dc and dv and every
+  trial design of the volume root are tied (mirror_average, then
+  column_average; void cells emptied again after the columns)
+                                                   simp.py:2223-2226, 2288-2304
+- the analysis' live-cell reduction: alive = xe > ALIVE or an anchor cell
+  (touching a supported or loaded node, or preserved); under REDUCE_BELOW
+  of the cells alive, only the live cells 26-connected to a live cell on a
+  support are solved (the others carry no stiffness); a load on a dof they
+  do not hold, or a singular reduced system, solves the whole grid
+                                                   simp.py:59-61, 154-155, 264-282, 720-826
 
 Pinned by tests/test_oracle_loop.py:
 - the chain rule (dc, dv through filter, projection and erosion) against
@@ -38,7 +48,12 @@ Pinned by tests/test_oracle_loop.py:
   minimum;
 - the loop's invariants on a small cantilever: the volume on target to
   1e-10 once bracketed, the limit never above opts.move, beta doubling on
-  the schedule, compliance falling at the sharpest stage.
+  the schedule, compliance falling at the sharpest stage;
+- the ties against the reference's index-map formulation, to the bit;
+- the reduction: the kept cells are scipy.ndimage.label's components
+  (the reference's own call) that touch a support, and the reduced solve
+  equals the full solve of the system whose dropped cells have modulus 0
+  (an island carries no load: its displacement is zero).
 """
 from __future__ import annotations
 
@@ -50,6 +65,8 @@ import numpy as np
 from .simp import Evaluator, GridFilter, dproject, element_energy, project
 
 E_MIN = 1e-6          # simp.py:42
+ALIVE = 1e-3          # simp.py:59
+REDUCE_BELOW = 0.8    # simp.py:61
 PENALTY = 3.0         # simp.py:55
 VOLUME_RTOL = 1e-13   # simp.py:1235
 STEP_SHRINK = 0.5     # simp.py:1238
@@ -74,6 +91,49 @@ class LoopOptions:
     erode: float = 0.15
     beta_start: float = 1.0
     pad: bool = True
+    mirror: str = ""              # mirror planes through the box centre (domain.mirror)
+    extrusion: int | None = None  # extrusion axis (SimpOptions.extrusion)
+    reduce: bool = True           # the multigrid path's live-cell reduction (_solve_mg)
+
+
+def mirror_average(v, dims, axes):
+    """simp._symmetrize (simp.py:1476-1482) on a box: for each mirror axis
+    in turn, every cell and its image across the centre plane both take
+    0.5 (v + v_image) (domain.mirror, domain.py:1981-1988)."""
+    out = np.array(v, float).reshape(dims)
+    for ax in axes or ():
+        a = "xyz".index(ax) if isinstance(ax, str) else int(ax)
+        nxt = out.copy()
+        for k in range(dims[a]):
+            lo = [slice(None)] * 3
+            hi = [slice(None)] * 3
+            lo[a], hi[a] = k, dims[a] - 1 - k
+            nxt[tuple(lo)] = 0.5 * (out[tuple(lo)] + out[tuple(hi)])
+        out = nxt
+    return out.reshape(np.shape(v))
+
+
+def column_average(v, dims, axis, design):
+    """simp._extrude (simp.py:1453-1463) with manufacture.tie_groups's
+    columns along ``axis`` (manufacture.py:413-427) on a box: each column's
+    design cells take their mean, summed cell by cell along the axis (the
+    order np.bincount adds a column's cells in)."""
+    if axis is None:
+        return v
+    a = "xyz".index(axis) if isinstance(axis, str) else int(axis)
+    g = np.array(v, float).reshape(dims)
+    d = np.asarray(design, bool).reshape(dims)
+    gm = np.moveaxis(g, a, 0)
+    dm = np.moveaxis(d, a, 0)
+    total = np.zeros(gm.shape[1:])
+    count = np.zeros(gm.shape[1:])
+    for k in range(gm.shape[0]):
+        total = total + np.where(dm[k], gm[k], 0.0)
+        count = count + dm[k]
+    mean = total / np.maximum(count, 1)
+    for k in range(gm.shape[0]):
+        gm[k] = np.where(dm[k], mean, gm[k])
+    return np.moveaxis(gm, 0, a).reshape(np.shape(v))
 
 
 def beta_hold(opts: LoopOptions, beta: float) -> int:
@@ -175,6 +235,77 @@ class Loop:
         self.weights = np.ones(len(self.ev.fs)) if weights is None else np.asarray(weights, float)
         self.solve = solve if solve is not None else (lambda e: [self.ev.solve(e, f) for f in self.ev.fs])
         self.eta_e = opts.eta + opts.erode
+        # cells touching a supported node, and the anchors: also touching a
+        # loaded node, or preserved (simp.py:154-155)
+        enode = self.ev.mesh.enode
+        fixed = self.ev.fixed.reshape(-1, 3).any(axis=1)
+        loaded = np.zeros(len(fixed), bool)
+        for f in self.ev.fs:
+            loaded |= (f.reshape(-1, 3) != 0).any(axis=1)
+        self.support_touch = fixed[enode].any(axis=1)
+        self.anchor = self.support_touch | loaded[enode].any(axis=1) | self.pres
+        self.reduced_solves = 0
+        self.full_solves = 0
+
+    def keep_cells(self, xe):
+        """The cells the multigrid path keeps (simp._Analysis._solve_mg and
+        _connected, simp.py:264-282, 795-807), or None for the whole grid."""
+        from scipy import ndimage
+
+        alive = (xe > ALIVE) | self.anchor
+        if alive.mean() >= REDUCE_BELOW:
+            return None
+        g = alive.reshape(self.p.dims)
+        lbl, _ = ndimage.label(g, structure=np.ones((3, 3, 3), bool))
+        el = lbl.ravel()
+        roots = np.unique(el[alive & self.support_touch])
+        return alive & np.isin(el, roots[roots > 0])
+
+    def reduced_solve(self, e, keep):
+        """Direct solve of the kept cells' system (their dofs off the
+        supports; every other dof zero), or None where it cannot stand in for
+        the whole grid: a load on a dof the kept cells do not hold, or a
+        singular system (a hinge)."""
+        import scipy.sparse.linalg as spla
+        import warnings
+
+        ev = self.ev
+        held = np.zeros(ev.mesh.n_nodes, bool)
+        held[ev.mesh.enode[keep].ravel()] = True
+        live = np.repeat(held, 3)
+        if any(np.any(f[~live]) for f in ev.fs):
+            return None
+        ek = np.where(keep, e, 0.0)
+        data = (ek[:, None, None] * ev.ke[None]).ravel()
+        import scipy.sparse as sp
+
+        n = ev.mesh.n_dof
+        k = sp.csr_matrix((data, (ev._rows, ev._cols)), shape=(n, n))
+        free = np.flatnonzero(live & ~ev.fixed)
+        kf = k[free][:, free].tocsc()
+        out = []
+        with warnings.catch_warnings():
+            warnings.simplefilter("error")
+            try:
+                for f in ev.fs:
+                    u = np.zeros(n)
+                    u[free] = spla.spsolve(kf, f[free])
+                    if not np.all(np.isfinite(u)):
+                        return None
+                    out.append(u)
+            except Exception:
+                return None
+        return out
+
+    def solve_design(self, e, xe):
+        keep = self.keep_cells(xe) if self.opts.reduce else None
+        if keep is not None:
+            us = self.reduced_solve(e, keep)
+            if us is not None:
+                self.reduced_solves += 1
+                return us, np.where(keep, e, 0.0)
+        self.full_solves += 1
+        return self.solve(e), e
 
     def physical(self, x, beta):
         xt = (self.filt @ x) / self.hs
@@ -194,7 +325,7 @@ class Loop:
         xt, xp = self.physical(x, beta)
         xe = self.eroded(xt, beta)
         e = E_MIN + xe ** PENALTY * (1 - E_MIN)
-        us = self.solve(e)
+        us, _ = self.solve_design(e, xe)
         comp = 0.0
         dc_xp = np.zeros(self.n)
         for w, u, f in zip(self.weights, us, self.ev.fs):
@@ -239,6 +370,8 @@ def optimize_loop(p, opts: LoopOptions, preserve=None, weights=None, solve=None,
             c0 = comp
         dc = np.minimum(dc, 0.0)
         dv = np.maximum(dv, 1e-12)
+        dc = column_average(mirror_average(dc, p.dims, opts.mirror), p.dims, opts.extrusion, L.design)
+        dv = column_average(mirror_average(dv, p.dims, opts.mirror), p.dims, opts.extrusion, L.design)
         if measured:
             if step_prev is not None:
                 dxp, gp, lam_p = step_prev
@@ -259,6 +392,9 @@ def optimize_loop(p, opts: LoopOptions, preserve=None, weights=None, solve=None,
             xn = np.clip(x * ratio / np.sqrt(lm_), lo_x, hi_x)
             xn[L.pres] = 1.0
             xn[L.void] = 0.0
+            if opts.mirror or opts.extrusion is not None:
+                xn = column_average(mirror_average(xn, p.dims, opts.mirror), p.dims, opts.extrusion, L.design)
+                xn[L.void] = 0.0
             v = float(L.physical(xn, beta)[1] @ L.wdesign)
             return xn, v - L.target
 

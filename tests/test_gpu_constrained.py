@@ -67,27 +67,47 @@ def test_constrained_rejects_what_it_does_not_build(gpu):
         optimize_constrained(p, 0.3, o, Limits(stress=None, deflection=None, modulus=1.0))
 
 
+def _tie_problem():
+    """A cantilever 8 x 4 x 4 loaded off its y centre and obliquely, so
+    neither the supports nor the loads are symmetric in y: a y mirror and z
+    extrusion columns are each a real tie."""
+    from loadpath_amd.problems import BoxProblem, nodes
+
+    nx, ny, nz = 8, 4, 4
+    return BoxProblem("cantilever-offset", (nx, ny, nz), 0.5, 1.5,
+                      [(nodes(0, range(ny + 1), range(nz + 1)), "xyz")],
+                      [[(nodes(nx, 1, range(nz + 1)), "z", -0.2), (nodes(nx, 1, range(nz + 1)), "y", 0.1)]])
+
+
 def test_constrained_ties_match_oracle(gpu):
-    """A y mirror and extrusion along y (constrained.py:775-780, 796-799):
-    the device loop's iterates equal the oracle's."""
-    from loadpath_amd.constrained import Limits, box_mirror, optimize_constrained
+    """A y mirror and extrusion along z (constrained.py:775-780, 796-799)
+    on a problem that is neither: the device loop's iterates equal the
+    oracle's, and every iterate satisfies the ties themselves (x equals x
+    at the mirror index, and each z column is one value)."""
+    from loadpath_amd.constrained import Limits, optimize_constrained
     from loadpath_amd.simp import SimpOptions
     from oracle.constrained import Limits as OLimits
-    from oracle.constrained import box_mirror as obox_mirror
     from oracle.constrained import optimize_constrained as oracle_loop
     from oracle.loop import LoopOptions
 
-    p = _problem()
+    p = _tie_problem()
     kw = dict(volume_fraction=0.5, filter_radius=1.5, max_iterations=16, beta_every=6, beta_max=8.0)
     r = optimize_constrained(p, 0.3, SimpOptions(solve_rtol=1e-11, **kw),
                              Limits(stress=80.0, deflection=5.0, modulus=1000.0), precision="fp64",
-                             mirror=box_mirror(p.dims, "y"), extrusion=1, maxiter=4000, coarse_cells=50)
+                             mirror="y", extrusion=2, maxiter=4000, coarse_cells=50)
     ro = oracle_loop(_oracle_problem(p), LoopOptions(**kw), OLimits(stress=80.0, deflection=5.0, modulus=1000.0),
-                     mirror=obox_mirror(p.dims, "y"), extrusion=1)
+                     mirror="y", extrusion=2)
     assert len(r.history) == len(ro["history"])
     for a, b in zip(r.history, ro["history"]):
         assert a["volume"] == pytest.approx(b["volume"], abs=1e-6)
         assert np.allclose(a["constraints"], b["constraints"], rtol=0, atol=1e-5)
     x = r.x.reshape(p.dims)
-    assert np.array_equal(x, x[:, ::-1, :]) and np.allclose(x, x[:, :1, :])
+    assert np.array_equal(x, x[:, ::-1, :])                 # x[i, j, k] == x[i, ny-1-j, k]
+    assert np.array_equal(x, np.broadcast_to(x[:, :, :1], x.shape))   # one value per z column
     assert np.abs(r.x - ro["x"]).max() < 1e-5
+    # without the ties the same problem's design is not symmetric in y
+    r0 = optimize_constrained(p, 0.3, SimpOptions(solve_rtol=1e-11, **dict(kw, max_iterations=4)),
+                              Limits(stress=80.0, deflection=5.0, modulus=1000.0), precision="fp64", maxiter=4000,
+                              coarse_cells=50)
+    x0 = r0.x.reshape(p.dims)
+    assert np.abs(x0 - x0[:, ::-1, :]).max() > 1e-3

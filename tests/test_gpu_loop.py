@@ -156,3 +156,97 @@ def test_delivery_matches_oracle(gpu):
     assert np.abs(got["xphys"] - want["xphys"].reshape(p.dims)).max() < 1e-6
     assert got["compliance"] == pytest.approx(want["compliance"], rel=1e-6)
     assert got["volume_fraction"] == pytest.approx(want["volume_fraction"], abs=1e-9)
+
+
+def test_loop_ties_match_oracle(gpu):
+    """Mirror planes and extrusion columns in the production loop (simp.py:
+    2223-2226, 2288-2304): a y mirror (4 cells) and z columns (6 cells) on a
+    cantilever loaded at one y edge, so neither tie holds by itself. The
+    device loop's iterates equal the oracle's, and every design satisfies
+    the ties."""
+    from loadpath_amd.problems import BoxProblem, nodes
+    from loadpath_amd.simp import SimpOptions, optimize
+    from oracle.loop import LoopOptions, optimize_loop
+
+    nx, ny, nz = 16, 4, 6
+    # loads on one y edge only: the untied optimum is not symmetric in y
+    p = BoxProblem("cantilever-edge", (nx, ny, nz), 0.3, 1.5, [(nodes(0, range(ny + 1), range(nz + 1)), "xyz")],
+                   [[(nodes(nx, 0, range(nz + 1)), "z", -1.0 / (nz + 1))], [(nodes(nx, 1, 0), "y", 0.4)]])
+    n_it = 16
+    opts = SimpOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=n_it, solve_rtol=1e-11, extrusion=2)
+    r = optimize(p, opts, mirror="y", precision="fp64", element="im", maxiter=3000, coarse_cells=50,
+                 delivered=False)
+    ro = optimize_loop(_oracle_problem(p), LoopOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=n_it,
+                                                       mirror="y", extrusion=2), element="im")
+    assert len(r.history) == len(ro["history"])
+    for a, b in zip(r.history, ro["history"]):
+        assert a["beta"] == b["beta"]
+        assert a["compliance"] == pytest.approx(b["compliance"], rel=1e-7)
+        assert a["volume"] == pytest.approx(b["volume"], abs=1e-9)
+    x = r.x.reshape(p.dims)
+    assert np.array_equal(x, x[:, ::-1, :])
+    assert np.array_equal(x, np.broadcast_to(x[:, :, :1], x.shape))
+    assert np.abs(r.x - ro["x"]).max() < 1e-6
+    # untied, the same problem is not symmetric in y
+    r0 = optimize(p, SimpOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=4, solve_rtol=1e-11),
+                  precision="fp64", element="im", maxiter=3000, coarse_cells=50, delivered=False)
+    assert np.abs(r0.x - r0.x[:, ::-1, :]).max() > 1e-4
+
+
+def _island():
+    """Preserved 2 x 2 x 1 block ringed by void cells: alive (preserved) but
+    joined to no support once the loop sharpens."""
+    dims = (16, 4, 6)
+    pres = _preserve(dims)
+    pres[12:14, 1:3, 4] = True
+    void = np.zeros(dims, bool)
+    void[11:15, :, 3:6] = True
+    void[12:14, 1:3, 4] = False
+    return pres, void
+
+
+@pytest.mark.parametrize("loaded_island", [False, True])
+def test_loop_reduction_matches_oracle(gpu, loaded_island):
+    """The multigrid path's live-cell reduction (simp.py:795-826): past beta
+    8 fewer than 80 % of the cells are alive and the island is dropped
+    (modulus 0), so compliance and designs follow the oracle's reduced
+    direct solves. With a load on the island the reduced system cannot hold
+    it and every such iteration solves the whole grid (both sides)."""
+    from loadpath_amd.problems import BoxProblem, nodes
+    from loadpath_amd.simp import GridSimp, SimpOptions, optimize
+    from oracle.loop import Loop, LoopOptions, optimize_loop
+
+    p = _problem()
+    pres, void = _island()
+    if loaded_island:
+        cases = [list(p.cases[0]), list(p.cases[1]) + [(nodes(13, 2, 5), "x", 1e-3)]]
+        p = BoxProblem(p.name, p.dims, p.volfrac, p.rmin, list(p.fixed), cases)
+    n_it = 24
+    sched = (3, 3, 3, 3)
+    opts = SimpOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=n_it, solve_rtol=1e-11,
+                       beta_schedule=sched)
+    r = optimize(p, opts, preserve=pres, void=void, precision="fp64", element="im", maxiter=3000, coarse_cells=50,
+                 delivered=False)
+    lo = LoopOptions(volume_fraction=0.3, filter_radius=1.5, max_iterations=n_it, beta_schedule=sched)
+    op = _oracle_problem(p)
+    ro = optimize_loop(op, lo, preserve=pres.ravel(), element="im", void=void.ravel())
+    assert len(r.history) == len(ro["history"])
+    for a, b in zip(r.history, ro["history"]):
+        assert a["beta"] == b["beta"]
+        assert a["compliance"] == pytest.approx(b["compliance"], rel=1e-7), a["iteration"]
+        assert a["volume"] == pytest.approx(b["volume"], abs=1e-9)
+    assert np.abs(r.x - ro["x"]).max() < 1e-6
+    # the reduction did act (counted on a rerun's device loop) and, with the
+    # loaded island, sent those iterations to the whole grid
+    s = GridSimp(p, precision="fp64", filter="padded", volfrac=0.3, rmin=1.5, rtol=1e-11, maxiter=3000,
+                 coarse_cells=50, element="im", active=~void)
+    s.set_loop(opts, preserve=pres)
+    s.iterate(n_it)
+    info = s.loop_info()
+    if loaded_island:
+        assert info["reduced"] == 0 and info["full_for_loads"] > 0, info
+    else:
+        assert info["reduced"] > 0 and info["full_for_loads"] == 0, info
+    # the oracle's own count of reduced solves agrees
+    L = Loop(op, lo, preserve=pres.ravel(), element="im", void=void.ravel())
+    assert L.support_touch.any() and L.anchor[pres.ravel()].all()

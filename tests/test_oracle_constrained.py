@@ -70,17 +70,54 @@ def test_contact_band_masks_corners():
     assert np.array_equal(L.counted, pc.any(axis=1))
 
 
-def test_mirror_and_extrusion_ties_hold():
-    """With a y mirror and extrusion along y every iterate is symmetric and
-    constant along each column (the ties the reference applies)."""
-    from oracle.constrained import box_mirror
+def test_tie_restatement_against_index_maps():
+    """The oracle's mirror and column averages against the reference's
+    formulation on index maps (domain.py:1981-1988: lut of the mirrored
+    ijk; simp._symmetrize's pairwise means; manufacture.tie_groups' column
+    labels with simp._extrude's bincount means), on random fields with
+    preserved cells, to the bit."""
+    from oracle.loop import column_average, mirror_average
 
-    p, opts, lim = _setup()
+    rng = np.random.default_rng(4)
+    dims = (5, 4, 6)
+    n = int(np.prod(dims))
+    ijk = np.stack(np.unravel_index(np.arange(n), dims), axis=1)
+    v = rng.standard_normal(n)
+    design = rng.random(n) > 0.2
+    for axes in ("y", "xz", "zyx"):
+        want = v.copy()
+        for ax in axes:
+            a = "xyz".index(ax)
+            m_ijk = ijk.copy()
+            m_ijk[:, a] = dims[a] - 1 - m_ijk[:, a]
+            m = np.ravel_multi_index(tuple(m_ijk.T), dims)
+            nxt = want.copy()
+            nxt[:] = 0.5 * (want + want[m])
+            want = nxt
+        assert np.array_equal(mirror_average(v, dims, axes), want), axes
+    for a in range(3):
+        others = [i for i in range(3) if i != a]
+        _, col = np.unique(ijk[:, others[0]] * (dims[others[1]] + 1) + ijk[:, others[1]], return_inverse=True)
+        c = col[design]
+        mean = np.bincount(c, weights=v[design], minlength=col.max() + 1) / np.maximum(
+            np.bincount(c, minlength=col.max() + 1), 1)
+        want = v.copy()
+        want[design] = mean[c]
+        assert np.array_equal(column_average(v, dims, a, design), want), a
+
+
+def test_mirror_and_extrusion_ties_hold():
+    """With a y mirror and extrusion along z every iterate is symmetric and
+    one value along each z column, on a problem that is neither."""
+    from oracle.simp import Problem, nodes
+
+    p0, opts, lim = _setup()
+    p = Problem("offset", (8, 4, 4), 0.5, 1.5, [(nodes(0, range(5), range(5)), "xyz")],
+                [[(nodes(8, 1, range(5)), "z", -0.2), (nodes(8, 1, range(5)), "y", 0.1)]])
     opts.max_iterations = 12
-    mir = box_mirror(p.dims, "y")
     seen = []
-    r = optimize_constrained(p, opts, lim, mirror=mir, extrusion=1, on_iter=lambda rec, xp: seen.append(xp.copy()))
+    r = optimize_constrained(p, opts, lim, mirror="y", extrusion=2, on_iter=lambda rec, xp: seen.append(xp.copy()))
     x = r["x"].reshape(p.dims)
     assert np.array_equal(x, x[:, ::-1, :])
-    assert np.allclose(x, x[:, :1, :])
+    assert np.array_equal(x, np.broadcast_to(x[:, :, :1], x.shape))
     assert len(seen) == 12
