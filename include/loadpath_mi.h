@@ -121,6 +121,11 @@ int lp_solve(lp_grid* g, int nrhs, const double* b, double* x, int warm, int dev
  * (femops.py:236-242): se (nrhs, nx, ny, nz) = u_c^T Ke u_c. */
 int lp_strain_energy(lp_grid* g, int nrhs, const double* u, double* se, int device_ptr);
 
+/* b = f masked to the free live dofs of the current hierarchy (mg.py:938-
+ * 939: b = to_grid(f) * levels[0].mask), nrhs node-grid fields: the
+ * right-hand side lp_solve takes, formed on the device. */
+int lp_mask_rhs(lp_grid* g, int nrhs, const double* f, double* b, int device_ptr);
+
 /* The bilinear form a_c^T Ke b_c per cell (the adjoint term of
  * constrained.py:659, lambda_e^T Ke u_e): out (nrhs, nx, ny, nz). */
 int lp_element_product(lp_grid* g, int nrhs, const double* a, const double* b, double* out, int device_ptr);
@@ -233,6 +238,10 @@ typedef struct lp_loop_opts {
  * the iteration after which the loop stops. */
 int lp_simp_set_loop(lp_simp* s, const lp_loop_opts* o, const uint8_t* preserve, const double* case_weights);
 
+/* SIMP moduli and their derivative (simp.py:1861, 1897-1904): e = emin +
+ * x^penal (1 - emin), de = penal x^(penal-1) (1 - emin) (de may be NULL). */
+int lp_simp_moduli(long long n, const double* x, double penal, double emin, double* e, double* de, int device_ptr);
+
 /* The production loop's design ties (simp.py:1650, 2223-2226, 2288-2289,
  * 2296-2304): mirror planes through the box centre across the given axes
  * (0 x, 1 y, 2 z; domain.mirror, applied in this order as
@@ -271,6 +280,12 @@ int lp_simp_displacements(lp_simp* s, double* out);
  * grid, zero outside), simp.GridFilter.__matmul__ (simp.py:1164-1172). */
 int lp_filter_apply(int nx, int ny, int nz, double radius, int reach, const double* v, double* out,
                     int device_ptr);
+/* simp.GridFilter(domain, radius) @ v (simp.py:1148-1172) on the domain's
+ * box of nx x ny x nz cells of side pitch: reach floor(radius / pitch),
+ * weights radius - |d| pitch, summed in (dx, dy, dz) order, zero outside
+ * the box; cells outside the mesh carry v = 0 (the caller scatters). */
+int lp_grid_filter(int nx, int ny, int nz, double pitch, double radius, const double* v, double* out,
+                   int device_ptr);
 
 /* verify.nodal_stress / nodal_von_mises (verify.py:197-206, 307-337) on a
  * voxel grid of nx x ny x nz cells: each present cell's stress at its 8
@@ -317,6 +332,35 @@ int lp_stress_averaged(int nx, int ny, int nz, double pitch, double nu, int elem
                        const double* u, const double* xe, const uint8_t* readable, double p, double q,
                        double limit, const double* bond, double* kappa, int it, double* pm, double* smax,
                        double* rhs, double* gx);
+
+/* ---- kernel-level switches: mgops / femops on the device --------------- */
+/* mgops.apply_masked(kind, u, a, b, cells, m, out, finish) (mgops.py:247-
+ * 261): out = K (u m) over a level's cells, and with finish out = out m +
+ * (1 - m) u. kind 0 "scaled" (a = Ke (24, 24), b = e (n_active)), 1
+ * "weights" (a = M_q (8, 24, 24), b = w (8, n_active)), 2 "explicit" (a =
+ * K (24, 24, n_active), b unused); rows (nx, ny, nz) int64 the Cells.rows
+ * map (a cell's column, or -1); u, m, out (3, nx+1, ny+1, nz+1); m NULL:
+ * no mask (finish needs one). dtype_bytes 4 or 8 for every field. */
+int lp_cells_apply(int kind, int nx, int ny, int nz, const long long* rows, long long n_active, const void* a,
+                   const void* b, const void* u, const void* m, void* out, int finish, int dtype_bytes);
+/* mgops.coarse_residual(b, ax, m, cm, coarse_shape) (mgops.py:753-763):
+ * rc (3, cx, cy, cz) = restrict(((b - ax) m) padded to (3, 2cx-1, 2cy-1,
+ * 2cz-1)) cm; b, ax, m (3, fx, fy, fz). */
+int lp_coarse_residual(const void* b, const void* ax, const void* m, const void* cm, void* rc, int fx, int fy, int fz,
+                       int cx, int cy, int cz, int dtype_bytes);
+/* mgops.add_correction(xc, cm, m, x) (mgops.py:766-773): x (3, sx, sy, sz)
+ * += prolong(xc cm)[:, :sx, :sy, :sz] m; xc, cm (3, cx, cy, cz). */
+int lp_add_correction(const void* xc, const void* cm, const void* m, void* x, int cx, int cy, int cz, int sx, int sy,
+                      int sz, int dtype_bytes);
+/* mgops.dot(a, b) (mgops.py:578-585): sum a b over n values, in double. */
+int lp_dot(const void* a, const void* b, long long n, int dtype_bytes, double* out);
+/* mgops.cast(a, dtype) (mgops.py:564-575): out = a converted, n values. */
+int lp_cast(const void* a, int a_bytes, void* out, int out_bytes, long long n);
+/* femops.element_energy(u, v, edof, ke) (femops.py:236-242): out (ne) =
+ * sum_j sum_k (u_j ke_jk) v_k over each element's 24 dofs edof (ne, 24)
+ * int64 (j outer, k inner, the reference's order); v NULL: u. */
+int lp_element_energy(const double* u, const double* v, const long long* edof, long long ne, long long ndof,
+                      const double* ke, double* out);
 
 /* ---- the assembled route: fem.Stiffness (fem.py:377-526) ---------------- */
 /* The CSR stiffness of the voxel mesh of the present cells of an nx x ny x

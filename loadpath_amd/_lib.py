@@ -117,6 +117,15 @@ _SIGS = {
     "lp_label26": (_I, [_I, _I, _I, _P, _P, _P]),
     "lp_label26_noalloc": (_I, [_I, _I, _I, _P, _P, _P]),
     "lp_simp_set_ties": (_I, [_P, _P, _I, _I]),
+    "lp_grid_filter": (_I, [_I, _I, _I, _D, _D, _P, _P, _I]),
+    "lp_simp_moduli": (_I, [C.c_longlong, _P, _D, _D, _P, _P, _I]),
+    "lp_mask_rhs": (_I, [_P, _I, _P, _P, _I]),
+    "lp_cells_apply": (_I, [_I, _I, _I, _I, _P, C.c_longlong, _P, _P, _P, _P, _P, _I, _I]),
+    "lp_coarse_residual": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I]),
+    "lp_add_correction": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I]),
+    "lp_dot": (_I, [_P, _P, C.c_longlong, _I, _P]),
+    "lp_cast": (_I, [_P, _I, _P, _I, C.c_longlong]),
+    "lp_element_energy": (_I, [_P, _P, _P, C.c_longlong, C.c_longlong, _P, _P]),
     "lp_simp_set_reduction": (_I, [_P, _P, _P]),
     "lp_simp_loop_info": (_I, [_P, _P, _P, _P]),
     "lp_set_solve_tol": (_I, [_P, _D, _I]),

@@ -66,6 +66,16 @@ __global__ void k_moduli(const double* __restrict__ xphys, double* e, size_t n, 
     e[i] = emin + pow(xphys[i], penal) * (1.0 - emin);
 }
 
+// e = emin + x^p (1 - emin) and de = de/dx = p x^(p-1) (1 - emin)
+__global__ void k_moduli_de(const double* __restrict__ x, double* e, double* de, size_t n, double penal,
+                            double emin) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi = x[i];
+    e[i] = emin + pow(xi, penal) * (1.0 - emin);
+    if (de) de[i] = penal * pow(xi, penal - 1.0) * (1.0 - emin);
+}
+
 // b = loads * mask
 __global__ void k_mask_loads(const double* __restrict__ f, const uint8_t* __restrict__ mask, double* b, size_t nn,
                              size_t stride) {
@@ -510,12 +520,15 @@ __global__ void k_inv(const double* __restrict__ a, double* b, size_t n) {
     if (i < n) b[i] = 1.0 / a[i];
 }
 
-std::vector<Tap> make_taps(double radius, int reach) {
+// cone weights w = radius - |d| pitch > 0 for offsets |d_i| <= reach, in
+// (dx, dy, dz) order (simp.GridFilter, simp.py:1152-1159)
+std::vector<Tap> make_taps(double radius, int reach, double pitch = 1.0) {
     std::vector<Tap> taps;
     for (int dx = -reach; dx <= reach; ++dx)
         for (int dy = -reach; dy <= reach; ++dy)
             for (int dz = -reach; dz <= reach; ++dz) {
-                const double w = radius - std::sqrt((double)(dx * dx + dy * dy + dz * dz));
+                const double dist = std::sqrt((double)(dx * dx + dy * dy + dz * dz));
+                const double w = radius - (pitch == 1.0 ? dist : dist * pitch);
                 if (w > 0) taps.push_back(Tap{dx, dy, dz, 0, w});
             }
     return taps;
@@ -1261,6 +1274,38 @@ int lp_simp_set_loop(lp_simp* S, const lp_loop_opts* o, const uint8_t* preserve,
     });
 }
 
+int lp_simp_moduli(long long n, const double* x, double penal, double emin, double* e, double* de, int device_ptr) {
+    return guarded_s([&]() -> int {
+        if (n < 0 || !x || !e) return LP_ERR_ARG;
+        if (n == 0) return LP_OK;
+        DBuf dx, dee, dd;
+        const double* px = x;
+        double* pe = e;
+        double* pd = de;
+        if (!device_ptr) {
+            dx.alloc(n * 8);
+            dee.alloc(n * 8);
+            LP_HIP(hipMemcpy(dx.p, x, n * 8, hipMemcpyHostToDevice));
+            px = dx.as<double>();
+            pe = dee.as<double>();
+            if (de) {
+                dd.alloc(n * 8);
+                pd = dd.as<double>();
+            }
+        }
+        hipLaunchKernelGGL(k_moduli_de, dim3(nblk((size_t)n)), dim3(kBlock), 0, nullptr, px, pe, pd, (size_t)n, penal,
+                           emin);
+        LP_HIP(hipGetLastError());
+        if (!device_ptr) {
+            LP_HIP(hipMemcpy(e, pe, n * 8, hipMemcpyDeviceToHost));
+            if (de) LP_HIP(hipMemcpy(de, pd, n * 8, hipMemcpyDeviceToHost));
+        } else {
+            LP_HIP(hipDeviceSynchronize());
+        }
+        return LP_OK;
+    });
+}
+
 int lp_simp_set_ties(lp_simp* S, const int* mirror_axes, int n_mirror, int extrusion_axis) {
     return guarded_s([&]() -> int {
         if (!S || n_mirror < 0 || n_mirror > 3 || (n_mirror > 0 && !mirror_axes) || extrusion_axis < -1 ||
@@ -1369,10 +1414,27 @@ int lp_simp_displacements(lp_simp* S, double* out) {
     });
 }
 
+static int filter_apply(int nx, int ny, int nz, const std::vector<Tap>& taps, const double* v, double* out,
+                        int device_ptr);
+
 int lp_filter_apply(int nx, int ny, int nz, double radius, int reach, const double* v, double* out, int device_ptr) {
     return guarded_s([&]() -> int {
         if (nx < 1 || ny < 1 || nz < 1 || !v || !out || reach < 0) return LP_ERR_ARG;
-        std::vector<Tap> taps = make_taps(radius, reach);
+        return filter_apply(nx, ny, nz, make_taps(radius, reach), v, out, device_ptr);
+    });
+}
+
+int lp_grid_filter(int nx, int ny, int nz, double pitch, double radius, const double* v, double* out,
+                   int device_ptr) {
+    return guarded_s([&]() -> int {
+        if (nx < 1 || ny < 1 || nz < 1 || !v || !out || !(pitch > 0) || !(radius >= 0)) return LP_ERR_ARG;
+        const int reach = (int)std::floor(radius / pitch);
+        return filter_apply(nx, ny, nz, make_taps(radius, reach, pitch), v, out, device_ptr);
+    });
+}
+
+static int filter_apply(int nx, int ny, int nz, const std::vector<Tap>& taps, const double* v, double* out,
+                        int device_ptr) {
         const size_t n = (size_t)nx * ny * nz;
         DBuf t, a, b;
         t.alloc(std::max<size_t>(1, taps.size()) * sizeof(Tap));
@@ -1393,7 +1455,6 @@ int lp_filter_apply(int nx, int ny, int nz, double radius, int reach, const doub
         if (!device_ptr) LP_HIP(hipMemcpy(out, dout, n * 8, hipMemcpyDeviceToHost));
         else LP_HIP(hipDeviceSynchronize());
         return LP_OK;
-    });
 }
 
 }  // extern "C"

@@ -902,6 +902,30 @@ int lp_solve(lp_grid* g, int nrhs, const double* b, double* x, int warm, int dev
     });
 }
 
+__global__ void k_mask_rhs(const double* __restrict__ f, const uint8_t* __restrict__ mask, double* b, size_t nn) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int rhs = blockIdx.y;
+    if (i >= 3 * nn) return;
+    const int c = (int)(i / nn);
+    b[rhs * 3 * nn + i] = ((mask[i % nn] >> c) & 1) ? f[rhs * 3 * nn + i] : 0.0;
+}
+
+int lp_mask_rhs(lp_grid* g, int nrhs, const double* f, double* b, int device_ptr) {
+    return guarded([&]() -> int {
+        if (!g || !g->built || nrhs < 1 || !f || !b) return LP_ERR_ARG;
+        Grid& G = *g;
+        const size_t n = 3 * G.g0.nn;
+        const double* df = static_cast<const double*>(to_device(G, G.stage_a, f, n * nrhs * 8, device_ptr));
+        double* db = device_ptr ? b : (G.stage_b.alloc(n * nrhs * 8), G.stage_b.as<double>());
+        hipLaunchKernelGGL(k_mask_rhs, dim3(nblk(n), nrhs), dim3(kBlock), 0, G.stream, df, G.lv[0].mask.as<uint8_t>(),
+                           db, G.g0.nn);
+        LP_HIP(hipGetLastError());
+        if (!device_ptr) from_device(G, b, db, n * nrhs * 8);
+        else LP_HIP(hipStreamSynchronize(G.stream));
+        return LP_OK;
+    });
+}
+
 int lp_strain_energy(lp_grid* g, int nrhs, const double* u, double* se, int device_ptr) {
     return guarded([&]() -> int {
         if (!g || nrhs < 1 || !u || !se) return LP_ERR_ARG;

@@ -166,6 +166,61 @@ class GridElasticity:
         _lib.check(lib.lp_set_moduli(self._h, _lib.ptr(eg), 2 if cheb else 0), "set_moduli")
         self._refresh(lmax=cheb)
 
+    @property
+    def cell_order_is_grid(self) -> bool:
+        """The mesh's cells are the padded cell grid in C order (a full box
+        with even dims): element vectors and cell-grid fields coincide, so
+        device tensors pass through unchanged."""
+        if not hasattr(self, "_grid_order"):
+            n = int(np.prod(self.cells))
+            ijk = self.elem_ijk
+            self._grid_order = len(ijk) == n and bool(np.array_equal(
+                np.ravel_multi_index(tuple(ijk.T), self.cells), np.arange(n)))
+        return self._grid_order
+
+    def set_moduli_device(self, e):
+        """set_moduli from a CUDA float64 tensor of the padded cell grid's
+        moduli (C order); the hierarchy is rebuilt from device memory."""
+        import torch
+
+        e = e.detach().to(torch.float64).contiguous()
+        if not e.is_cuda or e.numel() != int(np.prod(self.cells)):
+            raise ValueError("set_moduli_device: a CUDA tensor over the padded cell grid")
+        self._e = None
+        torch.cuda.synchronize()
+        cheb = self.smoother == "chebyshev"
+        _lib.check(_lib.lib().lp_set_moduli(self._h, C.c_void_p(e.data_ptr()), 1 | (2 if cheb else 0)),
+                   "set_moduli")
+        self._refresh(lmax=cheb)
+
+    def solve_device(self, f_grid, x0=None):
+        """Batched solve of CUDA node-grid tensors (nrhs, 3, nodes...):
+        masked on the device (lp_mask_rhs), solved in place of x0 (or from
+        zero). Returns (x, iterations, converged) with x a CUDA tensor."""
+        import torch
+
+        f = f_grid.detach().to(torch.float64).contiguous()
+        nrhs = f.shape[0]
+        b = torch.empty_like(f)
+        x = torch.zeros_like(f) if x0 is None else x0.detach().to(torch.float64).contiguous().clone()
+        torch.cuda.synchronize()
+        lib = _lib.lib()
+        _lib.check(lib.lp_mask_rhs(self._h, nrhs, C.c_void_p(f.data_ptr()), C.c_void_p(b.data_ptr()), 1), "mask_rhs")
+        if x0 is not None:
+            _lib.check(lib.lp_mask_rhs(self._h, nrhs, C.c_void_p(x.data_ptr()), C.c_void_p(x.data_ptr()), 1),
+                       "mask_rhs")
+        its = np.zeros(nrhs, np.int32)
+        ok = np.zeros(nrhs, np.int32)
+        rc = lib.lp_solve(self._h, nrhs, C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), int(x0 is not None), 1,
+                          _lib.ptr(its), _lib.ptr(ok))
+        if rc not in (_lib.LP_OK, _lib.LP_NOT_CONVERGED):
+            _lib.check(rc, "solve")
+        # the host path's x * mask, kept on the device
+        _lib.check(lib.lp_mask_rhs(self._h, nrhs, C.c_void_p(x.data_ptr()), C.c_void_p(x.data_ptr()), 1), "mask_rhs")
+        self._refresh(lmax=False)
+        self.last_iters = int(its.max())
+        return x, its, ok, b
+
     def _refresh(self, lmax: bool):
         lib = _lib.lib()
         nl = C.c_int()

@@ -6,7 +6,7 @@ Mirrors:
   :214-257): SIMP p = 3, Emin 1e-9, density filter H/Hs, OC bisection with
   l2 = 1e9 and relative tolerance 1e-3, stop when change <= 0.01 or 200
   iterations, returning ``x, iterations, compliance_reported, seconds``;
-- ``density_filter(dims, radius, pad)`` / ``GridFilter`` (simp.py:1143-1189):
+- ``density_filter(domain, radius, pad)`` / ``GridFilter(domain, radius)`` (simp.py:1143-1189):
   ``h @ v``, ``h.T``, ``hs``, on the GPU.
 
 - the production loop ``simp.optimize`` (simp.py:1570-1640, its loop
@@ -29,36 +29,70 @@ from . import _lib
 from .fem import hex8_element, hex8_ke
 
 
-def cone_taps(radius: float, reach: int):
+def cone_taps(radius: float, reach: int, pitch: float = 1.0):
+    """[((dx, dy, dz), w)], w = radius - |d| pitch > 0, |d_i| <= reach."""
     taps = []
     for dx in range(-reach, reach + 1):
         for dy in range(-reach, reach + 1):
             for dz in range(-reach, reach + 1):
-                w = radius - np.sqrt(dx * dx + dy * dy + dz * dz)
+                d = np.sqrt(dx * dx + dy * dy + dz * dz)
+                w = radius - (d if pitch == 1.0 else d * pitch)
                 if w > 0:
                     taps.append(((dx, dy, dz), float(w)))
     return taps
 
 
 class GridFilter:
-    """simp.GridFilter on a full box (unit pitch): ``h @ v`` runs the cone
-    stencil on the GPU; symmetric, so ``h.T is h``."""
+    """simp.GridFilter(domain, radius) (simp.py:1143-1179): the cone
+    filter's unnormalised, symmetric matrix H as a stencil on the domain's
+    box, ``h @ v`` on the GPU (lp_grid_filter). domain: an object with
+    ``mesh.elem_ijk`` (the mesh's cells), ``dims`` (the box) and ``pitch``,
+    as loadpath's Domain; or a dims tuple for a full box of unit cells.
+    Reach floor(radius / pitch), weights radius - |d| pitch; cells outside
+    the mesh count as empty. ``h.T is h``; ``h.sum()`` the row sums."""
 
-    def __init__(self, dims, radius: float, reach: int | None = None):
-        self.dims = tuple(int(d) for d in dims)
+    def __init__(self, domain, radius: float, reach: int | None = None):
+        if hasattr(domain, "mesh"):
+            self.dims = tuple(int(d) for d in domain.dims)
+            self.pitch = float(domain.pitch)
+            ijk = np.asarray(domain.mesh.elem_ijk, np.int64)
+        else:
+            self.dims = tuple(int(d) for d in domain)
+            self.pitch = 1.0
+            ijk = None
         self.radius = float(radius)
-        self.reach = int(np.floor(radius)) if reach is None else int(reach)
-        self.taps = cone_taps(self.radius, self.reach)
-        self.n = int(np.prod(self.dims))
+        if reach is not None and self.pitch != 1.0:
+            raise ValueError("an explicit reach is for unit cells")
+        self.reach = int(np.floor(self.radius / self.pitch)) if reach is None else int(reach)
+        self.taps = cone_taps(self.radius, self.reach, self.pitch)
+        box = int(np.prod(self.dims))
+        if ijk is None or (len(ijk) == box and np.array_equal(
+                np.ravel_multi_index(tuple(ijk.T), self.dims), np.arange(box))):
+            self._flat = None
+        else:
+            self._flat = np.ravel_multi_index(tuple(ijk.T), self.dims)
+        self.n = box if self._flat is None else len(self._flat)
         self.shape = (self.n, self.n)
         self.T = self
 
     def __matmul__(self, v):
-        v = np.ascontiguousarray(np.asarray(v, np.float64).reshape(self.dims))
+        v = np.asarray(v, np.float64).ravel()
+        if self._flat is None:
+            g = np.ascontiguousarray(v.reshape(self.dims))
+        else:
+            g = np.zeros(self.dims)
+            g.reshape(-1)[self._flat] = v
         out = np.empty(self.dims)
-        _lib.check(_lib.lib().lp_filter_apply(*self.dims, self.radius, self.reach, _lib.ptr(v), _lib.ptr(out), 0),
-                   "filter")
-        return out.ravel()
+        if self.pitch == 1.0:
+            rc = _lib.lib().lp_filter_apply(*self.dims, self.radius, self.reach, _lib.ptr(g), _lib.ptr(out), 0)
+        else:
+            rc = _lib.lib().lp_grid_filter(*self.dims, self.pitch, self.radius, _lib.ptr(g), _lib.ptr(out), 0)
+        _lib.check(rc, "filter")
+        out = out.ravel()
+        return out if self._flat is None else out[self._flat]
+
+    def tocsr(self):
+        return self
 
     def sum(self, axis=1):
         return self @ np.ones(self.n)
@@ -84,9 +118,13 @@ def dproject(xt, beta, eta):
     return _project_call(xt, beta, eta, False, True)[1]
 
 
-def density_filter(dims, radius: float, pad: bool = True):
-    """(h, hs) as simp.density_filter (simp.py:1181-1189) on a full box."""
-    h = GridFilter(dims, radius)
+def density_filter(domain, radius: float, pad: bool = True):
+    """(h, hs) as simp.density_filter(domain, radius, pad) (simp.py:1181-
+    1222): h the GridFilter stencil over the mesh's cells, hs the full
+    kernel's weight (pad: cells outside the design space count as void) or
+    h's row sums. domain as GridFilter takes it (a Domain-like object or a
+    box's dims)."""
+    h = GridFilter(domain, radius)
     hs = np.full(h.n, sum(w for _, w in h.taps)) if pad else h.sum()
     return h, hs
 

@@ -82,26 +82,32 @@ def top3d_filter(dims, rmin: float, active=None):
 
 
 class GridFilter:
-    """loadpath's stencil filter on a full box (simp.py:1143-1189): h @ v on
-    the cell grid, zero outside; hs = full kernel weight (pad) or row sums."""
+    """loadpath's stencil filter (simp.py:1143-1189): h @ v over the cells
+    ijk (default: the full box) of a box of dims cells of side pitch, zero
+    outside them; hs = full kernel weight (pad) or row sums."""
 
-    def __init__(self, dims, radius: float, pitch: float = 1.0, pad: bool = True):
+    def __init__(self, dims, radius: float, pitch: float = 1.0, pad: bool = True, ijk=None):
         self.dims = tuple(int(d) for d in dims)
         self.reach = int(np.floor(radius / pitch))
         self.taps = cone_taps(radius, self.reach, pitch)
         self.T = self
-        n = int(np.prod(self.dims))
+        self.ijk = None if ijk is None else tuple(np.asarray(ijk, np.int64).T)
+        n = int(np.prod(self.dims)) if ijk is None else len(ijk)
         self.hs = np.full(n, sum(w for _, w in self.taps)) if pad else self @ np.ones(n)
 
     def __matmul__(self, v):
         r = self.reach
         nx, ny, nz = self.dims
         g = np.zeros((nx + 2 * r, ny + 2 * r, nz + 2 * r))
-        g[r:r + nx, r:r + ny, r:r + nz] = np.asarray(v, float).reshape(self.dims)
+        inner = g[r:r + nx, r:r + ny, r:r + nz]
+        if self.ijk is None:
+            inner[...] = np.asarray(v, float).reshape(self.dims)
+        else:
+            inner[self.ijk] = v
         out = np.zeros(self.dims)
         for (dx, dy, dz), w in self.taps:
             out += w * g[r + dx:r + dx + nx, r + dy:r + dy + ny, r + dz:r + dz + nz]
-        return out.ravel()
+        return out.ravel() if self.ijk is None else out[self.ijk]
 
 
 # --------------------------------------------------------------------------

@@ -110,3 +110,23 @@ def test_projection_closed_forms():
         assert np.allclose(dproject(x, beta, eta), fd, rtol=1e-6, atol=1e-9)
     x = np.array([0.2, 0.45, 0.55, 0.8])
     assert np.allclose(project(x, 512.0, 0.5), [0, 0, 1, 1], atol=1e-10)
+
+
+def test_mesh_filter_against_brute_force():
+    """The oracle's stencil over a mesh's cells (a box subset, pitch 0.7)
+    against the dense H of the reference's definition: w_ij = r - |d| p for
+    cells within reach floor(r / p) on every axis, w > 0 (simp.py:1152-1172,
+    1193-1212)."""
+    from oracle.simp import GridFilter
+
+    rng = np.random.default_rng(8)
+    dims, pitch, r = (6, 5, 4), 0.7, 1.6
+    ijk = np.argwhere(rng.random(dims) < 0.55)
+    reach = int(np.floor(r / pitch))
+    d = ijk[:, None, :] - ijk[None, :, :]
+    w = r - np.sqrt((d ** 2).sum(axis=2)) * pitch
+    dense = np.where((np.abs(d).max(axis=2) <= reach) & (w > 0), w, 0.0)
+    f = GridFilter(dims, r, pitch=pitch, pad=False, ijk=ijk)
+    v = rng.random(len(ijk))
+    assert np.allclose(f @ v, dense @ v, rtol=1e-13, atol=0)
+    assert np.allclose(f.hs, dense.sum(axis=1), rtol=1e-13, atol=0)
