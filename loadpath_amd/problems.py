@@ -6,8 +6,10 @@ what they are:
 
 - ``cantilever_48x24x24``: clamped x = 0 face, unit downward point load at
   node (48, 0, 12); vf 0.3, rmin 1.5.
-- ``mbb_192x64x64``: half MBB beam: ux = 0 on x = 0, uz = 0 on the bottom
-  edge of x = 192, unit line load down along the top edge at x = 0;
+- ``mbb_192x64x64``: half MBB beam: ux = 0 on x = 0, a roller along the
+  bottom edge of x = 192 that moves in x only (uy = uz = 0: with uz alone
+  the rigid y-translation is free and K singular; top88's MBB, which this
+  extends, is plane), unit line load down along the top edge at x = 0;
   density U(0.05, 1) from default_rng(2).
 - ``lbracket_128x64x64``: the box minus x in [64, 128) x z in [32, 64); the
   top face (z = 64, x < 64) clamped; unit downward traction on the x = 128
@@ -119,13 +121,14 @@ def cantilever_48x24x24():
                       [[(nodes(48, 0, 12), "z", -1.0)]])
 
 
-def mbb_192x64x64(seed=2):
-    nx, ny, nz = 192, 64, 64
+def mbb_192x64x64(seed=2, nx=192, ny=64, nz=64):
+    """Scaled copies (other nx, ny, nz) keep the supports, the load and the
+    seeded density draw on their own grid."""
     top = nodes(0, range(ny + 1), nz)
     rng = np.random.default_rng(seed)
     rho = rng.uniform(0.05, 1.0, (nx, ny, nz))
-    return BoxProblem("mbb-192x64x64", (nx, ny, nz), 0.5, 2.0,
-                      [(nodes(0, range(ny + 1), range(nz + 1)), "x"), (nodes(nx, range(ny + 1), 0), "z")],
+    return BoxProblem(f"mbb-{nx}x{ny}x{nz}", (nx, ny, nz), 0.5, 2.0,
+                      [(nodes(0, range(ny + 1), range(nz + 1)), "x"), (nodes(nx, range(ny + 1), 0), "yz")],
                       [[(top, "z", -1.0 / len(top))]], density=rho)
 
 
