@@ -31,10 +31,11 @@ for _v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the fine-level FP16-MFMA operator (8 right-hand
-# sides, mean over the 52 launches of one mixed SIMP iteration): FETCH_SIZE x 2
-# + WRITE_SIZE x 1, the factors from a 1 GiB copy in the same run; from
+# sides, mean over the 52 V-cycle launches of one mixed SIMP iteration, 13 of
+# each of its four epilogue kinds): FETCH_SIZE x 2 + WRITE_SIZE x 1, the
+# factors from a 1 GiB copy in the same run; from
 # profiles/counters_multiload_256x128x128.txt
-FINE_MFMA_TRAFFIC_BYTES = 2.9107e9
+FINE_MFMA_TRAFFIC_BYTES = 2.2218e9
 WORKLOAD = dict(nx=256, ny=128, nz=128, cases=8)
 FINE_MFMA_KERNEL = ("k_apply_wave<K, false> (V-cycle fine-level operator: FP16 MFMA contraction, node sums "
                     "across lanes in registers, fused Chebyshev / residual epilogues in FP32)")
@@ -59,6 +60,8 @@ def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None, keep_
             lib.lp_tune(s.grid._h, 15, int(os.environ["LP_CG32_DELTA"]))
     if precision == "mixed" and os.environ.get("LP_BF16"):
         _lib.check(lib.lp_tune(s.grid._h, 16, int(os.environ["LP_BF16"])), "tune")
+    if precision != "fp64" and os.environ.get("LP_L1_EXPLICIT32"):
+        _lib.check(lib.lp_tune(s.grid._h, 18, int(os.environ["LP_L1_EXPLICIT32"])), "tune")
     hist = []
     for k in range(warmup):
         hist += s.iterate(1)
@@ -223,6 +226,8 @@ def run_studies_bench(log=None, max_iterations=40, loop=False):
     to change <= 0.01 or max_iterations SIMP iterations; with loop, each
     runs the production loop (simp.optimize's) to its stop or
     max_iterations."""
+    import numpy as np
+
     from loadpath_amd.batch import run_studies
     from loadpath_amd.problems import batched_studies
 
@@ -235,6 +240,14 @@ def run_studies_bench(log=None, max_iterations=40, loop=False):
                     autotune=False, loop=loop)
     if log:
         log(f"  sequential fp64: {q['wall_s']:.2f} s")
+
+    def stopped(st):
+        # reached its own stop test before the iteration cap
+        last = st["history"][-1]
+        return bool(last["converged"]) if loop else last["change"] <= 0.01
+
+    conv = [st for st in b["studies"] if stopped(st)]
+    conv_s = sorted(st["seconds"] for st in conv)
     return {
         "metric": "studies/s, 32 uneven SIMP studies (nx 32-96, ny/nz 16-48) in one batch"
                   + (", production loop" if loop else ""),
@@ -244,6 +257,10 @@ def run_studies_bench(log=None, max_iterations=40, loop=False):
         "sequential_fp64_p50_s": round(q["p50_s"], 3), "sequential_fp64_p99_s": round(q["p99_s"], 3),
         "speedup_vs_sequential_fp64": round(q["wall_s"] / b["wall_s"], 3),
         "iterations": [s["iterations"] for s in b["studies"]],
+        "converged": len(conv),
+        "converged_p50_s": round(float(np.percentile(conv_s, 50)), 3) if conv else None,
+        "converged_p99_s": round(float(np.percentile(conv_s, 99)), 3) if conv else None,
+        "sequential_fp64_converged": sum(stopped(st) for st in q["studies"]),
         "config": {"workload": "batched_studies(count=32, seed=5): cantilever / half-MBB alternating, point or "
                                "4x4-node patch load in a seeded direction, vf U(0.1, 0.4), rmin 1.5 / 2.0, "
                                + ("the production loop (padded filter, eroded analysis, continuation, measured "
@@ -390,6 +407,8 @@ def main():
     ap.add_argument("--studies", action="store_true",
                     help="instead: the 32 uneven batched studies (concurrent, mixed, graphs, autotuned) against "
                          "the same studies one after another on the plain FP64 path")
+    ap.add_argument("--studies-iters", type=int, default=40,
+                    help="with --studies: each study's iteration cap (its own stop test ends it earlier)")
     ap.add_argument("--loop", action="store_true",
                     help="instead: the production loop (simp.optimize's) on the same workload, mixed against FP64")
     ap.add_argument("--constrained", action="store_true",
@@ -405,7 +424,7 @@ def main():
         sys.exit(2)
     log = (lambda m: print(m, file=sys.stderr, flush=True)) if a.verbose else None
     if a.studies:
-        print(json.dumps(run_studies_bench(log, loop=a.loop)))
+        print(json.dumps(run_studies_bench(log, max_iterations=a.studies_iters, loop=a.loop)))
         return
     if a.constrained:
         print(json.dumps(run_constrained_bench(a.steps, a.warmup, a.no_cpu, log)))

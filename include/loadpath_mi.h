@@ -381,6 +381,30 @@ int lp_stiffness_pattern(lp_stiff* s, long long* indptr, long long* indices);
  * pointers. */
 int lp_stiffness_assemble(lp_stiff* s, const double* scale, double* data, int device_ptr);
 int lp_stiffness_destroy(lp_stiff* s);
+/* n_dof and nnz of a stiffness. */
+int lp_stiffness_size(lp_stiff* s, long long* n_dof, long long* nnz);
+
+/* The assembled route's direct solve on the device: the factorization that
+ * direct.DirectSolver (direct.py:158-246) and fem.Solver's rescue
+ * (fem.py:586-792) run on the host, as a block-tridiagonal Cholesky of the
+ * band of the stiffness in the node order perm (n_dof int64, new position ->
+ * mesh dof; NULL: mesh order). The band (max |i - j| over the pattern in
+ * that order) sets the block size b (>= band + 1, a multiple of 64); the
+ * factor takes (2 n_dof b) doubles of device memory (bytes out). Fails with
+ * LP_ERR_ARG when that does not fit. */
+typedef struct lp_band lp_band;
+int lp_band_create(lp_band** out, lp_stiff* s, const long long* perm, long long* band, long long* bytes);
+/* Assemble the stiffness for per-cell moduli scale (ncell = nx ny nz box
+ * cells, as lp_stiffness_assemble) on the device and factor it; LP_SINGULAR
+ * on a non-positive pivot (a mechanism). */
+int lp_band_factor(lp_band* b, const double* scale, int ncell, int device_ptr);
+/* Factor the CSR values data (nnz, the stiffness's pattern) instead. */
+int lp_band_factor_csr(lp_band* b, const double* data, int device_ptr);
+/* x (nrhs x n_dof, right-hand side major) = K^-1 f, then refine steps of
+ * iterative refinement against the CSR matrix (x += K^-1 (f - K x));
+ * relres = max over the right-hand sides of |f - K x| / |f| after them. */
+int lp_band_solve(lp_band* b, int nrhs, const double* f, double* x, int device_ptr, int refine, double* relres);
+int lp_band_destroy(lp_band* b);
 
 /* 26-connected components of a cell mask, scipy.ndimage.label(mask,
  * structure=ones((3,3,3))) as simp._Analysis._connected and _prune_dead use
@@ -481,7 +505,18 @@ int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best);
  * of solves that moved to the strict hierarchy (value ignored). Param 12:
  * the all-FP64 path's first coarse level is applied as its cell matrices
  * sum_q w_q M_q, formed at each hierarchy build (1, the default), or
- * matrix-free as sum_q w_q (M_q u) (0); from the next lp_set_moduli. */
+ * matrix-free as sum_q w_q (M_q u) (0); from the next lp_set_moduli.
+ * Param 13 (mixed-precision grids): the outer CG in FP32 with reliable FP64
+ * updates (1) or in FP64 (0, the default). Param 14 returns the number of
+ * reliable updates so far (value ignored). Param 15: the reliable-update
+ * threshold in thousandths (1-999; negative: the flexible beta at a
+ * replaced step). Param 16: the fine level's MFMA operands in BF16 without
+ * per-cell scales (1) or scaled FP16 (0, the default). Param 17: the MFMA
+ * kernels run a tile's right-hand sides in consecutive blocks of one XCD
+ * (1, the default) or the right-hand side as the slow block index (0).
+ * Param 18: the FP32 and mixed V-cycles' first coarse level applied as its
+ * formed FP32 cell matrices by the batched explicit kernel (1) or
+ * matrix-free (0, the default); from the next lp_set_moduli. */
 int lp_tune(lp_grid* g, int param, int value);
 
 #ifdef __cplusplus

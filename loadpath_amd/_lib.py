@@ -135,6 +135,12 @@ _SIGS = {
     "lp_stiffness_pattern": (_I, [_P, _P, _P]),
     "lp_stiffness_assemble": (_I, [_P, _P, _P, _I]),
     "lp_stiffness_destroy": (_I, [_P]),
+    "lp_stiffness_size": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "lp_band_create": (_I, [C.POINTER(_P), _P, _P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "lp_band_factor": (_I, [_P, _P, _I, _I]),
+    "lp_band_factor_csr": (_I, [_P, _P, _I]),
+    "lp_band_solve": (_I, [_P, _I, _P, _P, _I, _I, C.POINTER(C.c_double)]),
+    "lp_band_destroy": (_I, [_P]),
     "lp_stress_averaged": (_I, [_I, _I, _I, _D, _D, _I, _D, _I, _P, _P, _P, _D, _D, _D, _P, _P, _I, _P, _P,
                                 _P, _P]),
     "lp_project": (_I, [C.c_longlong, _P, _D, _D, _P, _P, _I]),

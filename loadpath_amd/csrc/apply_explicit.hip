@@ -139,7 +139,7 @@ int explicit_blocks(const Grid& G, int level) {
 
 bool explicit_weights(const Grid& G, int level, int eb) {
     const Level& lv = G.lv[level];
-    return lv.kind == LK_WEIGHTS && eb == 8 && lv.k64_apply;
+    return lv.kind == LK_WEIGHTS && ((eb == 8 && lv.k64_apply) || (eb == 4 && lv.k32_apply));
 }
 
 void launch_apply_explicit(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs, const EpiArgs& E,

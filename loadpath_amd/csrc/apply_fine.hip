@@ -49,25 +49,29 @@ int apply_blocks(const Grid& Gd, int level, int eb) {
     return Gd.lv[level].kind == LK_FINE ? 8 * G.per : G.nblk;
 }
 
-// algorithmic bytes of one operator launch: the node vectors the epilogue
-// reads and writes, the input, the cell moduli (fine level) and the mask at
-// one bit per dof (the per-apply unit of work's compulsory traffic)
+// algorithmic bytes of one operator launch: per right-hand side the node
+// vectors the epilogue reads and writes (the operand included), once per
+// launch the inverse diagonal the kinds that scale by it read (one vector
+// shared by every right-hand side), the cell moduli (fine level) and the
+// mask at one bit per dof (the per-apply unit of work's compulsory traffic)
 static double apply_bytes(const Grid& G, int level, int eb, int nrhs, int epi) {
-    int vecs = 2;
+    int vecs = 2;          // per right-hand side
+    bool dinv = false;     // plus one shared inverse diagonal
     switch (epi) {
-        case EPI_CHEB: vecs = 7; break;
-        case EPI_CHEB_FIRST: vecs = 5; break;
-        case EPI_CHEB_ONLY: vecs = 3; break;
-        case EPI_CHEB_LAST: vecs = 5; break;
-        case EPI_RESID_SCALED: vecs = 5; break;
-        case EPI_RESID_MASK: vecs = 3; break;
-        case EPI_JACOBI: vecs = 4; break;
-        case EPI_POWER: vecs = 3; break;
-        case EPI_RESID: vecs = 3; break;
-        default: vecs = 2; break;
+        case EPI_CHEB: vecs = 6; dinv = true; break;          // d, x, r, dinv -> x, r, d2
+        case EPI_CHEB_FIRST: vecs = 4; dinv = true; break;    // b, dinv -> x, r, d2
+        case EPI_CHEB_ONLY: vecs = 2; dinv = true; break;     // b, dinv -> x
+        case EPI_CHEB_LAST: vecs = 4; dinv = true; break;     // d, x, r, dinv -> x
+        case EPI_RESID_SCALED: vecs = 4; dinv = true; break;  // x, b, dinv -> r, d2
+        case EPI_RESID_MASK: vecs = 3; break;                 // x, b -> out
+        case EPI_JACOBI: vecs = 3; dinv = true; break;        // x, b, dinv -> out
+        case EPI_POWER: vecs = 2; dinv = true; break;         // u, dinv -> out
+        case EPI_RESID: vecs = 3; break;                      // x, b -> out
+        default: vecs = 2; break;                             // u -> out
     }
     const Level& L = G.lv[level];
-    double b = (double)nrhs * vecs * 3.0 * (double)L.g.nn * eb + 3.0 * (double)L.g.nn / 8.0;
+    const double v = 3.0 * (double)L.g.nn * eb;
+    double b = (double)nrhs * vecs * v + (dinv ? v : 0.0) + 3.0 * (double)L.g.nn / 8.0;
     if (L.kind == LK_FINE) b += (double)L.g.ncell * eb;
     return b;
 }

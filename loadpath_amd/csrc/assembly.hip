@@ -239,6 +239,13 @@ int lp_stiffness_assemble(lp_stiff* S, const double* scale, double* data, int de
     });
 }
 
+int lp_stiffness_size(lp_stiff* S, long long* n_dof, long long* nnz) {
+    if (!S || !n_dof || !nnz) return LP_ERR_ARG;
+    *n_dof = S->n_dof;
+    *nnz = S->nnz;
+    return LP_OK;
+}
+
 int lp_stiffness_destroy(lp_stiff* S) {
     delete S;
     return LP_OK;

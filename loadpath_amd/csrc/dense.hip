@@ -218,6 +218,11 @@ bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s) {
     return hf == 0;
 }
 
+void launch_potrf_diag(double* a, int lda, int k, int kb, double* dinv, int* fail, hipStream_t s) {
+    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, s, a, lda, k, kb, dinv, fail);
+    LP_HIP(hipGetLastError());
+}
+
 void launch_dense_matvec(const double* ainv, int n, const void* b, void* x, int eb, int nrhs, size_t stride,
                          hipStream_t s) {
     dim3 grid((n + 3) / 4, nrhs);

@@ -337,10 +337,12 @@ __global__ void k_coarse_mask(const double* __restrict__ d, const double* __rest
     held[n] = (uint8_t)hc;
 }
 
-// materialise the weights level's cell matrices (for the coarsest assembly)
+// materialise the weights level's cell matrices (the FP64 and FP32
+// explicit applies, the coarsest assembly); summed in FP64
+template <typename T>
 __global__ void k_weights_explicit(const double* __restrict__ fe, const double* __restrict__ mq,
                                    const int* __restrict__ pidx, const double* __restrict__ patch, Geom f, Geom c,
-                                   double* k64) {
+                                   T* k64) {
     const size_t cell = blockIdx.x;
     const int tid = threadIdx.x;
     const int Z = (int)(cell % c.nz), Y = (int)((cell / c.nz) % c.ny), X = (int)(cell / ((size_t)c.nz * c.ny));
@@ -355,7 +357,7 @@ __global__ void k_weights_explicit(const double* __restrict__ fe, const double* 
             v += mq[q * 576 + tid] * fe[cidx(fx, fy, fz, f.ny, f.nz)];
         }
     }
-    k64[(size_t)tid * c.ncell + cell] = v;
+    k64[(size_t)tid * c.ncell + cell] = (T)v;
 }
 
 // dense coarsest matrix (component-major dofs c*nn + node), one thread per
@@ -493,6 +495,7 @@ int build_hierarchy(Grid& G) {
         C.g.set(h[0], h[1], h[2]);
         C.fine = F.g;
         C.k64_apply = false;
+        C.k32_apply = false;
         dtmp.alloc(3 * C.g.nn * 8);
         LP_HIP(hipMemsetAsync(counter.p, 0, 8, s));
         LP_HIP(hipMemsetAsync(counter2.p, 0, 8, s));
@@ -533,9 +536,18 @@ int build_hierarchy(Grid& G) {
             C.k64_apply = need64 && G.tune_l1_explicit != 0;
             if (C.k64_apply) {
                 C.k64.alloc(576 * C.g.ncell * 8);
-                hipLaunchKernelGGL(k_weights_explicit, dim3((unsigned)C.g.ncell), dim3(576), 0, s, F.e64.as<double>(),
+                hipLaunchKernelGGL(k_weights_explicit<double>, dim3((unsigned)C.g.ncell), dim3(576), 0, s, F.e64.as<double>(),
                                    G.mq64.as<double>(), C.patch_idx.as<int>(), C.patch64.as<double>(), F.g, C.g,
                                    C.k64.as<double>());
+                LP_HIP(hipGetLastError());
+            }
+            // the FP32 and mixed V-cycles likewise, in FP32 (2.3 KB per cell)
+            C.k32_apply = !need64 && G.tune_l1_explicit32 != 0;
+            if (C.k32_apply) {
+                C.k32.alloc(576 * C.g.ncell * 4);
+                hipLaunchKernelGGL(k_weights_explicit<float>, dim3((unsigned)C.g.ncell), dim3(576), 0, s,
+                                   F.e64.as<double>(), G.mq64.as<double>(), C.patch_idx.as<int>(),
+                                   C.patch64.as<double>(), F.g, C.g, C.k32.as<float>());
                 LP_HIP(hipGetLastError());
             }
             LP_HIP(hipStreamSynchronize(s));   // ctr goes out of scope
@@ -605,7 +617,7 @@ int build_hierarchy(Grid& G) {
         LP_HIP(hipMemsetAsync(G.cinv.p, 0, ndof * ndof * 8, s));
         if (C.kind == LK_WEIGHTS && !C.k64_apply) {
             C.k64.alloc(576 * C.g.ncell * 8);
-            hipLaunchKernelGGL(k_weights_explicit, dim3((unsigned)C.g.ncell), dim3(576), 0, s,
+            hipLaunchKernelGGL(k_weights_explicit<double>, dim3((unsigned)C.g.ncell), dim3(576), 0, s,
                                G.lv[nl - 2].e64.as<double>(), G.mq64.as<double>(), C.patch_idx.as<int>(),
                                C.patch64.as<double>(), C.fine, C.g, C.k64.as<double>());
             LP_HIP(hipGetLastError());

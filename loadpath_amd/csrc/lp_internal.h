@@ -98,6 +98,7 @@ struct Level {
     int npatch = 0;
     DBuf k64, k32;              // explicit: (576, ncell)
     bool k64_apply = false;     // weights: k64 holds sum_q w_q M_q and the FP64 operator applies it
+    bool k32_apply = false;     // weights: k32 holds them in FP32 for the FP32 / mixed V-cycle
     double lmax = 0.0;
     long long n_active = 0;
     // V-cycle work vectors (elem size 8 or 4) for max_rhs right-hand sides
@@ -172,6 +173,7 @@ struct Grid {
     int tune_rhs_inner = 1;     // FP16-MFMA kernels: a tile's right-hand sides in consecutive blocks of one XCD
     int tune_bf16 = 0;          // mixed path, fine level: BF16 operands (1) or scaled FP16 ones (0)
     int tune_l1_explicit = 1;   // FP64 path: first coarse level as formed cell matrices (1) or matrix-free weights (0)
+    int tune_l1_explicit32 = 0; // FP32 / mixed V-cycles: the same in FP32 (1) or matrix-free weights (0)
     int tune_flex = -1;         // flexible (Polak-Ribiere) beta: 1 on, 0 off, -1 on for the mixed path only
     bool flex() const { return tune_flex < 0 ? opt.precision == LP_PREC_MIXED : tune_flex != 0; }
     // mixed path: a solve the FP16 levels fail is repeated with FP32
@@ -346,6 +348,10 @@ void level_cell_dofs(const Grid& G, int level, int64_t* out_dev);
 // Factor the SPD n x n matrix a (lower Cholesky, in place) and overwrite it
 // with its inverse. Returns false on a non-positive pivot.
 bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s);
+// Factor the kb x kb diagonal block at (k, k) of the row-major matrix a
+// (leading dimension lda; kb <= 64) in place, lower; inv(L_kk) to dinv
+// (64 x 64 row-major); *fail set on a non-positive pivot.
+void launch_potrf_diag(double* a, int lda, int k, int kb, double* dinv, int* fail, hipStream_t s);
 void launch_dense_matvec(const double* ainv, int n, const void* b, void* x, int elem_bytes, int nrhs,
                          size_t stride, hipStream_t s);
 

@@ -507,6 +507,15 @@ __global__ void k_reduce_sum(const double* __restrict__ partials, int nb, double
     if (threadIdx.x == 0) *out = s;
 }
 
+// block partial sums of a cell field (one per block; k_reduce_sum adds them
+// in block order)
+__global__ void k_sum_partials(const double* __restrict__ a, size_t n, double* __restrict__ partials) {
+    __shared__ double sred[4];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double s = bsum(i < n ? a[i] : 0.0, sred);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 __global__ void k_change(const double* __restrict__ a, const double* __restrict__ b, size_t n, unsigned long long* out) {
     double m = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -1162,18 +1171,22 @@ int lp_simp_iterate(lp_simp* S, int niter, lp_simp_stats* stats) {
             LP_HIP(hipMemsetAsync(S->chg.p, 0, 8, s));
             hipLaunchKernelGGL(k_change, dim3(std::min(nb, 1024u)), dim3(kBlock), 0, s, S->xnew.as<double>(),
                                S->x.as<double>(), n, S->chg.as<unsigned long long>());
+            hipLaunchKernelGGL(k_sum_partials, dim3(nb), dim3(kBlock), 0, s, S->xphys.as<double>(), n,
+                               S->partials.as<double>());
+            hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(kBlock), 0, s, S->partials.as<double>(), (int)nb, scal + 1);
             LP_HIP(hipGetLastError());
             std::swap(S->x.p, S->xnew.p);
-            double h[2];
+            double h[3];
             Bisect bh;
             LP_HIP(hipMemcpyAsync(h, scal, 8, hipMemcpyDeviceToHost, s));
             LP_HIP(hipMemcpyAsync(h + 1, S->chg.p, 8, hipMemcpyDeviceToHost, s));
+            LP_HIP(hipMemcpyAsync(h + 2, scal + 1, 8, hipMemcpyDeviceToHost, s));
             LP_HIP(hipMemcpyAsync(&bh, S->bis.p, sizeof(bh), hipMemcpyDeviceToHost, s));
             LP_HIP(hipStreamSynchronize(s));
             lp_simp_stats& st = stats[it];
             st.compliance = h[0];
             st.change = h[1];
-            st.volume = 0;
+            st.volume = h[2] / (double)n;
             st.cg_iters = *std::max_element(iters.begin(), iters.end());
             st.cg_iters_total = 0;
             for (int k = 0; k < S->nrhs; ++k) st.cg_iters_total += iters[k];

@@ -1166,6 +1166,12 @@ int lp_tune(lp_grid* g, int param, int value) {
         g->tune_l1_explicit = value;
         return old;
     }
+    if (param == 18) {
+        if (value != 0 && value != 1) return LP_ERR_ARG;
+        const int old = g->tune_l1_explicit32;
+        g->tune_l1_explicit32 = value;
+        return old;
+    }
     return LP_ERR_ARG;
 }
 
@@ -1194,7 +1200,7 @@ int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best) {
         if (ms_best) *ms_best = tbest;
         // the weights level's FP16-MFMA kernel: node planes per block (a
         // coarse grid wants short chunks, or it runs a handful of blocks)
-        if (G.nlevels > 1 && G.lv[1].kind == LK_WEIGHTS && eb == 4 && G.fp16()) {
+        if (G.nlevels > 1 && weights_mfma_path(G, 1, eb)) {
             const int wcand[] = {2, 4, 8, 16};
             int wbest = G.tune_weights_xchunk;
             double wt = 1e300;

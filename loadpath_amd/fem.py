@@ -191,3 +191,146 @@ class Stiffness:
                           shape=(self.mesh.n_dof, self.mesh.n_dof))
         k.has_sorted_indices = True
         return k
+
+
+def band_order(mesh) -> np.ndarray:
+    """The dof order of the band factor: mesh nodes sorted with the longest
+    axis of their box slowest (the band is then 3 x the node count of a
+    cross-section of the two shorter axes), three dofs per node; new
+    position -> mesh dof."""
+    ijk = np.asarray(mesh.node_ijk)
+    ext = ijk.max(axis=0) - ijk.min(axis=0)
+    slow, mid, fast = np.argsort(-ext, kind="stable")
+    nodes = np.lexsort((ijk[:, fast], ijk[:, mid], ijk[:, slow]))
+    return (3 * nodes[:, None] + np.arange(3)[None, :]).ravel().astype(np.int64)
+
+
+class DirectSolver:
+    """direct.DirectSolver (direct.py:158-246) on the device: one Cholesky
+    factor of the assembled stiffness, shared by every right-hand side of
+    that matrix, as a block-tridiagonal factor of its band in HBM
+    (lp_band_*). ``stiffness``: a Stiffness of the mesh (its pattern is the
+    reference's fem.Stiffness layout). ``solve_many(k, fs)`` factors k (a CSR
+    matrix with that pattern, e.g. Stiffness.assemble's) unless it is the
+    matrix already factored, then solves with one step of iterative
+    refinement; ``factor(scale)`` assembles and factors on the device
+    directly. Raises RuntimeError on a non-positive pivot (a mechanism)."""
+
+    def __init__(self, stiffness: Stiffness, refine: int = 1):
+        _lib.require_gpu()
+        self.stiffness = stiffness
+        self.refine = int(refine)
+        self.perm = band_order(stiffness.mesh)
+        h = C.c_void_p()
+        band, nbytes = C.c_longlong(), C.c_longlong()
+        _lib.check(_lib.lib().lp_band_create(C.byref(h), stiffness._h, _lib.ptr(self.perm), C.byref(band),
+                                             C.byref(nbytes)), "band_create")
+        self._h = h
+        self.band = int(band.value)
+        self.factor_bytes = int(nbytes.value)
+        self._k = None
+        self.last_relres = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().lp_band_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _check(self, rc, what):
+        if rc == _lib.LP_SINGULAR:
+            raise RuntimeError(f"{what}: the factorization met a non-positive pivot; "
+                               "the structure may be a mechanism (check supports)")
+        _lib.check(rc, what)
+
+    def factor(self, scale):
+        """Assemble K for per-element moduli scale (mesh order) and factor it."""
+        st = self.stiffness
+        st._scale[st._cells] = np.asarray(scale, float)
+        self._k = None
+        self._check(_lib.lib().lp_band_factor(self._h, _lib.ptr(st._scale), len(st._scale), 0), "band_factor")
+        self._k = "scale"
+
+    def _factor_csr(self, k):
+        st = self.stiffness
+        if k.shape != (st.mesh.n_dof, st.mesh.n_dof) or k.nnz != st.nnz or not np.array_equal(k.indptr, st.indptr):
+            raise ValueError("the matrix does not have this stiffness's pattern")
+        self._k = None
+        self._check(_lib.lib().lp_band_factor_csr(self._h, _lib.ptr(np.ascontiguousarray(k.data, np.float64)), 0),
+                    "band_factor")
+        self._k = k
+
+    def solve_many(self, k, fs, x0s=None):
+        """x_i = K^-1 f_i for each right-hand side (x0s ignored: a direct
+        solve needs no start)."""
+        del x0s
+        if k is not None and k is not self._k:
+            self._factor_csr(k)
+        if self._k is None:
+            raise RuntimeError("DirectSolver: nothing factored")
+        fs = [np.asarray(f, float) for f in fs]
+        out = []
+        for j in range(0, len(fs), 64):
+            f = np.ascontiguousarray(np.stack(fs[j:j + 64]))
+            x = np.empty_like(f)
+            rr = C.c_double()
+            self._check(_lib.lib().lp_band_solve(self._h, f.shape[0], _lib.ptr(f), _lib.ptr(x), 0, self.refine,
+                                                 C.byref(rr)), "band_solve")
+            self.last_relres = float(rr.value)
+            out.extend(x)
+        return out
+
+
+class Solver:
+    """fem.Solver(mesh, rtol, maxiter, ...) (fem.py:586-792) for the
+    assembled route, on the device: ``solve(k, f, x0)`` / ``solve_many(k, fs,
+    x0s)`` with k a CSR stiffness in the reference's fem.Stiffness layout.
+    The reference runs CG with a smoothed-aggregation AMG preconditioner and
+    falls back on a direct factor (direct.py) where CG stalls; here every
+    matrix is factored once on the device (DirectSolver: its band Cholesky
+    in HBM) and its right-hand sides are solved directly with iterative
+    refinement against K itself, the steps doubled (1, 2, 4, 8, at most
+    maxiter) until |f - K x| <= rtol |f|; one suffices in practice.
+    last_iters: the refinement steps of the last solve."""
+
+    def __init__(self, mesh, rtol=1e-6, maxiter=800, rebuild_after_iters=25, refresh=True):
+        del rebuild_after_iters, refresh   # the AMG hierarchy's knobs: nothing to rebuild here
+        self.mesh = mesh
+        self.rtol = float(rtol)
+        self.maxiter = int(maxiter)
+        self.last_iters = 0
+        self._direct = None
+        self._pattern = None
+
+    def invalidate(self):
+        if self._direct is not None:
+            self._direct._k = None
+
+    def _solver(self, k):
+        if self._direct is None:
+            # the pattern is the mesh's: any ke and supports give it
+            self._pattern = Stiffness(self.mesh, 0.3, np.zeros(self.mesh.n_dof, bool))
+            self._direct = DirectSolver(self._pattern)
+        return self._direct
+
+    def solve_many(self, k, fs, x0s=None):
+        d = self._solver(k)
+        fs = [np.asarray(f, float) for f in fs]
+        steps = 1
+        while True:
+            d.refine = steps
+            xs = d.solve_many(k, fs)
+            if d.last_relres <= self.rtol or steps >= min(self.maxiter, 8):
+                break
+            steps *= 2
+        self.last_iters = steps
+        if d.last_relres > self.rtol:
+            raise RuntimeError(f"linear solve did not converge (relative residual {d.last_relres:.2e} after "
+                               f"{steps} refinement steps); the structure may be a mechanism (check supports)")
+        return xs
+
+    def solve(self, k, f, x0=None):
+        return self.solve_many(k, [f])[0]

@@ -137,6 +137,49 @@ def test_weights_level_matches_oracle(wave):
     assert np.abs(got - want).max() <= TOL_FP16 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_explicit32_first_coarse_level(precision):
+    """lp_tune param 18: the first coarse level of the FP32 / mixed V-cycle
+    as its formed FP32 cell matrices (patch cells included) equals the
+    oracle's level to FP32 rounding, and a solve through it lands on the
+    direct solve with the CG count of the matrix-free level."""
+    from loadpath_amd import _lib
+    from loadpath_amd.mg import GridElasticity
+    from oracle.fem import assemble, hex8_ke
+    from oracle.mg import GridOracle
+
+    mesh, fixed, f, e_solid = cantilever_bar(16, 10, 8, seed=13)
+    rng = np.random.default_rng(14)
+    e = e_solid.copy()
+    e[rng.random(mesh.n_elem) < 0.1] = 0.0
+    e[mesh.elem_ijk[:, 0] == 0] = 1.0
+    kw = dict(coarse_cells=20, rtol=1e-10, maxiter=2000)
+    g = GridElasticity(mesh, 0.3, fixed, precision=precision, **kw)
+    assert _lib.lib().lp_tune(g._h, 18, 1) == 0
+    g.set_moduli(e)
+    o = GridOracle(mesh, 0.3, fixed, vdtype=np.float64, **kw)
+    o.set_moduli(e)
+    lv = o.levels[1]
+    u = rng.standard_normal((2,) + lv.node_shape).astype(np.float32)
+    got = g.apply(u, level=1, dtype=np.float32)
+    want = np.stack([lv.apply(v.astype(np.float64)) for v in u])
+    assert np.abs(got - want).max() <= 2e-5 * np.abs(want).max()
+
+    # the solve on the design without holes (a direct solve needs no
+    # floating cells)
+    g.set_moduli(e_solid)
+    k = assemble(mesh, hex8_ke(0.3), e_solid)
+    free = ~fixed
+    ref = np.zeros(mesh.n_dof)
+    ref[free] = spla.spsolve(k[free][:, free].tocsc(), f[free])
+    x = g.solve_many([f])[0]
+    assert np.linalg.norm(x - ref) <= 1e-7 * np.linalg.norm(ref)
+    g0 = GridElasticity(mesh, 0.3, fixed, precision=precision, **kw)
+    g0.set_moduli(e_solid)
+    g0.solve_many([f])
+    assert abs(g.last_iters - g0.last_iters) <= 2, (g.last_iters, g0.last_iters)
+
+
 @pytest.mark.parametrize("degree", [2, 3])
 def test_mixed_vcycle_tracks_fp32(degree):
     """The mixed V-cycle (FP16-MFMA operators on the two finest levels, the

@@ -1,6 +1,6 @@
 """The device SIMP loop against the oracle's top3d loop and the reference's
-golden design (benchmarks/reference/cantilever-3d.top3d.npz, copied to
-tests/golden/)."""
+golden designs (benchmarks/reference/{cantilever-3d,bridge-3d}.top3d.npz,
+copied to tests/golden/, with the oracle's histories of them)."""
 from pathlib import Path
 
 import numpy as np
@@ -36,26 +36,55 @@ def test_first_iterations_match_oracle(gpu):
     assert np.allclose(s.xphys, ro["x"], rtol=0, atol=1e-8)
 
 
-def test_top3d_golden_cantilever(gpu):
-    """The golden run stops at iteration 133 on change <= 0.01, a rule that
-    is sensitive to solver rounding (the oracle itself runs to the 200 cap
-    with an MMD-ordered LU instead of the default one). So the device loop
-    runs the golden's 133 iterations and must land on the same design: the
-    same compliance to 1e-3, volume 0.3 and a near-identical density field."""
-    from loadpath_amd.problems import cantilever_3d
-    from loadpath_amd.simp import optimize_top3d
+@pytest.mark.parametrize("name", ["cantilever-3d", "bridge-3d"])
+def test_top3d_golden_runs(gpu, name):
+    """The reference's golden top3d runs (plain element), through the oracle's
+    history of each: the oracle's loop with the reference evaluator's own
+    direct solve lands on the golden exactly (stop iteration, compliance to
+    1e-12, design to 1e-6; asserted when the fixture was written, and in
+    test_oracle_simp.py), and its per-iteration compliance and designs are
+    the fixture (tests/golden/make_top3d_histories.py). The device loop
+    (multigrid PCG to rtol 1e-12) runs the golden's iteration count:
 
-    z = np.load(GOLDEN / "cantilever-3d.top3d.npz")
-    r = optimize_top3d(cantilever_3d(), iterations=int(z["iterations"]), element="plain", rtol=1e-11,
-                       maxiter=3000, coarse_cells=500)
-    print("device c", r["compliance_reported"], "golden", float(z["compliance_reported"]),
-          "max|dx|", float(np.abs(r["x"] - z["x"]).max()), "mean|dx|", float(np.abs(r["x"] - z["x"]).mean()))
-    assert r["compliance_reported"] == pytest.approx(float(z["compliance_reported"]), rel=1e-3)
-    assert abs(r["x"].mean() - 0.3) < 2e-3
-    # a few grey cells still chatter between runs of the loop (the change
-    # metric never drops below 0.01 here): the field agrees on average
-    assert np.abs(r["x"] - z["x"]).mean() < 3e-2
-    assert ((r["x"] > 0.5) != (z["x"] > 0.5)).mean() < 0.03
+    - compliance equal to the oracle's to 1e-8 at each of the first 50
+      iterations, the designs after 10 and 50 to 1e-5;
+    - at the golden's count, compliance and design no further from the
+      golden than rounding alone moves this loop, times 10: the same loop
+      with an MMD-ordered LU (another rounding of the same direct solves)
+      lands mmd_* from the golden. The change <= 0.01 stop itself is that
+      sensitive (the MMD run does not stop at the golden's iteration), so
+      the device run is compared at a fixed count."""
+    from loadpath_amd.problems import bridge_3d, cantilever_3d
+    from loadpath_amd.simp import GridSimp
+
+    z = np.load(GOLDEN / f"{name}.top3d.npz")
+    h = np.load(GOLDEN / f"{name}.history.npz")
+    p = {"cantilever-3d": cantilever_3d, "bridge-3d": bridge_3d}[name]()
+    n = int(z["iterations"])
+    s = GridSimp(p, element="plain", rtol=1e-12, maxiter=4000, coarse_cells=500, precision="fp64")
+    hist = s.iterate(10)
+    x10 = s.xphys.copy()
+    hist += s.iterate(40)
+    x50 = s.xphys.copy()
+    hist += s.iterate(n - 50)
+    c = np.array([st["compliance"] for st in hist])
+    rel = np.abs(c - h["compliance"][:n]) / h["compliance"][:n]
+    dx10 = float(np.abs(x10 - h["xphys_10"]).max())
+    dx50 = float(np.abs(x50 - h["xphys_50"]).max())
+    dxg = np.abs(s.xphys - z["x"])
+    dcg = abs(c[-1] - float(z["compliance_reported"])) / float(z["compliance_reported"])
+    dcm = abs(float(h["mmd_compliance_reported"]) - float(z["compliance_reported"])) / float(z["compliance_reported"])
+    flip = float(((s.xphys > 0.5) != (z["x"] > 0.5)).mean())
+    print(f"{name}: compliance rel err its 1-50 {rel[:50].max():.2e}, 1-{n} {rel.max():.2e}; max|dx| it 10 "
+          f"{dx10:.2e}, it 50 {dx50:.2e}; at {n}: compliance vs golden {dcg:.2e} (MMD LU {dcm:.2e}), "
+          f"max|dx| {dxg.max():.2e} (MMD {float(h['mmd_max_dx']):.2e}), mean|dx| {dxg.mean():.2e} "
+          f"(MMD {float(h['mmd_mean_dx']):.2e}), flipped {flip:.4f} (MMD {float(h['mmd_flipped']):.4f})")
+    assert rel[:50].max() <= 1e-8
+    assert dx10 <= 1e-5 and dx50 <= 1e-5
+    assert dcg <= max(10 * dcm, 1e-6)
+    assert dxg.mean() <= max(10 * float(h["mmd_mean_dx"]), 1e-5)
+    assert dxg.max() <= max(10 * float(h["mmd_max_dx"]), 1e-4)
+    assert abs(s.xphys.mean() - p.volfrac) < 2e-3
 
 
 def test_mma_iterations_match_oracle(gpu):
