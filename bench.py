@@ -30,6 +30,7 @@ for _v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
     os.environ[_v] = str(_CORES)
 
 HBM_PEAK_GBS = 8000.0
+FP16_MFMA_PEAK_TFLOPS = 2500.0   # dense FP16 MFMA, MI355X
 # HBM bytes per launch of the fine-level FP16-MFMA operator (8 right-hand
 # sides, mean over the 52 V-cycle launches of one mixed SIMP iteration, 13 of
 # each of its four epilogue kinds): FETCH_SIZE x 2 + WRITE_SIZE x 1, the
@@ -233,11 +234,15 @@ def run_studies_bench(log=None, max_iterations=40, loop=False):
 
     ps = batched_studies()
     run_studies(ps[:2], precision="mixed", max_iterations=1, loop=loop)   # first-use costs out of the timing
-    b = run_studies(ps, precision="mixed", max_iterations=max_iterations, concurrent=True, loop=loop)
+    def done(k, st):
+        if log:
+            log(f"  study {k} ({st.get('name')}): {st.get('iterations')} iterations, {st.get('seconds', 0.0):.2f} s")
+
+    b = run_studies(ps, precision="mixed", max_iterations=max_iterations, concurrent=True, loop=loop, progress=done)
     if log:
         log(f"  batch: {b['wall_s']:.2f} s, p50 {b['p50_s']:.2f} s, p99 {b['p99_s']:.2f} s")
     q = run_studies(ps, precision="fp64", max_iterations=max_iterations, concurrent=False, graph=False,
-                    autotune=False, loop=loop)
+                    autotune=False, loop=loop, progress=done)
     if log:
         log(f"  sequential fp64: {q['wall_s']:.2f} s")
 
@@ -462,12 +467,31 @@ def main():
     prof = mixed["prof"]
     roof = None
     if prof and prof["launches"]:
-        gbs = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9
+        # the byte model counts every right-hand side of a launch, but the
+        # kernel skips the blocks of cases whose CG has converged: only the
+        # active share of the per-case bytes moves. Active share = CG
+        # iterations summed over the cases / (cases x the batch's
+        # iterations), over the timed steps; the shared part of a launch
+        # (the inverse diagonal in 3 of the 4 kinds, moduli, mask bits)
+        # moves whatever the count.
+        nc = WORKLOAD["cases"]
+        tot = sum(st["cg_iters_total"] for st in mixed["timed"])
+        mx = sum(st["cg_iters"] for st in mixed["timed"])
+        f_act = tot / (nc * mx) if mx else 1.0
+        nn = (nx + 1) * (ny + 1) * (nz + 1)
+        shared = 0.75 * 3 * nn * 4 + nx * ny * nz * 4 + 3 * nn / 8
+        L = prof["launches"]
+        active_bytes = L * shared + (prof["bytes"] - L * shared) * f_act
+        gbs = active_bytes / (prof["ms"] * 1e-3) / 1e9
+        flops = 1176.0 * nx * ny * nz * nc * f_act * L
+        tfs = flops / (prof["ms"] * 1e-3) / 1e12
         roof = dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
                     traffic=FINE_MFMA_TRAFFIC_BYTES if (nx, ny, nz) == (WORKLOAD["nx"], WORKLOAD["ny"], WORKLOAD["nz"])
                     else None, kernel=FINE_MFMA_KERNEL,
-                    launches=prof["launches"], avg_us=round(prof["ms"] * 1e3 / prof["launches"], 2),
-                    avg_bytes=round(prof["bytes"] / prof["launches"]))
+                    launches=L, avg_us=round(prof["ms"] * 1e3 / L, 2), avg_bytes=round(active_bytes / L),
+                    avg_bytes_all_cases=round(prof["bytes"] / L), active_case_fraction=round(f_act, 4),
+                    compute=dict(achieved=round(tfs, 2), peak=FP16_MFMA_PEAK_TFLOPS, unit="TFLOP/s",
+                                 frac=round(tfs / FP16_MFMA_PEAK_TFLOPS, 4), flop_per_cell=1176))
     cfg1 = None if a.no_config1 else config1_bench(50, 0 if a.no_cpu else a.config1_cpu_iters, log)
     cpu = None if a.no_cpu else cpu_baseline(mixed["timed"], nx, ny, nz, log)
     ms = 1e3 * mixed["seconds"] / a.steps

@@ -26,13 +26,15 @@ import numpy as np
 __all__ = ["run_studies"]
 
 
-def _one(problem, precision, max_iterations, tol, graph, autotune, out, k, start_barrier, loop=False):
+def _one(problem, precision, max_iterations, tol, graph, autotune, out, k, start_barrier, loop=False, progress=None):
     try:
         _run_one(problem, precision, max_iterations, tol, graph, autotune, out, k, start_barrier, loop)
     except Exception as e:  # reported by run_studies
         if start_barrier is not None:
             start_barrier.abort()
         out[k] = dict(name=problem.name, error=f"{type(e).__name__}: {e}")
+    if progress is not None:
+        progress(k, out[k])
 
 
 def _run_one(problem, precision, max_iterations, tol, graph, autotune, out, k, start_barrier, loop=False):
@@ -64,25 +66,25 @@ def _run_one(problem, precision, max_iterations, tol, graph, autotune, out, k, s
 
 
 def run_studies(problems, precision="mixed", max_iterations=40, tol=0.01, concurrent=True, graph=True,
-                autotune=True, loop=False):
+                autotune=True, loop=False, progress=None):
     """Run the studies concurrently (one host thread and stream each) or one
     after another. Returns per-study results and the batch's studies/s and
     p50 / p99 time to converge (seconds per study, from its first
-    iteration)."""
+    iteration). progress(k, result) is called as each study ends."""
     n = len(problems)
     out = [None] * n
     t0 = time.perf_counter()
     if concurrent:
         barrier = threading.Barrier(n)
         threads = [threading.Thread(target=_one, args=(p, precision, max_iterations, tol, graph, autotune, out, k,
-                                                       barrier, loop)) for k, p in enumerate(problems)]
+                                                       barrier, loop, progress)) for k, p in enumerate(problems)]
         for t in threads:
             t.start()
         for t in threads:
             t.join()
     else:
         for k, p in enumerate(problems):
-            _one(p, precision, max_iterations, tol, graph, autotune, out, k, None, loop)
+            _one(p, precision, max_iterations, tol, graph, autotune, out, k, None, loop, progress)
     wall = time.perf_counter() - t0
     bad = [o for o in out if o is None or "error" in o]
     if bad:
