@@ -42,14 +42,9 @@ constexpr int P = 64;   // panel width (launch_potrf_diag's block)
             throw std::runtime_error(std::string("rocBLAS error ") + rocblas_status_to_string(st_)); \
     } while (0)
 
-// row-major C (M x N, ldc) = alpha op(A) op(B) + beta C, as the column-major
-// product C^T = op(B)^T op(A)^T
 void gemm_rm(rocblas_handle h, bool ta, bool tb, int M, int N, int K, double alpha, const double* A, int lda,
              const double* B, int ldb, double beta, double* C, int ldc) {
-    if (M <= 0 || N <= 0) return;
-    LP_RB(rocblas_dgemm(h, tb ? rocblas_operation_transpose : rocblas_operation_none,
-                        ta ? rocblas_operation_transpose : rocblas_operation_none, N, M, K, &alpha, B, ldb, A, lda,
-                        &beta, C, ldc));
+    blas_gemm_rm(h, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 // the lower triangle of each (permuted) CSR row into the diagonal block or

@@ -11,7 +11,10 @@
 // the diagonal block in LDS and inverts it; the panel is a GEMM with that
 // inverse; the trailing update a GEMM. Then Linv by block rows and
 // A^-1 = Linv^T Linv.
+#include <rocblas/rocblas.h>
+
 #include <algorithm>
+#include <stdexcept>
 
 #include "lp_internal.h"
 
@@ -31,92 +34,58 @@ __global__ void __launch_bounds__(256) k_potrf_diag(double* a, int n, int k, int
         const int i = e / kb, j = e % kb;
         s[i][j] = a[(size_t)(k + i) * n + k + j];
     }
+    __shared__ double red[4][NB];
     __syncthreads();
-    for (int j = 0; j < kb; ++j) {
-        if (tid == 0) {
-            const double d = s[j][j];
-            if (!(d > 0.0)) {
-                *fail = 1;
-                s[j][j] = 1.0;
-            } else {
-                s[j][j] = sqrt(d);
+    // left-looking, one column a step: thread (part, i) adds every fourth
+    // term of L[i, :j] . L[j, :j]; the four partial sums meet in LDS (a fixed
+    // order), every thread forms the pivot of column j from row j's sum, and
+    // row i > j divides by it
+    {
+        const int i = tid & (NB - 1), part = tid >> 6;
+        for (int j = 0; j < kb; ++j) {
+            double acc = 0.0;
+            if (i >= j && i < kb)
+                for (int m = part; m < j; m += 4) acc += s[i][m] * s[j][m];
+            red[part][i] = acc;
+            const double ajj = s[j][j];   // read before thread j overwrites it below
+            __syncthreads();
+            if (tid < kb && tid >= j) {
+                const double dj = ajj - ((red[0][j] + red[1][j]) + (red[2][j] + red[3][j]));
+                const bool ok = dj > 0.0;
+                const double piv = ok ? sqrt(dj) : 1.0;
+                if (tid == j) {
+                    if (!ok) *fail = 1;
+                    s[j][j] = piv;
+                } else {
+                    s[tid][j] = (s[tid][j] - ((red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]))) / piv;
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
-        const double piv = s[j][j];
-        for (int i = j + 1 + tid; i < kb; i += blockDim.x) s[i][j] /= piv;
-        __syncthreads();
-        for (int e = tid; e < (kb - j - 1) * (kb - j - 1); e += blockDim.x) {
-            const int i = j + 1 + e / (kb - j - 1), c = j + 1 + e % (kb - j - 1);
-            if (c <= i) s[i][c] -= s[i][j] * s[c][j];
-        }
-        __syncthreads();
     }
-    // inv(L): column c solved by thread c
-    if (tid < kb) {
-        const int c = tid;
+    // inv(L) by rows: inv[i][c] = (delta_ic - sum_{c <= m < i} L[i][m] inv[m][c]) / L[i][i];
+    // thread (part, c) adds every fourth term of column c, the four partial
+    // sums meet in LDS (a fixed order)
+    {
+        const int c = tid & (NB - 1), part = tid >> 6;
         for (int i = 0; i < kb; ++i) {
-            double v = (i == c) ? 1.0 : 0.0;
-            for (int m = c; m < i; ++m) v -= s[i][m] * inv[m][c];
-            inv[i][c] = (i < c) ? 0.0 : v / s[i][i];
+            double acc = 0.0;
+            if (c < i)
+                for (int m = c + part; m < i; m += 4) acc += s[i][m] * inv[m][c];
+            red[part][c] = acc;
+            __syncthreads();
+            if (tid < kb) {
+                const double v = (i == tid ? 1.0 : 0.0) - ((red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]));
+                inv[i][tid] = tid <= i ? v / s[i][i] : 0.0;
+            }
+            __syncthreads();
         }
     }
-    __syncthreads();
     for (int e = tid; e < kb * kb; e += blockDim.x) {
         const int i = e / kb, j = e % kb;
         a[(size_t)(k + i) * n + k + j] = j <= i ? s[i][j] : 0.0;
         dinv[i * NB + j] = inv[i][j];
     }
-}
-
-// C (M x N, ldc) = beta C + alpha * op(A) op(B); A: M x K (or K x M if ta),
-// B: K x N (or N x K if tb). Row-major. 64x64 tile per block, 256 threads,
-// 4x4 outputs per thread, K chunks of 16 staged in LDS.
-__global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, double alpha, const double* __restrict__ A, int lda,
-                                              int ta, const double* __restrict__ B, int ldb, int tb, double beta,
-                                              double* C, int ldc) {
-    __shared__ double As[16][64 + 1];
-    __shared__ double Bs[16][64 + 1];
-    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-    double acc[4][4] = {};
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        for (int e = threadIdx.x; e < 16 * 64; e += 256) {
-            const int kk = e / 64, mm = e % 64;
-            const int gm = m0 + mm, gk = k0 + kk;
-            double va = 0;
-            if (gm < M && gk < K) va = ta ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
-            As[kk][mm] = va;
-            const int gn = n0 + mm;
-            double vb = 0;
-            if (gn < N && gk < K) vb = tb ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
-            Bs[kk][mm] = vb;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-            double av[4], bv[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) av[i] = As[kk][ty + 16 * i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][tx + 16 * j];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int gm = m0 + ty + 16 * i, gn = n0 + tx + 16 * j;
-            if (gm < M && gn < N) {
-                double* c = C + (size_t)gm * ldc + gn;
-                *c = (beta == 0.0 ? 0.0 : beta * *c) + alpha * acc[i][j];
-            }
-        }
 }
 
 __global__ void k_copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols) {
@@ -133,12 +102,9 @@ __global__ void k_zero_upper(double* a, int n) {
     if (j > i) a[e] = 0.0;
 }
 
-void gemm(hipStream_t s, int M, int N, int K, double alpha, const double* A, int lda, int ta, const double* B, int ldb,
+void gemm(void* h, int M, int N, int K, double alpha, const double* A, int lda, int ta, const double* B, int ldb,
           int tb, double beta, double* C, int ldc) {
-    if (M <= 0 || N <= 0) return;
-    dim3 grid((N + 63) / 64, (M + 63) / 64);
-    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, M, N, K, alpha, A, lda, ta, B, ldb, tb, beta, C, ldc);
-    LP_HIP(hipGetLastError());
+    blas_gemm_rm(h, ta != 0, tb != 0, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 // y (n, nrhs stride) = Ainv b, FP64 accumulate; one wave per row
@@ -160,7 +126,37 @@ __global__ void k_matvec(const double* __restrict__ ainv, int n, const T* __rest
 
 }  // namespace
 
+Scratch::~Scratch() { blas_release(blas); }
+
+void blas_gemm_rm(void* h, bool ta, bool tb, int M, int N, int K, double alpha, const double* A, int lda,
+                  const double* B, int ldb, double beta, double* C, int ldc) {
+    if (M <= 0 || N <= 0) return;
+    // row-major C = op(A) op(B) is the column-major C^T = op(B)^T op(A)^T
+    const rocblas_status st = rocblas_dgemm(static_cast<rocblas_handle>(h),
+                                            tb ? rocblas_operation_transpose : rocblas_operation_none,
+                                            ta ? rocblas_operation_transpose : rocblas_operation_none, N, M, K, &alpha,
+                                            B, ldb, A, lda, &beta, C, ldc);
+    if (st != rocblas_status_success)
+        throw std::runtime_error(std::string("rocBLAS dgemm: ") + rocblas_status_to_string(st));
+}
+
+void* blas_handle(void*& slot, hipStream_t s) {
+    if (!slot) {
+        rocblas_handle h = nullptr;
+        if (rocblas_create_handle(&h) != rocblas_status_success) throw std::runtime_error("rocblas_create_handle failed");
+        slot = h;
+    }
+    if (rocblas_set_stream(static_cast<rocblas_handle>(slot), s) != rocblas_status_success)
+        throw std::runtime_error("rocblas_set_stream failed");
+    return slot;
+}
+
+void blas_release(void* h) {
+    if (h) rocblas_destroy_handle(static_cast<rocblas_handle>(h));
+}
+
 bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s) {
+    void* bh = blas_handle(w.blas, s);
     const int nblk = (n + NB - 1) / NB;
     DBuf& dinv = w.dn_dinv;
     DBuf& fail = w.dn_fail;
@@ -183,13 +179,13 @@ bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s) {
         const int rest = n - k - kb;
         if (rest > 0) {
             // panel L21 = A21 inv(L11)^T  (rest x kb)
-            gemm(s, rest, kb, kb, 1.0, a + (size_t)(k + kb) * n + k, n, 0, Db, NB, 1, 0.0, panel.as<double>(), NB);
+            gemm(bh, rest, kb, kb, 1.0, a + (size_t)(k + kb) * n + k, n, 0, Db, NB, 1, 0.0, panel.as<double>(), NB);
             const int cnt = rest * kb;
             hipLaunchKernelGGL(k_copy2d, dim3((cnt + 255) / 256), dim3(256), 0, s, panel.as<double>(), NB,
                                a + (size_t)(k + kb) * n + k, n, rest, kb);
             LP_HIP(hipGetLastError());
             // trailing A22 -= L21 L21^T
-            gemm(s, rest, rest, kb, -1.0, panel.as<double>(), NB, 0, panel.as<double>(), NB, 1, 1.0,
+            gemm(bh, rest, rest, kb, -1.0, panel.as<double>(), NB, 0, panel.as<double>(), NB, 1, 1.0,
                  a + (size_t)(k + kb) * n + (k + kb), n);
         }
     }
@@ -205,13 +201,13 @@ bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s) {
         LP_HIP(hipGetLastError());
         if (k > 0) {
             // T (kb x k) = L[k:k+kb, :k] Linv[:k, :k]
-            gemm(s, kb, k, k, 1.0, a + (size_t)k * n, n, 0, Li, n, 0, 0.0, t.as<double>(), n);
+            gemm(bh, kb, k, k, 1.0, a + (size_t)k * n, n, 0, Li, n, 0, 0.0, t.as<double>(), n);
             // Linv[k:k+kb, :k] = -inv(L_kk) T
-            gemm(s, kb, k, kb, -1.0, Db, NB, 0, t.as<double>(), n, 0, 0.0, Li + (size_t)k * n, n);
+            gemm(bh, kb, k, kb, -1.0, Db, NB, 0, t.as<double>(), n, 0, 0.0, Li + (size_t)k * n, n);
         }
     }
     // A^-1 = Linv^T Linv
-    gemm(s, n, n, n, 1.0, Li, n, 1, Li, n, 0, 0.0, a, n);
+    gemm(bh, n, n, n, 1.0, Li, n, 1, Li, n, 0, 0.0, a, n);
     int hf = 0;
     LP_HIP(hipMemcpyAsync(&hf, fail.p, sizeof(int), hipMemcpyDeviceToHost, s));
     LP_HIP(hipStreamSynchronize(s));

@@ -116,6 +116,11 @@ struct Scratch {
     DBuf counter, counter2, dtmp, dmaxb, ctr;
     DBuf lmax_sc, lmax_av, lmax_tmp;
     DBuf dn_dinv, dn_fail, dn_panel, dn_linv, dn_t;
+    void* blas = nullptr;       // rocBLAS handle for the dense coarsest factor (its grid's stream)
+    Scratch() = default;
+    Scratch(const Scratch&) = delete;
+    Scratch& operator=(const Scratch&) = delete;
+    ~Scratch();
 };
 
 struct Grid {
@@ -348,6 +353,13 @@ void level_cell_dofs(const Grid& G, int level, int64_t* out_dev);
 // Factor the SPD n x n matrix a (lower Cholesky, in place) and overwrite it
 // with its inverse. Returns false on a non-positive pivot.
 bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s);
+// Row-major C (M x N, ldc) = alpha op(A) op(B) + beta C by rocBLAS dgemm on
+// handle h (the library's plain GEMM); blas_handle creates the handle in
+// slot on first use and binds it to stream s.
+void blas_gemm_rm(void* h, bool ta, bool tb, int M, int N, int K, double alpha, const double* A, int lda,
+                  const double* B, int ldb, double beta, double* C, int ldc);
+void* blas_handle(void*& slot, hipStream_t s);
+void blas_release(void* h);
 // Factor the kb x kb diagonal block at (k, k) of the row-major matrix a
 // (leading dimension lda; kb <= 64) in place, lower; inv(L_kk) to dinv
 // (64 x 64 row-major); *fail set on a non-positive pivot.
