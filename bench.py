@@ -59,6 +59,8 @@ def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None, keep_
         _lib.check(lib.lp_tune(s.grid._h, 13, int(cg32)), "tune")
         if os.environ.get("LP_CG32_DELTA"):
             lib.lp_tune(s.grid._h, 15, int(os.environ["LP_CG32_DELTA"]))
+        if os.environ.get("LP_CG32_RESTART"):
+            lib.lp_tune(s.grid._h, 19, int(os.environ["LP_CG32_RESTART"]))
     if precision == "mixed" and os.environ.get("LP_BF16"):
         _lib.check(lib.lp_tune(s.grid._h, 16, int(os.environ["LP_BF16"])), "tune")
     if precision != "fp64" and os.environ.get("LP_L1_EXPLICIT32"):

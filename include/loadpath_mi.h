@@ -516,7 +516,11 @@ int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best);
  * (1, the default) or the right-hand side as the slow block index (0).
  * Param 18: the FP32 and mixed V-cycles' first coarse level applied as its
  * formed FP32 cell matrices by the batched explicit kernel (1) or
- * matrix-free (0, the default); from the next lp_set_moduli. */
+ * matrix-free (0, the default); from the next lp_set_moduli. Param 19
+ * (with param 13): restart the FP32 PCG from each recomputed FP64 residual
+ * (1: iterative refinement as gpu.py:429-481's solve_fp32 does it, with
+ * the inner tolerance param 15's threshold, 3 for its 3e-3) or carry the
+ * direction on (0, the default). */
 int lp_tune(lp_grid* g, int param, int value);
 
 #ifdef __cplusplus

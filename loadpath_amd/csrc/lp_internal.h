@@ -179,6 +179,7 @@ struct Grid {
     int tune_bf16 = 0;          // mixed path, fine level: BF16 operands (1) or scaled FP16 ones (0)
     int tune_l1_explicit = 1;   // FP64 path: first coarse level as formed cell matrices (1) or matrix-free weights (0)
     int tune_l1_explicit32 = 0; // FP32 / mixed V-cycles: the same in FP32 (1) or matrix-free weights (0)
+    int tune_cg32_restart = 0;  // FP32 CG: restart at each reliable update (iterative refinement)
     int tune_flex = -1;         // flexible (Polak-Ribiere) beta: 1 on, 0 off, -1 on for the mixed path only
     bool flex() const { return tune_flex < 0 ? opt.precision == LP_PREC_MIXED : tune_flex != 0; }
     // mixed path: a solve the FP16 levels fail is repeated with FP32

@@ -535,7 +535,12 @@ static void pcg32_run(Grid& G, int nrhs, bool warm, int* iters, int* converged) 
             if (!any) break;
         }
         vcycle_run(G, nrhs);
-        if (flex && (!replaced || G.cg32_flex_replaced)) {
+        if (replaced && G.tune_cg32_restart) {
+            // iterative refinement: the FP32 PCG restarts from the FP64
+            // residual (gpu.py:429-481's outer step)
+            launch_dot2_f32(G, nrhs, r32, z32, nullptr, s);
+            launch_cg_finalize(G, nrhs, 10, nbd, s);
+        } else if (flex && (!replaced || G.cg32_flex_replaced)) {
             launch_dot2_f32(G, nrhs, r32, z32, G.ap32.as<float>(), s);
             launch_cg_beta_flex(G, nrhs, nbd, s);
         } else {
@@ -1164,6 +1169,12 @@ int lp_tune(lp_grid* g, int param, int value) {
         if (value != 0 && value != 1) return LP_ERR_ARG;
         const int old = g->tune_l1_explicit;
         g->tune_l1_explicit = value;
+        return old;
+    }
+    if (param == 19) {
+        if (value != 0 && value != 1) return LP_ERR_ARG;
+        const int old = g->tune_cg32_restart;
+        g->tune_cg32_restart = value;
         return old;
     }
     if (param == 18) {

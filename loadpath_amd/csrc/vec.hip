@@ -226,6 +226,11 @@ __global__ void k_cg_finalize(const double* __restrict__ partials, int nb, CgSca
                 c.active = 0;
             }
             break;
+        case 10:  // rz of a restart: the next direction is z itself
+            if (!c.active) break;
+            c.beta = 0.0;
+            c.rz = s;
+            break;
         default: break;
     }
 }

@@ -288,13 +288,14 @@ def optimize_mma(p: Problem, iterations, solve=None, penal=3.0, move=0.2, elemen
 
 
 def optimize_top3d(p: Problem, solve=None, maxit=200, iterations=None, penal=3.0, move=0.2,
-                   element="im", on_iter=None, filt=None):
+                   element="im", on_iter=None, filt=None, on_design=None):
     """The top3d loop (benchmarks/reference.py:214-257) for the 3-D case.
 
     solve(e) -> list of displacement vectors (mesh dof numbering), one per
     load case; defaults to the assembled direct solve. `iterations` runs a
     fixed count instead of the change <= 0.01 stop (the configs' "N OC
-    iterations"). filt: (h, hs) or None for the top3d filter."""
+    iterations"). filt: (h, hs) or None for the top3d filter. on_design(loop,
+    xphys): the filtered design after each iteration."""
     ev = Evaluator(p, element)
     if solve is None:
         solve = lambda e: [ev.solve(e, f) for f in ev.fs]
@@ -323,5 +324,7 @@ def optimize_top3d(p: Problem, solve=None, maxit=200, iterations=None, penal=3.0
         hist.append((loop, c, change))
         if on_iter is not None:
             on_iter(loop, c, change)
+        if on_design is not None:
+            on_design(loop, xphys)
     return dict(x=xphys.reshape(p.dims), iterations=loop, compliance_reported=c, seconds=time.time() - t0,
                 history=hist)

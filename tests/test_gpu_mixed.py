@@ -324,6 +324,38 @@ def test_fp32_cg_reliable_updates_match_direct(rtol):
     assert i32 <= 1.15 * i64 + 3, res
 
 
+@pytest.mark.parametrize("rtol", [1e-8, 1e-11])
+def test_fp32_refinement_matches_direct(rtol):
+    """Iterative refinement (lp_tune params 13 + 19, inner tolerance 3e-3 by
+    param 15): the FP32 PCG restarts from each FP64 residual, as
+    gpu.py:429-481's solve_fp32 does; the answer meets the tolerance on its
+    FP64 residual, warm starts included."""
+    from oracle.fem import assemble, hex8_ke
+    from loadpath_amd import _lib
+    from loadpath_amd.mg import GridElasticity
+
+    mesh, fixed, f, e = cantilever_bar(16, 8, 8, seed=2)
+    k = assemble(mesh, hex8_ke(0.3), e)
+    free = ~fixed
+    ref = np.zeros(mesh.n_dof)
+    ref[free] = spla.spsolve(k[free][:, free].tocsc(), f[free])
+    lib = _lib.lib()
+    g = GridElasticity(mesh, 0.3, fixed, precision="mixed", rtol=rtol, coarse_cells=100, maxiter=2000)
+    lib.lp_tune(g._h, 13, 1)
+    lib.lp_tune(g._h, 15, 3)
+    assert lib.lp_tune(g._h, 19, 1) == 0
+    g.set_moduli(e)
+    u = g.solve_many([f, 3.0 * f])
+    for j, s in enumerate((1.0, 3.0)):
+        r = (s * f - k @ u[j])[free]
+        assert np.linalg.norm(r) <= 1.05 * rtol * np.linalg.norm(s * f[free]), j
+    assert np.linalg.norm(u[0] - ref) <= max(1e3 * rtol, 1e-7) * np.linalg.norm(ref)
+    g.set_moduli(e * 1.1)
+    w = g.solve_many([f], [u[0]])[0]
+    assert np.linalg.norm(w - ref / 1.1) <= max(1e3 * rtol, 1e-7) * np.linalg.norm(ref / 1.1)
+    assert int(lib.lp_tune(g._h, 14, 0)) >= 2 and int(lib.lp_tune(g._h, 6, 0)) == 0
+
+
 def test_fp32_cg_simp_tracks_fp64():
     from loadpath_amd import _lib
     from loadpath_amd.problems import multiload_256x128x128
