@@ -520,7 +520,12 @@ int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best);
  * (with param 13): restart the FP32 PCG from each recomputed FP64 residual
  * (1: iterative refinement as gpu.py:429-481's solve_fp32 does it, with
  * the inner tolerance param 15's threshold, 3 for its 3e-3) or carry the
- * direction on (0, the default). */
+ * direction on (0, the default). Param 20: the mixed V-cycle's Galerkin
+ * levels (below the first coarse one) applied as FP16 cell matrices with a
+ * power-of-two scale per cell, formed at each hierarchy build (1, the
+ * default), those and the first coarse level's sum_q w_q M_q as well (2;
+ * measured slower there than the FP16-MFMA weights kernel), or FP32 cell
+ * matrices (0); from the next lp_set_moduli. */
 int lp_tune(lp_grid* g, int param, int value);
 
 #ifdef __cplusplus

@@ -131,6 +131,285 @@ k_apply_explicit_batched64(const double* __restrict__ u0, size_t vstride, const 
     explicit_batched_body<double>(u0, vstride, mask, G, k, nrhs, E);
 }
 
+// ---------------------------------------------------------------------------
+// FP16 cell matrices (the mixed V-cycle's coarse levels)
+//
+// Each cell's 576 entries are scaled by 2^k, k chosen so its largest |entry|
+// lands in [2^14, 2^15) (entries down to 2^-29 of the largest stay normal
+// FP16), rounded to FP16 and stored as 72 halves per corner a (rows 3a..3a+2,
+// 24 columns each, row-major) in 9 groups of 8: group (a, g) of cell ci is the
+// 16-byte slot (a * 9 + g) * ncell + ci, so the threads of consecutive nodes
+// read consecutive 16-byte slots. sc16[ci] = 2^-k.
+//
+// The apply is the batched explicit kernel's (thread = node x one of its 8
+// cells, the 8 cells' sums met in LDS) with the matrix entries in FP16 and
+// the arithmetic in FP32, half the matrix traffic of the FP32 form. A
+// rounded matrix no longer annihilates translations, so each row subtracts
+// its own rounded entries' sum per component times the cell's corner-0 value:
+// the product is that of the rounded matrix with the corner vector relative
+// to corner 0 (as the FP16-MFMA operators form it), and a displacement's
+// common offset, which can be far larger than the cell's deformation, drops
+// out at FP32 rounding. Cells whose exact matrix does not annihilate
+// translations (those touching held dofs: patch matrices, masked Galerkin
+// levels) are flagged by a negative scale and applied without it.
+constexpr int kHalfGroups = 9;      // 16-byte groups per corner (72 halves)
+
+typedef unsigned uint4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float half_lo(unsigned w) {
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu));
+}
+__device__ __forceinline__ float half_hi(unsigned w) {
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16));
+}
+
+// acc + h * v with h the low (HI = false) or high half of w, as FP16 read by
+// the mixed-precision FMA itself: the matrix stays packed in registers (36
+// instead of 72 VGPRs; a plain conversion is hoisted out of the right-hand
+// side loop by the compiler, unpacking every entry)
+template <bool HI>
+__device__ __forceinline__ float fma_half(unsigned w, float v, float acc) {
+    float r;
+    if (HI)
+        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(w), "v"(v), "v"(acc));
+    else
+        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(w), "v"(v), "v"(acc));
+    return r;
+}
+
+// one 576-thread block per cell, entry tid = row * 24 + col; the weights
+// level's entries sum_q w_q M_q (patch cells: the patch matrix) in FP64 as
+// the FP64 and FP32 explicit forms make them, other levels from their FP32
+// Galerkin matrices
+template <bool WEIGHTS>
+__global__ void __launch_bounds__(576) k_form_half(const double* __restrict__ fe, const double* __restrict__ mq,
+                                                   const int* __restrict__ pidx, const double* __restrict__ patch,
+                                                   Geom f, Geom c, const float* __restrict__ k32,
+                                                   unsigned short* __restrict__ k16, float* __restrict__ sc16) {
+    __shared__ double wmax[9];
+    const size_t cell = blockIdx.x;
+    const int tid = (int)threadIdx.x;
+    double v = 0.0;
+    if (WEIGHTS) {
+        const int Z = (int)(cell % c.nz), Y = (int)((cell / c.nz) % c.ny), X = (int)(cell / ((size_t)c.nz * c.ny));
+        const int pi = pidx[cell];
+        if (pi >= 0) {
+            v = patch[(size_t)pi * 576 + tid];
+        } else {
+            for (int q = 0; q < 8; ++q) {
+                const int fx = 2 * X + (q & 1), fy = 2 * Y + ((q >> 1) & 1), fz = 2 * Z + ((q >> 2) & 1);
+                if (fx >= f.nx || fy >= f.ny || fz >= f.nz) continue;
+                v += mq[q * 576 + tid] * fe[((size_t)fx * f.ny + fy) * f.nz + fz];
+            }
+        }
+    } else {
+        v = (double)k32[(size_t)tid * c.ncell + cell];
+    }
+    // does the cell's matrix annihilate translations? Every row's entries
+    // summed per component vanish (to the entries' own rounding) unless the
+    // cell touches held dofs (patch matrices, strict Galerkin levels); only
+    // then may the apply subtract the corner-0 translation
+    __shared__ double ent[576];
+    ent[tid] = v;
+    __syncthreads();
+    bool moves = false;
+    if (tid < 72) {
+        const int row = tid / 3, cc = tid - 3 * (tid / 3);
+        double sum = 0.0, mag = 0.0;
+        for (int b = 0; b < 8; ++b) {
+            const double x = ent[row * 24 + 3 * b + cc];
+            sum += x;
+            mag += fabs(x);
+        }
+        moves = fabs(sum) > 1e-4 * mag;
+    }
+    const bool keep_offset = __syncthreads_or(moves);
+    double a = fabs(v);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o, 64));
+    if ((tid & 63) == 0) wmax[tid >> 6] = a;
+    __syncthreads();
+    double amax = 0.0;
+    for (int w = 0; w < 9; ++w) amax = fmax(amax, wmax[w]);
+    int ex = 0;
+    if (amax > 0.0) (void)frexp(amax, &ex);            // amax = m 2^ex, m in [0.5, 1)
+    const int k = min(max(15 - ex, -100), 100);
+    const int row = tid / 24, col = tid - 24 * (tid / 24);
+    const int ca = row / 3, e = 24 * (row - 3 * ca) + col;
+    const _Float16 h = (_Float16)(float)ldexp(v, k);
+    k16[((size_t)(ca * kHalfGroups + (e >> 3)) * c.ncell + cell) * 8 + (e & 7)] =
+        __builtin_bit_cast(unsigned short, h);
+    // the sign carries the flag: negative = apply the matrix as it is
+    if (tid == 0) sc16[cell] = keep_offset ? -ldexpf(1.0f, -k) : ldexpf(1.0f, -k);
+}
+
+template <int EK>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 4)))
+k_apply_half(const float* __restrict__ u0, size_t vstride, const uint8_t* __restrict__ mask, AGeom G,
+             const uint4v* __restrict__ k16, const float* __restrict__ sc16, int nrhs, EpiArgs E) {
+    using namespace apply_detail;
+    constexpr bool need_raw_all = EK == EPI_STORE_DOT || EK == EPI_JACOBI || EK == EPI_POWER;
+    // first Chebyshev step from zero: the operand is d = (dinv b) s of b
+    constexpr bool STAGE_B = EK == EPI_CHEB_FIRST || EK == EPI_CHEB_ONLY;
+    (void)need_raw_all;
+    __shared__ float red[kXR][8][kXN][3];
+    const int t = (int)threadIdx.x;
+    const int c8 = t / kXN, ln = t - c8 * kXN;
+    const unsigned nn = (unsigned)G.nn;
+    const unsigned nid = blockIdx.x * kXN + ln;
+    const bool valid = nid < nn;
+    const unsigned nyz = (unsigned)G.Ny * (unsigned)G.Nz;
+    const int x = valid ? (int)(nid / nyz) : 0;
+    const unsigned rem = valid ? nid - (unsigned)x * nyz : 0;
+    const int y = (int)(rem / (unsigned)G.Nz);
+    const int z = (int)(rem - (unsigned)y * (unsigned)G.Nz);
+    const int cx = c8 & 1, cy = (c8 >> 1) & 1, cz = (c8 >> 2) & 1;
+    const int X = x - 1 + cx, Y = y - 1 + cy, Z = z - 1 + cz;
+    const bool has = valid && X >= 0 && Y >= 0 && Z >= 0 && X < G.nx && Y < G.ny && Z < G.nz;
+    const int a = 7 - c8;   // the node's corner in the cell
+    const size_t nc = G.ncell;
+    const size_t ci = has ? ((size_t)X * G.ny + Y) * G.nz + Z : 0;
+    // the node's 3 rows of its cell's matrix: 72 halves in 36 words
+    unsigned kw[4 * kHalfGroups];
+#pragma unroll
+    for (int g = 0; g < kHalfGroups; ++g) {
+        const uint4v w = has ? k16[(size_t)(a * kHalfGroups + g) * nc + ci] : uint4v{0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) kw[4 * g + j] = w[j];
+    }
+    const float sflag = has ? sc16[ci] : 0.0f;
+    const float scl = fabsf(sflag);
+    // cells whose matrix does not annihilate translations (negative scale):
+    // no corner-0 correction
+    const float rs_on = sflag < 0.0f ? 0.0f : 1.0f;
+    auto kv = [&](int i, int col) -> float {
+        const int e = 24 * i + col;
+        return (e & 1) ? half_hi(kw[e >> 1]) : half_lo(kw[e >> 1]);
+    };
+    // each row's rounded entries summed per component
+    float rs[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            float sacc = 0.0f;
+#pragma unroll
+            for (int b = 0; b < 8; ++b) sacc += kv(i, 3 * b + cc);
+            rs[i][cc] = sacc * rs_on;
+        }
+    unsigned off[8];
+    int mk[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        off[b] = has ? ((unsigned)(X + (b & 1)) * (unsigned)G.Ny + (unsigned)(Y + ((b >> 1) & 1))) * (unsigned)G.Nz +
+                           (unsigned)(Z + ((b >> 2) & 1))
+                     : 0u;
+        mk[b] = has ? (int)mask[off[b]] : 0;
+    }
+    const float s_b = (float)E.c3;
+    float wdv[STAGE_B ? 24 : 1];
+    if (STAGE_B) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+                wdv[STAGE_B ? 3 * b + cc : 0] =
+                    ((mk[b] >> cc) & 1) ? static_cast<const float*>(E.dinv)[(size_t)cc * nn + off[b]] : 0.0f;
+    }
+    for (int r0 = 0; r0 < nrhs; r0 += kXR) {
+        const int nr = min(kXR, nrhs - r0);
+        // the 24 operand values of right-hand side r0 + q, gathered one
+        // right-hand side ahead of the products (zeros for an inactive one, a
+        // cell outside the grid or a masked-out dof: offsets past the buffer)
+        auto gather = [&](int q, float (&v)[24]) {
+            const int rhs = min(r0 + q, nrhs - 1);
+            const bool go = has && q < nr && !(E.scal != nullptr && !E.scal[rhs].active);
+            const __amdgpu_buffer_rsrc_t ru = make_rsrc(u0 + (size_t)rhs * vstride, 3u * nn * 4u);
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc)
+                    v[3 * b + cc] = bload<float>(ru, (go && ((mk[b] >> cc) & 1)) ? (cc * nn + off[b]) * 4u : kOOB);
+        };
+        auto product = [&](int q, const float (&v)[24]) {
+            if (q >= nr) return;
+            float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, v0[3];
+#pragma unroll
+            for (int b = 0; b < 8; ++b)
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    float w = v[3 * b + cc];
+                    if (STAGE_B) w = (wdv[STAGE_B ? 3 * b + cc : 0] * w) * s_b;
+                    if (b == 0) v0[cc] = w;
+                    const int e = 3 * b + cc;
+                    if (e & 1) {
+                        t0 = fma_half<true>(kw[e >> 1], w, t0);
+                        t1 = fma_half<true>(kw[(24 + e) >> 1], w, t1);
+                        t2 = fma_half<true>(kw[(48 + e) >> 1], w, t2);
+                    } else {
+                        t0 = fma_half<false>(kw[e >> 1], w, t0);
+                        t1 = fma_half<false>(kw[(24 + e) >> 1], w, t1);
+                        t2 = fma_half<false>(kw[(48 + e) >> 1], w, t2);
+                    }
+                }
+            t0 -= rs[0][0] * v0[0] + rs[0][1] * v0[1] + rs[0][2] * v0[2];
+            t1 -= rs[1][0] * v0[0] + rs[1][1] * v0[1] + rs[1][2] * v0[2];
+            t2 -= rs[2][0] * v0[0] + rs[2][1] * v0[1] + rs[2][2] * v0[2];
+            red[q][c8][ln][0] = t0 * scl;
+            red[q][c8][ln][1] = t1 * scl;
+            red[q][c8][ln][2] = t2 * scl;
+        };
+        float va[24], vb[24];
+        gather(0, va);
+        for (int q = 0; q < nr; q += 2) {
+            gather(q + 1, vb);
+            product(q, va);
+            gather(q + 2, va);
+            product(q + 1, vb);
+        }
+        __syncthreads();
+        // thread (q, node): the node's sum over its 8 cells, then the epilogue
+        const int q = t / kXN;
+        const int rhs = r0 + q;
+        double part = 0.0;
+        const bool live = q < nr && valid && !(E.scal != nullptr && !E.scal[rhs].active);
+        if (live) {
+            float yv[3];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const float s01 = red[q][0][ln][cc] + red[q][1][ln][cc], s23 = red[q][2][ln][cc] + red[q][3][ln][cc];
+                const float s45 = red[q][4][ln][cc] + red[q][5][ln][cc], s67 = red[q][6][ln][cc] + red[q][7][ln][cc];
+                yv[cc] = (s01 + s23) + (s45 + s67);
+            }
+            const float* __restrict__ u = u0 + (size_t)rhs * vstride;
+            const int m = mask[nid];
+            const size_t o = (size_t)rhs * vstride + nid;
+            EpiIn<float> in;
+            epi_load<float, EK>(E, o, nn, nid, in);
+            float ur[3];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const float raw = u[(size_t)cc * nn + nid];
+                if (STAGE_B) {
+                    ur[cc] = (in.v[3][cc] * raw) * s_b;
+                    in.v[0][cc] = raw;
+                } else {
+                    ur[cc] = raw;
+                }
+                if (!((m >> cc) & 1)) yv[cc] = ur[cc];
+            }
+            epi_finish<float, EK, float>(E, m, o, nn, yv, ur, in, part);
+        }
+        if (E.partials != nullptr) {
+            // the kXN threads of one rhs are one half of a wave
+#pragma unroll
+            for (int o = kXN / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+            if (ln == 0 && q < nr) E.partials[(size_t)rhs * gridDim.x + blockIdx.x] = part;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 int explicit_blocks(const Grid& G, int level) {
@@ -154,6 +433,60 @@ void launch_apply_explicit(const Grid& G, int level, int eb, const void* u, size
         hipLaunchKernelGGL(k_apply_explicit_batched, dim3(explicit_blocks(G, level)), dim3(kBlock), 0, s,
                            static_cast<const float*>(u), vstride, lv.mask.as<uint8_t>(), A, lv.k32.as<float>(), nrhs,
                            E);
+    LP_HIP(hipGetLastError());
+}
+
+bool half_explicit(const Grid& G, int level, int eb) {
+    return G.fp16() && eb == 4 && level >= 1 && G.lv[level].k16_apply;
+}
+
+void form_half_matrices(const Grid& G, int level, const float* src32, hipStream_t s) {
+    const Level& C = G.lv[level];
+    if (C.g.ncell == 0) return;
+    unsigned short* k16 = C.k16.as<unsigned short>();
+    float* sc = C.sc16.as<float>();
+    if (src32 == nullptr) {
+        const Level& F = G.lv[level - 1];
+        hipLaunchKernelGGL(k_form_half<true>, dim3((unsigned)C.g.ncell), dim3(576), 0, s, F.e64.as<double>(),
+                           G.mq64.as<double>(), C.patch_idx.as<int>(), C.patch64.as<double>(), F.g, C.g,
+                           (const float*)nullptr, k16, sc);
+    } else {
+        hipLaunchKernelGGL(k_form_half<false>, dim3((unsigned)C.g.ncell), dim3(576), 0, s, (const double*)nullptr,
+                           (const double*)nullptr, (const int*)nullptr, (const double*)nullptr, C.g, C.g, src32, k16,
+                           sc);
+    }
+    LP_HIP(hipGetLastError());
+}
+
+void launch_apply_half(const Grid& G, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,
+                       hipStream_t s) {
+    const Level& lv = G.lv[level];
+    const AGeom A = make_apply_geom(G, level);
+    const dim3 grid(explicit_blocks(G, level)), block(kBlock);
+    const float* uf = static_cast<const float*>(u);
+    const uint8_t* m = lv.mask.as<uint8_t>();
+    const uint4v* k = lv.k16.as<uint4v>();
+    const float* sc = lv.sc16.as<float>();
+#define LP_HALF_CASE(K) \
+    case K: hipLaunchKernelGGL((k_apply_half<K>), grid, block, 0, s, uf, vstride, m, A, k, sc, nrhs, E); break;
+    switch (E.kind) {
+        LP_HALF_CASE(EPI_STORE)
+        LP_HALF_CASE(EPI_STORE_DOT)
+        LP_HALF_CASE(EPI_CHEB)
+        LP_HALF_CASE(EPI_RESID_SCALED)
+        LP_HALF_CASE(EPI_RESID_MASK)
+        LP_HALF_CASE(EPI_JACOBI)
+        LP_HALF_CASE(EPI_POWER)
+        LP_HALF_CASE(EPI_RESID)
+        LP_HALF_CASE(EPI_MASKED)
+        LP_HALF_CASE(EPI_CHEB_FIRST)
+        LP_HALF_CASE(EPI_CHEB_ONLY)
+        LP_HALF_CASE(EPI_CHEB_LAST)
+        default:
+            set_error("level operator: unknown epilogue");
+            throw ArgFailure();
+    }
+#undef LP_HALF_CASE
     LP_HIP(hipGetLastError());
 }
 

@@ -42,6 +42,7 @@ AGeom make_apply_geom(const Grid& Gd, int level) {
 int fine_vector_blocks(const Grid& Gd) { return 8 * make_apply_geom(Gd, 0).per; }
 
 int apply_blocks(const Grid& Gd, int level, int eb) {
+    if (half_explicit(Gd, level, eb)) return explicit_blocks(Gd, level);
     if (weights_mfma_path(Gd, level, eb)) return mfma_blocks(Gd, level);
     if (Gd.lv[level].kind == LK_FINE && eb == 4 && Gd.fp16()) return mfma_blocks(Gd, level);
     if (Gd.lv[level].kind == LK_EXPLICIT || explicit_weights(Gd, level, eb)) return explicit_blocks(Gd, level);
@@ -95,14 +96,21 @@ int launch_apply(const Grid& G, int level, int eb, const void* u, size_t vstride
                 launch_apply_fine(G, level, eb, u, vstride, nrhs, E, s);
             break;
         case LK_WEIGHTS:
-            if (weights_mfma_path(G, level, eb))
+            if (half_explicit(G, level, eb))
+                launch_apply_half(G, level, u, vstride, nrhs, E, s);
+            else if (weights_mfma_path(G, level, eb))
                 launch_apply_weights_mfma(G, level, u, vstride, nrhs, E, s);
             else if (explicit_weights(G, level, eb))
                 launch_apply_explicit(G, level, eb, u, vstride, nrhs, E, s);
             else
                 launch_apply_weights(G, level, eb, u, vstride, nrhs, E, s);
             break;
-        default: launch_apply_explicit(G, level, eb, u, vstride, nrhs, E, s); break;
+        default:
+            if (half_explicit(G, level, eb))
+                launch_apply_half(G, level, u, vstride, nrhs, E, s);
+            else
+                launch_apply_explicit(G, level, eb, u, vstride, nrhs, E, s);
+            break;
     }
     if (rec && 2 * (P.used + 1) <= P.ev.size()) {
         LP_HIP(hipEventRecord(P.ev[2 * P.used + 1], s));

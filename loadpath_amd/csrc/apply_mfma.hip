@@ -915,11 +915,13 @@ int mfma_blocks(const Grid& Gd, int level) {
 
 bool mfma_level(const Grid& G, int level, int eb) {
     return G.fp16() && eb == 4 &&
-           (G.lv[level].kind == LK_FINE || (G.lv[level].kind == LK_WEIGHTS && !G.lv[level].k32_apply));
+           (G.lv[level].kind == LK_FINE || (G.lv[level].kind == LK_WEIGHTS && !G.lv[level].k32_apply) ||
+            G.lv[level].k16_apply);
 }
 
 bool weights_mfma_path(const Grid& G, int level, int eb) {
-    return G.fp16() && eb == 4 && level >= 1 && G.lv[level].kind == LK_WEIGHTS && !G.lv[level].k32_apply;
+    return G.fp16() && eb == 4 && level >= 1 && G.lv[level].kind == LK_WEIGHTS && !G.lv[level].k32_apply &&
+           !G.lv[level].k16_apply;
 }
 
 void launch_apply_fine_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,

@@ -496,6 +496,10 @@ int build_hierarchy(Grid& G) {
         C.fine = F.g;
         C.k64_apply = false;
         C.k32_apply = false;
+        C.k16_apply = false;
+        // FP16 cell matrices on the mixed V-cycle's Galerkin levels (param 20
+        // = 1) and on the first coarse level as well (2)
+        const int half = G.opt.precision == LP_PREC_MIXED ? G.tune_half_explicit : 0;
         dtmp.alloc(3 * C.g.nn * 8);
         LP_HIP(hipMemsetAsync(counter.p, 0, 8, s));
         LP_HIP(hipMemsetAsync(counter2.p, 0, 8, s));
@@ -550,6 +554,13 @@ int build_hierarchy(Grid& G) {
                                    C.patch64.as<double>(), F.g, C.g, C.k32.as<float>());
                 LP_HIP(hipGetLastError());
             }
+            // the mixed V-cycle: the same matrices in FP16 (1.1 KB per cell)
+            C.k16_apply = half >= 2 && !C.k32_apply;
+            if (C.k16_apply) {
+                C.k16.alloc(576 * C.g.ncell * 2);
+                C.sc16.alloc(C.g.ncell * 4);
+                form_half_matrices(G, nl, nullptr, s);
+            }
             LP_HIP(hipStreamSynchronize(s));   // ctr goes out of scope
         } else {
             C.kind = LK_EXPLICIT;
@@ -577,6 +588,12 @@ int build_hierarchy(Grid& G) {
                                    counter2.as<unsigned long long>());
             }
             LP_HIP(hipGetLastError());
+            C.k16_apply = half >= 1;
+            if (C.k16_apply) {
+                C.k16.alloc(576 * C.g.ncell * 2);
+                C.sc16.alloc(C.g.ncell * 4);
+                form_half_matrices(G, nl, C.k32.as<float>(), s);
+            }
             if (need64)
                 hipLaunchKernelGGL(k_explicit_diag<double>, dim3(nblk(C.g.nn)), dim3(kBlock), 0, s, C.k64.as<double>(),
                                    C.g, dtmp.as<double>());

@@ -99,6 +99,11 @@ struct Level {
     DBuf k64, k32;              // explicit: (576, ncell)
     bool k64_apply = false;     // weights: k64 holds sum_q w_q M_q and the FP64 operator applies it
     bool k32_apply = false;     // weights: k32 holds them in FP32 for the FP32 / mixed V-cycle
+    // mixed V-cycle, coarse levels: the cell matrices in FP16 with a power-of-two
+    // scale per cell (k16: (corner, 16-byte group, cell) of 8 halves, the
+    // corner's 3 rows x 24 columns row-major in 9 groups; sc16: per cell)
+    DBuf k16, sc16;
+    bool k16_apply = false;
     double lmax = 0.0;
     long long n_active = 0;
     // V-cycle work vectors (elem size 8 or 4) for max_rhs right-hand sides
@@ -179,6 +184,7 @@ struct Grid {
     int tune_bf16 = 0;          // mixed path, fine level: BF16 operands (1) or scaled FP16 ones (0)
     int tune_l1_explicit = 1;   // FP64 path: first coarse level as formed cell matrices (1) or matrix-free weights (0)
     int tune_l1_explicit32 = 0; // FP32 / mixed V-cycles: the same in FP32 (1) or matrix-free weights (0)
+    int tune_half_explicit = 1; // mixed V-cycle: Galerkin levels (1) / every coarse level (2) as FP16 cell matrices, or none (0)
     int tune_cg32_restart = 0;  // FP32 CG: restart at each reliable update (iterative refinement)
     int tune_flex = -1;         // flexible (Polak-Ribiere) beta: 1 on, 0 off, -1 on for the mixed path only
     bool flex() const { return tune_flex < 0 ? opt.precision == LP_PREC_MIXED : tune_flex != 0; }
@@ -275,6 +281,13 @@ int fine_vector_blocks(const Grid& G);
 int explicit_blocks(const Grid& G, int level);
 // the FP64 operator of a weights level applies its formed cell matrices (k64)
 bool explicit_weights(const Grid& G, int level, int eb);
+// the mixed V-cycle's coarse level applied from its FP16 cell matrices
+bool half_explicit(const Grid& G, int level, int eb);
+void launch_apply_half(const Grid& G, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,
+                       hipStream_t s);
+// FP16 cell matrices with a per-cell scale from the FP64 weights-level sums
+// (src32 == nullptr) or from an FP32 (576, cells) array
+void form_half_matrices(const Grid& G, int level, const float* src32, hipStream_t s);
 void launch_apply_fine(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs,
                        const EpiArgs& E, hipStream_t s);
 void launch_apply_weights(const Grid& G, int level, int eb, const void* u, size_t vstride, int nrhs,

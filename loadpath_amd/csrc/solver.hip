@@ -1177,6 +1177,14 @@ int lp_tune(lp_grid* g, int param, int value) {
         g->tune_cg32_restart = value;
         return old;
     }
+    if (param == 20) {
+        // mixed V-cycle: the Galerkin levels (1) or every coarse level (2) as
+        // FP16 cell matrices (takes effect at the next hierarchy build)
+        if (value < 0 || value > 2) return LP_ERR_ARG;
+        const int old = g->tune_half_explicit;
+        g->tune_half_explicit = value;
+        return old;
+    }
     if (param == 18) {
         if (value != 0 && value != 1) return LP_ERR_ARG;
         const int old = g->tune_l1_explicit32;

@@ -57,7 +57,7 @@ def test_mfma_operator_cancels_rigid_offset():
     assert np.abs(got - want).max() <= TOL_FP16 * np.abs(want).max()
 
 
-@pytest.mark.parametrize("level", [0, 1])
+@pytest.mark.parametrize("level", [0, 1, 2])
 def test_mfma_operator_tiny_magnitudes(level):
     """Cells whose corner differences are below 2^-113 (FP32 vectors of the
     V-cycle in void regions reach such values) must keep their FP16
@@ -91,6 +91,8 @@ def test_wave_tile_kernel_matches_block_tile_kernel():
     mesh, fixed, f, e = cantilever_bar(33, 17, 19, seed=17)
     g, o = _grids(mesh, fixed, e, coarse_cells=40)
     lib = _lib.lib()
+    lib.lp_tune(g._h, 20, 0)   # the weights level as the FP16-MFMA kernels
+    g.set_moduli(e)
     rng = np.random.default_rng(18)
     out = {}
     for wave in (2, 0):
@@ -129,6 +131,8 @@ def test_weights_level_matches_oracle(wave):
     e[rng.random(mesh.n_elem) < 0.1] = 0.0
     e[mesh.elem_ijk[:, 0] == 0] = 1.0
     g, o = _grids(mesh, fixed, e, coarse_cells=20)
+    _lib.lib().lp_tune(g._h, 20, 0)   # the weights level as the FP16-MFMA kernels
+    g.set_moduli(e)
     _lib.lib().lp_tune(g._h, 7, wave)
     lv = o.levels[1]
     u = rng.standard_normal((2,) + lv.node_shape).astype(np.float32)
