@@ -525,7 +525,10 @@ int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best);
  * power-of-two scale per cell, formed at each hierarchy build (1, the
  * default), those and the first coarse level's sum_q w_q M_q as well (2;
  * measured slower there than the FP16-MFMA weights kernel), or FP32 cell
- * matrices (0); from the next lp_set_moduli. */
+ * matrices (0); from the next lp_set_moduli. Param 21: the FP64 CG's
+ * direction and solution updates (p = z + beta p, x += alpha p) done by the
+ * next iteration's operator launch as it stages p (1, the default; the same
+ * arithmetic, bitwise the same iterates) or by a separate pass (0). */
 int lp_tune(lp_grid* g, int param, int value);
 
 #ifdef __cplusplus

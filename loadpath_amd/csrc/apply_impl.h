@@ -29,6 +29,8 @@
 //
 // This header is included by one translation unit per cell-matrix kind.
 #pragma once
+#include <type_traits>
+
 #include "ke_pattern.h"
 #include "lp_internal.h"
 
@@ -339,11 +341,23 @@ struct FineSmem {
     static constexpr int kComps = 3;   // staged words per node and plane
 };
 
-template <typename T, bool SYM>
+//
+// ZB > 0 (FP64 only): the CG's direction update fused into the operator
+// (EPI_CG_APPLY): the operand is p = z + beta p_old, formed as each plane is
+// staged, with z the preconditioned residual of element size ZB (E.d) and
+// p_old the operand vector u; each owned node writes p (E.d2), A p (E.out)
+// and its p . Ap partial, and adds alpha p_old to x (E.x) where the last
+// iteration's step is pending. A right-hand side that is no longer active
+// but has a pending step only updates x. The arithmetic is k_cg_update's.
+// MAXE: staged entries per thread (W <= MAXE * 256; 3 for z extents up to
+// 255 nodes, 4 beyond)
+template <typename T, bool SYM, int ZB = 0, int MAXE = 4>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))) k_apply_fine(const T* __restrict__ u0, size_t vstride,
                                                     const uint8_t* __restrict__ mask, AGeom G,
                                                     const T* __restrict__ ecell, const T* __restrict__ ke,
                                                     KeVals<T> kv, EpiArgs E) {
+    static_assert(ZB == 0 || sizeof(T) == 8, "the fused CG update is FP64");
+    using ZT = typename std::conditional<ZB == 4, float, double>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* lds = reinterpret_cast<T*>(smem_raw);
     const int rhs = blockIdx.z;
@@ -351,6 +365,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
     const int logical = (L & 7) * G.per + (L >> 3);
     double part = 0.0;
     const bool skip = (E.scal != nullptr && !E.scal[rhs].active) || logical >= G.nblk;
+    if (ZB && skip && logical < G.nblk && E.scal[rhs].xpend) {
+        // converged at the last step: its pending x += alpha p only
+        const int xc = logical / G.nyzb;
+        const int t = (logical - xc * G.nyzb) * (int)blockDim.x + (int)threadIdx.x;
+        const int nyz = G.Ny * G.Nz;
+        if (t < nyz) {
+            const double a = E.scal[rhs].alpha;
+            const double* pr = static_cast<const double*>((const void*)u0) + (size_t)rhs * vstride;
+            double* xr = static_cast<double*>(E.x) + (size_t)rhs * vstride;
+            for (int X = xc * G.xchunk; X < min(xc * G.xchunk + G.xchunk, G.Nx); ++X)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const size_t i = (size_t)c * G.nn + (size_t)X * nyz + t;
+                    xr[i] = fma(a, pr[i], xr[i]);
+                }
+        }
+    }
     if (!skip) {
         const int xc = logical / G.nyzb;
         const int yb = logical - xc * G.nyzb;
@@ -373,9 +404,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
         const __amdgpu_buffer_rsrc_t re = make_rsrc(ecell, (unsigned)G.ncell * (unsigned)sizeof(T));
         const bool cyv[2] = {active && y > 0, active && y < G.ny};
         const bool czv[2] = {active && z > 0, active && z < G.nz};
-        constexpr int MAXE = 4;                            // staged entries per thread (W <= 4 * 256)
         T pre[MAXE][3];
         int pm[MAXE];
+        // fused CG update: z at the staged nodes, combined at the store
+        ZT pz[ZB ? MAXE : 1][3];
+        const __amdgpu_buffer_rsrc_t rz =
+            make_rsrc(ZB ? static_cast<const ZT*>(E.d) + (size_t)rhs * vstride : nullptr, ZB ? 3u * nnu * ZB : 0u);
+        const double beta = ZB ? E.scal[rhs].beta : 0.0;
 
         // load plane x's staged range into registers, zero outside the grid;
         // the loads do not depend on the mask (applied at the store), so they
@@ -392,7 +427,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
                 const unsigned n = (unsigned)(x * nyz + f);
                 pm[k] = bload_u8(rm, n | oob);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) pre[k][c] = bload<T>(ru, (unsigned)((c * nnu + n) * sizeof(T)) | oob);
+                for (int c = 0; c < 3; ++c) {
+                    pre[k][c] = bload<T>(ru, (unsigned)((c * nnu + n) * sizeof(T)) | oob);
+                    if (ZB) pz[ZB ? k : 0][c] = bload<ZT>(rz, (unsigned)((c * nnu + n) * ZB) | oob);
+                }
             }
         };
         auto store = [&](int buf) {
@@ -402,7 +440,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
                 const int e = (int)threadIdx.x + k * (int)blockDim.x;
                 if (e < W) {
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) p[c * W + e] = ((pm[k] >> c) & 1) ? pre[k][c] : T(0);
+                    for (int c = 0; c < 3; ++c) {
+                        const T v = ZB ? (T)fma(beta, (double)pre[k][c], (double)pz[ZB ? k : 0][c]) : pre[k][c];
+                        p[c * W + e] = ((pm[k] >> c) & 1) ? v : T(0);
+                    }
                 }
             }
         };
@@ -519,7 +560,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))
             ec_load(X + 1, ecn);
             fetch(more ? X + 2 : -2);
             layer(X, bl, bh, true, ecc);
-            if (active) {
+            if (ZB && active) {
+                // p = z + beta p_old at the node (as staged), x += alpha p_old
+                T yv[3] = {acc0, acc1, acc2};
+                const CgScal& sc = E.scal[rhs];
+                const bool xp = sc.xpend != 0;
+                const double a = sc.alpha;
+                const size_t o = (size_t)rhs * vstride + n;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double po = (double)bload<T>(ru, (unsigned)((c * nnu + n) * sizeof(T)));
+                    const double zv = (double)bload<ZT>(rz, (unsigned)((c * nnu + n) * ZB));
+                    const double pn = fma(beta, po, zv);
+                    const size_t i = o + (size_t)c * G.nn;
+                    if (!((m >> c) & 1)) yv[c] = (T)pn;
+                    static_cast<double*>(E.out)[i] = yv[c];
+                    static_cast<double*>(E.d2)[i] = pn;
+                    part += pn * (double)yv[c];
+                    if (xp) {
+                        double* xr = static_cast<double*>(E.x);
+                        xr[i] = fma(a, po, xr[i]);
+                    }
+                }
+            } else if (active) {
                 T yv[3] = {acc0, acc1, acc2};
                 const T* q = lds + bl * planeT + ((int)threadIdx.x + halo);
                 // the staged value is the operand itself where the dof is
@@ -671,10 +734,44 @@ void launch_kind(const Grid& Gd, int level, const T* u, size_t vstride, int nrhs
             throw ArgFailure();
         }
         const size_t smem = (size_t)3 * W * FineSmem<T>::kComps * sizeof(T);
-        if (Gd.ke_sym)
-            hipLaunchKernelGGL((k_apply_fine<T, true>), grid, block, smem, s, u, vstride, m, G, e, ke, kv, E);
-        else
-            hipLaunchKernelGGL((k_apply_fine<T, false>), grid, block, smem, s, u, vstride, m, G, e, ke, kv, E);
+        // the kernel for (Ke symmetry, z size of the fused CG update, staged
+        // entries per thread)
+        auto go = [&](auto sym, auto zb, auto me) {
+            hipLaunchKernelGGL((k_apply_fine<T, decltype(sym)::value, decltype(zb)::value, decltype(me)::value>), grid,
+                               block, smem, s, u, vstride, m, G, e, ke, kv, E);
+        };
+        using I3 = std::integral_constant<int, 3>;
+        using I4 = std::integral_constant<int, 4>;
+        using Z0 = std::integral_constant<int, 0>;
+        auto by_entries = [&](auto sym, auto zb) {
+            if (W <= 3 * kBlock)
+                go(sym, zb, I3{});
+            else
+                go(sym, zb, I4{});
+        };
+        auto by_sym = [&](auto zb) {
+            if (Gd.ke_sym)
+                by_entries(std::true_type{}, zb);
+            else
+                by_entries(std::false_type{}, zb);
+        };
+        if (E.kind == EPI_CG_APPLY) {
+            if constexpr (sizeof(T) == 8) {
+                if ((E.zbytes != 4 && E.zbytes != 8) || !E.scal || !E.d || !E.d2 || !E.x || !E.out) {
+                    set_error("fine level: the fused CG update needs z, p, x, A p and the CG scalars");
+                    throw ArgFailure();
+                }
+                if (E.zbytes == 4)
+                    by_sym(std::integral_constant<int, 4>{});
+                else
+                    by_sym(std::integral_constant<int, 8>{});
+            } else {
+                set_error("fine level: the fused CG update runs on the FP64 operator");
+                throw ArgFailure();
+            }
+        } else {
+            by_sym(Z0{});
+        }
     } else {
         CSrc<T> C{};
         if (KIND == LK_WEIGHTS) {

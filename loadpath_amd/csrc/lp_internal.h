@@ -147,6 +147,7 @@ struct Grid {
     DBuf cwork;                 // factor workspace
     // CG vectors (FP64), max_rhs each
     DBuf x, r, p, ap, b;        // FP64 CG state (z = M r is never stored in FP64)
+    DBuf p2;                    // the next direction, written by the fused CG operator (swapped with p)
     DBuf x0;                    // the warm start kept for a repeat in the strict hierarchy
     // mixed path, FP32 CG with reliable FP64 updates: direction, its product
     // and the solution since the last update (FP32); G.x holds the FP64 sum
@@ -186,6 +187,7 @@ struct Grid {
     int tune_l1_explicit32 = 0; // FP32 / mixed V-cycles: the same in FP32 (1) or matrix-free weights (0)
     int tune_half_explicit = 1; // mixed V-cycle: Galerkin levels (1) / every coarse level (2) as FP16 cell matrices, or none (0)
     int tune_cg32_restart = 0;  // FP32 CG: restart at each reliable update (iterative refinement)
+    int tune_cg_fused = 1;      // FP64 CG: direction and x updates fused into the next operator launch
     int tune_flex = -1;         // flexible (Polak-Ribiere) beta: 1 on, 0 off, -1 on for the mixed path only
     bool flex() const { return tune_flex < 0 ? opt.precision == LP_PREC_MIXED : tune_flex != 0; }
     // mixed path: a solve the FP16 levels fail is repeated with FP32
@@ -250,6 +252,10 @@ enum EpiKind {
     EPI_CHEB_FIRST = 9,   // d = (dinv b) s: x = d; r = b - y; d2 = c1 d + c2 dinv r
     EPI_CHEB_ONLY = 10,   // d = (dinv b) s: x = d + c1 d + c2 dinv (b - y)
     EPI_CHEB_LAST = 11,   // x += d + c1 d + c2 dinv (r - y)
+    // the fine FP64 operator only: u = p_old; the operand is p = z + beta p_old
+    // (z = d, of element size zbytes); d2 = p; out = A p; partial += p . Ap;
+    // x += alpha p_old where the CG step is pending (alpha, beta from scal)
+    EPI_CG_APPLY = 12,
 };
 
 struct EpiArgs {
@@ -266,6 +272,7 @@ struct EpiArgs {
     double* partials = nullptr;   // per (rhs, block)
     const CgScal* scal = nullptr; // skip inactive rhs when set
     const double* scale_dev = nullptr;  // optional per-rhs scale read on device
+    int zbytes = 0;               // EPI_CG_APPLY: element size of z (4 or 8)
 };
 
 // Launch y = A_level u with epilogue, for nrhs vectors of stride vstride.

@@ -395,13 +395,29 @@ static void pcg_run(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
     launch_cast(L0.vx.p, eb, G.p.p, 8, n, n, nrhs, s);
     std::vector<CgScal> h;
     bool pending = false;   // alpha of the last iteration not yet added to x
+    // from the second iteration on, the operator launch also forms the new
+    // direction p = z + beta p_old (into p2, then swapped) and adds the last
+    // step alpha p_old to x: the separate update pass is gone
+    const bool fused = G.tune_cg_fused != 0;
+    bool deferred = false;  // the last iteration's p and x updates left to the next operator launch
     for (int it = 1; it <= G.opt.maxiter; ++it) {
         EpiArgs E;
         E.kind = EPI_STORE_DOT;
         E.out = G.ap.p;
         E.partials = P;
         E.scal = S;
+        if (deferred) {
+            E.kind = EPI_CG_APPLY;
+            E.d = L0.vx.p;
+            E.zbytes = eb;
+            E.d2 = G.p2.p;
+            E.x = G.x.p;
+        }
         launch_apply(G, 0, 8, G.p.p, n, nrhs, E, s);
+        if (deferred) {
+            swap_buf(G.p, G.p2);
+            deferred = false;
+        }
         launch_cg_finalize(G, nrhs, 0, nba, s);
         launch_cg_step(G, nrhs, L0.vb.p, eb, s);
         launch_cg_finalize(G, nrhs, 1, nbd, s);
@@ -418,11 +434,18 @@ static void pcg_run(Grid& G, int nrhs, bool warm, int* iters, int* converged) {
             launch_cast_dot(L0.vx.p, eb, G.r.as<double>(), nullptr, n, n, nrhs, P, S, s);
             launch_cg_finalize(G, nrhs, 2, nbd, s);
         }
-        // x += alpha p with this iteration's p, then p = z + beta p
-        launch_cg_update(G, nrhs, L0.vx.p, eb, false, s);
+        // x += alpha p with this iteration's p, then p = z + beta p (in the
+        // next operator launch when fused)
+        if (fused)
+            deferred = true;
+        else
+            launch_cg_update(G, nrhs, L0.vx.p, eb, false, s);
         pending = false;
     }
     if (pending) launch_cg_update(G, nrhs, L0.vx.p, eb, true, s);
+    // the cap reached with the updates still deferred: apply them (the
+    // direction too, as the separate pass would have)
+    if (deferred) launch_cg_update(G, nrhs, L0.vx.p, eb, false, s);
     read_scal(G, nrhs, h);
     for (int k = 0; k < nrhs; ++k) {
         iters[k] = h[k].iters + (h[k].failed ? 1 : 0);
@@ -763,6 +786,7 @@ int lp_grid_create(lp_grid** out, int nx, int ny, int nz, const uint8_t* fixed_g
             G->x.alloc(vb);
             G->r.alloc(vb);
             G->p.alloc(vb);
+            G->p2.alloc(vb);
             G->ap.alloc(vb);
             G->b.alloc(vb);
             G->x0.alloc(vb);
@@ -1183,6 +1207,14 @@ int lp_tune(lp_grid* g, int param, int value) {
         if (value < 0 || value > 2) return LP_ERR_ARG;
         const int old = g->tune_half_explicit;
         g->tune_half_explicit = value;
+        return old;
+    }
+    if (param == 21) {
+        // FP64 CG: the direction and x updates fused into the next operator
+        // launch (1, the default) or a separate pass (0)
+        if (value != 0 && value != 1) return LP_ERR_ARG;
+        const int old = g->tune_cg_fused;
+        g->tune_cg_fused = value;
         return old;
     }
     if (param == 18) {

@@ -146,8 +146,9 @@ __global__ void k_cg_update(double* __restrict__ x, double* __restrict__ p, cons
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const size_t k = rhs * stride + i;
         const double pk = p[k];
-        if (upd) x[k] += a * pk;
-        if (dir) p[k] = (double)z[k] + beta * pk;
+        // explicit FMAs: the fused operator (EPI_CG_APPLY) rounds the same way
+        if (upd) x[k] = fma(a, pk, x[k]);
+        if (dir) p[k] = fma(beta, pk, (double)z[k]);
     }
 }
 
