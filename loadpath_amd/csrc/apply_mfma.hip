@@ -40,6 +40,7 @@ namespace lp {
 namespace {
 
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef short short4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
@@ -491,6 +492,12 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
     const int jc = lane & 31, hh = lane >> 5, r = (lane >> 4) & 1, c = lane & 15;
     half4 A[3];
     short4v Ab[3];
+    // fine level: the contraction as one v_mfma_f32_32x32x16_f16 (gfx950's
+    // K = 16 form, twice the rate of K = 8) and one 32x32x8; lane half hh
+    // holds its 12 operands q = 0..11 as k = 8 hh + q (q < 8) and
+    // 16 + 4 hh + q - 8, the Ke columns permuted to match
+    half8 A16;
+    half4 A8;
     if (WTS) {
         if (!bskip)
             for (int k = tid; k < 8 * 3 * 64 * 4; k += kThreads) {
@@ -509,11 +516,15 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
             Ab[s] = __builtin_bit_cast(short4v, t);
         }
     } else {
+        // operand q of lane half hh sits at K index 8 (q >> 2) + 4 hh + (q & 3)
+        // of the three-K=8 form, whose Ke column is ke_row_of of it
 #pragma unroll
-        for (int s = 0; s < 3; ++s)
+        for (int q = 0; q < 8; ++q)
+            A16[q] = jc < 24 ? (_Float16)S.a[ke_row_of(jc) * 24 + ke_row_of(8 * (q >> 2) + 4 * hh + (q & 3))]
+                             : (_Float16)0.0f;
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                A[s][i] = jc < 24 ? (_Float16)S.a[ke_row_of(jc) * 24 + ke_row_of(8 * s + 4 * hh + i)] : (_Float16)0.0f;
+        for (int i = 0; i < 4; ++i)
+            A8[i] = jc < 24 ? (_Float16)S.a[ke_row_of(jc) * 24 + ke_row_of(16 + 4 * hh + i)] : (_Float16)0.0f;
     }
     if (!bskip && wt < G.nwt) {
         const int xc = wt / G.nyzb;
@@ -621,14 +632,22 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
             ex = max(ex, -100);       // a finite scale (see k_apply_mfma)
             const float sc = ldexpf(1.0f, 15 - ex);
             half4 B[3];
+            if (WTS) {
 #pragma unroll
-            for (int s = 0; s < 3; ++s)
+                for (int s = 0; s < 3; ++s)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) B[s][i] = (_Float16)(d[4 * s + i] * sc);
+                    for (int i = 0; i < 4; ++i) B[s][i] = (_Float16)(d[4 * s + i] * sc);
+            }
             if (!WTS) {
-                floatx16 acc = {};
+                half8 B16;
+                half4 B8;
 #pragma unroll
-                for (int s = 0; s < 3; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x8f16(A[s], B[s], acc, 0, 0, 0);
+                for (int q = 0; q < 8; ++q) B16[q] = (_Float16)(d[q] * sc);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) B8[i] = (_Float16)(d[8 + i] * sc);
+                floatx16 acc = {};
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(A16, B16, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x8f16(A8, B8, acc, 0, 0, 0);
                 const float f = ecell * ldexpf(1.0f, ex - 15);
 #pragma unroll
                 for (int q = 0; q < 12; ++q) y[q] = acc[q] * f;
