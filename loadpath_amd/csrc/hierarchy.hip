@@ -442,7 +442,9 @@ void level_cell_dofs(const Grid& G, int level, int64_t* out_dev) {
 }
 
 static void alloc_vectors(Grid& G, Level& L) {
-    const size_t n = 3 * L.g.nn * (size_t)G.max_rhs * (size_t)G.vbytes();
+    // a 64-byte tail: the FP32 restriction reads one element past a field's
+    // last row (12-byte row loads) and discards it
+    const size_t n = 3 * L.g.nn * (size_t)G.max_rhs * (size_t)G.vbytes() + 64;
     L.vb.alloc(n);
     L.vx.alloc(n);
     L.vr.alloc(n);
