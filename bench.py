@@ -65,6 +65,8 @@ def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None, keep_
         _lib.check(lib.lp_tune(s.grid._h, 16, int(os.environ["LP_BF16"])), "tune")
     if precision != "fp64" and os.environ.get("LP_L1_EXPLICIT32"):
         _lib.check(lib.lp_tune(s.grid._h, 18, int(os.environ["LP_L1_EXPLICIT32"])), "tune")
+    if precision == "mixed" and os.environ.get("LP_HALF_EXPLICIT"):
+        lib.lp_tune(s.grid._h, 20, int(os.environ["LP_HALF_EXPLICIT"]))
     hist = []
     for k in range(warmup):
         hist += s.iterate(1)

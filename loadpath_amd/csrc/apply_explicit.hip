@@ -321,15 +321,21 @@ k_apply_half(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
         // the 24 operand values of right-hand side r0 + q, gathered one
         // right-hand side ahead of the products (zeros for an inactive one, a
         // cell outside the grid or a masked-out dof: offsets past the buffer)
+        // corners b and b + 4 (bz = 0, 1) are z-neighbours: one 8-byte load
+        // each, the mask applied after it
         auto gather = [&](int q, float (&v)[24]) {
             const int rhs = min(r0 + q, nrhs - 1);
             const bool go = has && q < nr && !(E.scal != nullptr && !E.scal[rhs].active);
             const __amdgpu_buffer_rsrc_t ru = make_rsrc(u0 + (size_t)rhs * vstride, 3u * nn * 4u);
+            typedef unsigned u2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-            for (int b = 0; b < 8; ++b)
+            for (int b = 0; b < 4; ++b)
 #pragma unroll
-                for (int cc = 0; cc < 3; ++cc)
-                    v[3 * b + cc] = bload<float>(ru, (go && ((mk[b] >> cc) & 1)) ? (cc * nn + off[b]) * 4u : kOOB);
+                for (int cc = 0; cc < 3; ++cc) {
+                    const u2 w = __builtin_bit_cast(u2, __builtin_amdgcn_raw_buffer_load_b64(ru, go ? (cc * nn + off[b]) * 4u : kOOB, 0, 0));
+                    v[3 * b + cc] = ((mk[b] >> cc) & 1) ? __uint_as_float(w.x) : 0.0f;
+                    v[3 * (b + 4) + cc] = ((mk[b + 4] >> cc) & 1) ? __uint_as_float(w.y) : 0.0f;
+                }
         };
         auto product = [&](int q, const float (&v)[24]) {
             if (q >= nr) return;
