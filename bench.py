@@ -67,6 +67,8 @@ def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None, keep_
         _lib.check(lib.lp_tune(s.grid._h, 18, int(os.environ["LP_L1_EXPLICIT32"])), "tune")
     if precision == "mixed" and os.environ.get("LP_HALF_EXPLICIT"):
         lib.lp_tune(s.grid._h, 20, int(os.environ["LP_HALF_EXPLICIT"]))
+    if precision == "mixed" and os.environ.get("LP_HALF_BRICK"):
+        lib.lp_tune(s.grid._h, 22, int(os.environ["LP_HALF_BRICK"]))
     hist = []
     for k in range(warmup):
         hist += s.iterate(1)

@@ -520,15 +520,22 @@ int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best);
  * (with param 13): restart the FP32 PCG from each recomputed FP64 residual
  * (1: iterative refinement as gpu.py:429-481's solve_fp32 does it, with
  * the inner tolerance param 15's threshold, 3 for its 3e-3) or carry the
- * direction on (0, the default). Param 20: the mixed V-cycle's Galerkin
- * levels (below the first coarse one) applied as FP16 cell matrices with a
- * power-of-two scale per cell, formed at each hierarchy build (1, the
- * default), those and the first coarse level's sum_q w_q M_q as well (2;
- * measured slower there than the FP16-MFMA weights kernel), or FP32 cell
- * matrices (0); from the next lp_set_moduli. Param 21: the FP64 CG's
+ * direction on (0, the default). Param 20: the mixed V-cycle's coarse
+ * levels applied as FP16 cell matrices with a power-of-two scale per cell,
+ * formed at each hierarchy build: every coarse level, the first one's
+ * sum_q w_q M_q included (2, the default), the Galerkin levels below it
+ * only, the first coarse level through the FP16-MFMA weights kernel (1),
+ * or FP32 cell matrices on the Galerkin levels (0); from the next
+ * lp_set_moduli. Param 21: the FP64 CG's
  * direction and solution updates (p = z + beta p, x += alpha p) done by the
  * next iteration's operator launch as it stages p (1, the default; the same
- * arithmetic, bitwise the same iterates) or by a separate pass (0). */
+ * arithmetic, bitwise the same iterates) or by a separate pass (0).
+ * Param 22: the FP16 cell-matrix levels' operand staged in LDS per brick of
+ * ty x tz nodes marching x-chunks of node planes (value ty * 10000 + tz *
+ * 100 + x-chunk, with ty * tz <= 32 and (ty + 2)(tz + 2) <= 64; 40804,
+ * 4 x 8 nodes and 4 planes, by default; the same FMAs in the same order:
+ * bitwise the same products) or gathered per node (0). Param 23: the brick form only on levels of at least this many
+ * bricks (512 by default; smaller levels keep the gathers). */
 int lp_tune(lp_grid* g, int param, int value);
 
 #ifdef __cplusplus

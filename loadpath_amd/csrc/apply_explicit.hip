@@ -156,13 +156,6 @@ constexpr int kHalfGroups = 9;      // 16-byte groups per corner (72 halves)
 
 typedef unsigned uint4v __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float half_lo(unsigned w) {
-    return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu));
-}
-__device__ __forceinline__ float half_hi(unsigned w) {
-    return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16));
-}
-
 // acc + h * v with h the low (HI = false) or high half of w, as FP16 read by
 // the mixed-precision FMA itself: the matrix stays packed in registers (36
 // instead of 72 VGPRs; a plain conversion is hoisted out of the right-hand
@@ -282,10 +275,6 @@ k_apply_half(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
     // cells whose matrix does not annihilate translations (negative scale):
     // no corner-0 correction
     const float rs_on = sflag < 0.0f ? 0.0f : 1.0f;
-    auto kv = [&](int i, int col) -> float {
-        const int e = 24 * i + col;
-        return (e & 1) ? half_hi(kw[e >> 1]) : half_lo(kw[e >> 1]);
-    };
     // each row's rounded entries summed per component
     float rs[3][3];
 #pragma unroll
@@ -294,7 +283,10 @@ k_apply_half(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
         for (int cc = 0; cc < 3; ++cc) {
             float sacc = 0.0f;
 #pragma unroll
-            for (int b = 0; b < 8; ++b) sacc += kv(i, 3 * b + cc);
+            for (int b = 0; b < 8; ++b) {
+                const int e = 24 * i + 3 * b + cc;
+                sacc = (e & 1) ? fma_half<true>(kw[e >> 1], 1.0f, sacc) : fma_half<false>(kw[e >> 1], 1.0f, sacc);
+            }
             rs[i][cc] = sacc * rs_on;
         }
     unsigned off[8];
@@ -416,6 +408,252 @@ k_apply_half(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
     }
 }
 
+// The same product with the operand staged in LDS: a block owns a brick of
+// ty x tz nodes in (y, z) and marches an x-chunk of node planes. The three
+// node planes a step's two cell layers touch are staged for all 8
+// right-hand sides ((ty + 2) x (tz + 2) nodes each, masked; a ring of 3
+// plane slots), so a thread's 24 operand values per right-hand side are LDS
+// reads instead of 12 gathers from memory, and the plane the next step adds
+// is loaded into registers during this one. Staging: wave w loads
+// right-hand sides 2w and 2w + 1, one (rhs, component) per load with the
+// staged node = lane, so each load's rhs is wave-uniform. Thread (c8, node
+// slot) forms its cell's 3 rows as k_apply_half does (the same FP16 FMAs in
+// the same order: the same results); thread (rhs, node slot) then sums the 8
+// cells and runs the epilogue. Blocks are remapped so each XCD takes a
+// contiguous run of bricks (bz fastest): neighbouring bricks share halo rows
+// and the 128-byte lines of their matrix rows in that XCD's L2.
+constexpr int kHBS = 64;                 // staged nodes per (rhs, component) and plane at most
+struct HBrick {
+    int ty, tz;       // node brick in (y, z): ty * tz <= 32, (ty + 2)(tz + 2) <= 64
+    int nby, nbz, nxc;
+    int xchunk;       // node planes per block
+    int total, per;   // bricks, and bricks per XCD (grid = 8 * per)
+};
+constexpr int kHBW = kXR * 3 / (kBlock / 64);   // (rhs, component) loads per lane and plane (6)
+static_assert(kXN == 32 && kXR == 8 && kBlock == 256, "brick kernel: 32 node slots x 8 cells, 8 right-hand sides");
+
+template <int EK>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 4)))
+k_apply_half_brick(const float* __restrict__ u0, size_t vstride, const uint8_t* __restrict__ mask, AGeom G,
+                   HBrick B, const uint4v* __restrict__ k16, const float* __restrict__ sc16, int nrhs, int rbase,
+                   EpiArgs E) {
+    using namespace apply_detail;
+    constexpr bool STAGE_B = EK == EPI_CHEB_FIRST || EK == EPI_CHEB_ONLY;
+    __shared__ float ring[3][kXR * 3 * kHBS];   // [(rhs, component), staged node] per plane slot
+    __shared__ float red[kXR][8][kXN][3];
+    const int b = (int)blockIdx.x;
+    const int L = (b & 7) * B.per + (b >> 3);
+    const int t = (int)threadIdx.x;
+    if (L >= B.total) {                         // the padding of the last XCD's run
+        if (E.partials != nullptr && t % kXN == 0 && t / kXN < nrhs - rbase)
+            E.partials[(size_t)(rbase + t / kXN) * gridDim.x + blockIdx.x] = 0.0;
+        return;
+    }
+    const int c8 = t / kXN, ns = t - c8 * kXN;
+    const int nbyz = B.nby * B.nbz;
+    const int xc = L / nbyz, rem = L - xc * nbyz;
+    const int by = rem / B.nbz, bz = rem - by * B.nbz;
+    const int Y0 = by * B.ty, Z0 = bz * B.tz;
+    const int Ny = G.Ny, Nz = G.Nz;
+    const int x0 = xc * B.xchunk, x1 = min(x0 + B.xchunk, G.Nx);
+    const int SW = B.tz + 2, NS = (B.ty + 2) * SW;
+    const int nr = min(kXR, nrhs - rbase);
+    const unsigned nn = (unsigned)G.nn;
+    const unsigned nyz = (unsigned)Ny * (unsigned)Nz;
+    const size_t nc = G.ncell;
+    const float s_b = (float)E.c3;
+    const __amdgpu_buffer_rsrc_t rm = make_rsrc(mask, nn);
+
+    // staging: this lane's staged node (the same for every load)
+    // the wave index as a uniform value: the staging loads' buffer
+    // descriptors (one rhs per load) then stay in SGPRs
+    const int lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int sy = lane / SW, sz = lane - (lane / SW) * SW;
+    const int yy = Y0 - 1 + sy, zz = Z0 - 1 + sz;
+    const bool yz_ok = lane < NS && yy >= 0 && yy < Ny && zz >= 0 && zz < Nz;
+    const unsigned yzo = yz_ok ? (unsigned)yy * (unsigned)Nz + (unsigned)zz : 0u;
+    bool r_ok[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int q = 2 * wv + h;
+        r_ok[h] = q < nr && !(E.scal != nullptr && !E.scal[rbase + q].active);
+    }
+    float pv[kHBW];
+    int pm = 0;
+    float pd[STAGE_B ? 3 : 1];
+    auto fetch = [&](int xs) {
+        const bool ok = yz_ok && xs >= 0 && xs < G.Nx;
+        const unsigned n = (unsigned)xs * nyz + yzo;
+        pm = bload_u8(rm, ok ? n : kOOB);
+#pragma unroll
+        for (int j = 0; j < kHBW; ++j) {
+            const int h = j / 3, cc = j - 3 * h;
+            const int rhs = rbase + min(2 * wv + h, nr - 1);
+            const __amdgpu_buffer_rsrc_t ru = make_rsrc(u0 + (size_t)rhs * vstride, 3u * nn * 4u);
+            pv[j] = bload<float>(ru, ok && r_ok[h] ? ((unsigned)cc * nn + n) * 4u : kOOB);
+        }
+        if (STAGE_B) {
+            const __amdgpu_buffer_rsrc_t rd = make_rsrc(E.dinv, 3u * nn * 4u);
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) pd[STAGE_B ? cc : 0] = bload<float>(rd, ok ? ((unsigned)cc * nn + n) * 4u : kOOB);
+        }
+    };
+    auto store = [&](int slot) {
+        if (lane < NS) {
+#pragma unroll
+            for (int j = 0; j < kHBW; ++j) {
+                const int h = j / 3, cc = j - 3 * h;
+                float v = pv[j];
+                if (STAGE_B) v = (pd[STAGE_B ? cc : 0] * v) * s_b;
+                ring[slot][((2 * wv + h) * 3 + cc) * kHBS + lane] = ((pm >> cc) & 1) ? v : 0.0f;
+            }
+        }
+    };
+
+    // this thread's node and cell corner
+    const int ly = ns / B.tz, lz = ns - (ns / B.tz) * B.tz;
+    const int y = Y0 + ly, z = Z0 + lz;
+    const bool node_ok = ns < B.ty * B.tz && y < Ny && z < Nz;
+    const int cx = c8 & 1, cy = (c8 >> 1) & 1, cz = (c8 >> 2) & 1;
+    const int a = 7 - c8;
+    const int CY = y - 1 + cy, CZ = z - 1 + cz;
+    const bool yz_cell = node_ok && CY >= 0 && CZ >= 0 && CY < G.ny && CZ < G.nz;
+    // corner (by, bz) of the cell: staged node (ly + cy + by, lz + cz + bz)
+    const int sbase = (ly + cy) * SW + (lz + cz);
+    // epilogue thread (q, node slot)
+    const int eq = t / kXN, es = ns;
+    const int ey = Y0 + es / B.tz, ez = Z0 + es % B.tz;
+    const bool e_ok = eq < nr && es < B.ty * B.tz && ey < Ny && ez < Nz &&
+                      !(E.scal != nullptr && !E.scal[rbase + min(eq, nr - 1)].active);
+    double part = 0.0;
+
+    // ring slots of planes X - 1, X, X + 1
+    int sl0 = 0, sl1 = 1, sl2 = 2;
+    fetch(x0 - 1);
+    store(sl0);
+    fetch(x0);
+    store(sl1);
+    fetch(x0 + 1);
+    // the cell's 3 rows for step Xs (cell layer Xs - 1 + cx), loaded a step
+    // ahead of their products
+    auto load_k = [&](int Xs, unsigned (&kwd)[4 * kHalfGroups], float& sfl) {
+        const int CX = Xs - 1 + cx;
+        const bool has = yz_cell && CX >= 0 && CX < G.nx;
+        const size_t ci = has ? ((size_t)CX * G.ny + CY) * G.nz + CZ : 0;
+#pragma unroll
+        for (int g = 0; g < kHalfGroups; ++g) {
+            const uint4v w = has ? k16[(size_t)(a * kHalfGroups + g) * nc + ci] : uint4v{0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) kwd[4 * g + j] = w[j];
+        }
+        sfl = has ? sc16[ci] : 0.0f;
+    };
+    unsigned kw[4 * kHalfGroups], kwn[4 * kHalfGroups];
+    float sflag, sfn = 0.0f;
+    load_k(x0, kw, sflag);
+    for (int X = x0; X < x1; ++X) {
+        store(sl2);
+        __syncthreads();                          // planes X - 1 .. X + 1 staged; red free
+        if (X + 1 < x1) {
+            fetch(X + 2);                         // the next step's new plane and
+            load_k(X + 1, kwn, sfn);              // matrix rows, in flight during this step
+        }
+        const int CX = X - 1 + cx;
+        const bool has = yz_cell && CX >= 0 && CX < G.nx;
+        const float scl = fabsf(sflag);
+        const float rs_on = sflag < 0.0f ? 0.0f : 1.0f;
+        float rs[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                float sacc = 0.0f;
+#pragma unroll
+                for (int bb = 0; bb < 8; ++bb) {
+                    const int e = 24 * i + 3 * bb + cc;
+                    sacc = (e & 1) ? fma_half<true>(kw[e >> 1], 1.0f, sacc) : fma_half<false>(kw[e >> 1], 1.0f, sacc);
+                }
+                rs[i][cc] = sacc * rs_on;
+            }
+        // the plane slot of corners bx = 0, 1 (planes CX, CX + 1)
+        const float* P0 = &ring[cx ? sl1 : sl0][sbase];
+        const float* P1 = &ring[cx ? sl2 : sl1][sbase];
+        for (int q = 0; q < nr; ++q) {
+            float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, v0[3] = {0.0f, 0.0f, 0.0f};
+            if (has && !(E.scal != nullptr && !E.scal[rbase + q].active)) {
+#pragma unroll
+                for (int bb = 0; bb < 8; ++bb)
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) {
+                        const float* P = (bb & 1) ? P1 : P0;
+                        const float w = P[(q * 3 + cc) * kHBS + ((bb >> 1) & 1) * SW + ((bb >> 2) & 1)];
+                        if (bb == 0) v0[cc] = w;
+                        const int e = 3 * bb + cc;
+                        if (e & 1) {
+                            t0 = fma_half<true>(kw[e >> 1], w, t0);
+                            t1 = fma_half<true>(kw[(24 + e) >> 1], w, t1);
+                            t2 = fma_half<true>(kw[(48 + e) >> 1], w, t2);
+                        } else {
+                            t0 = fma_half<false>(kw[e >> 1], w, t0);
+                            t1 = fma_half<false>(kw[(24 + e) >> 1], w, t1);
+                            t2 = fma_half<false>(kw[(48 + e) >> 1], w, t2);
+                        }
+                    }
+                t0 -= rs[0][0] * v0[0] + rs[0][1] * v0[1] + rs[0][2] * v0[2];
+                t1 -= rs[1][0] * v0[0] + rs[1][1] * v0[1] + rs[1][2] * v0[2];
+                t2 -= rs[2][0] * v0[0] + rs[2][1] * v0[1] + rs[2][2] * v0[2];
+            }
+            red[q][c8][ns][0] = t0 * scl;
+            red[q][c8][ns][1] = t1 * scl;
+            red[q][c8][ns][2] = t2 * scl;
+        }
+        __syncthreads();                          // products in red; plane X - 1's slot is free
+        if (e_ok) {
+            const int q = eq, rhs = rbase + q;
+            const unsigned nid = (unsigned)X * nyz + (unsigned)ey * (unsigned)Nz + (unsigned)ez;
+            float yv[3];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const float s01 = red[q][0][es][cc] + red[q][1][es][cc], s23 = red[q][2][es][cc] + red[q][3][es][cc];
+                const float s45 = red[q][4][es][cc] + red[q][5][es][cc], s67 = red[q][6][es][cc] + red[q][7][es][cc];
+                yv[cc] = (s01 + s23) + (s45 + s67);
+            }
+            const float* __restrict__ u = u0 + (size_t)rhs * vstride;
+            const int m = mask[nid];
+            const size_t o = (size_t)rhs * vstride + nid;
+            EpiIn<float> in;
+            epi_load<float, EK>(E, o, nn, nid, in);
+            float ur[3];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const float raw = u[(size_t)cc * nn + nid];
+                if (STAGE_B) {
+                    ur[cc] = (in.v[3][cc] * raw) * s_b;
+                    in.v[0][cc] = raw;
+                } else {
+                    ur[cc] = raw;
+                }
+                if (!((m >> cc) & 1)) yv[cc] = ur[cc];
+            }
+            epi_finish<float, EK, float>(E, m, o, nn, yv, ur, in, part);
+        }
+        const int s = sl0;
+        sl0 = sl1;
+        sl1 = sl2;
+        sl2 = s;
+#pragma unroll
+        for (int j = 0; j < 4 * kHalfGroups; ++j) kw[j] = kwn[j];
+        sflag = sfn;
+    }
+    if (E.partials != nullptr) {
+        // the kXN threads of one rhs are one half of a wave; the block's
+        // partial covers its whole brick and x-chunk
+#pragma unroll
+        for (int o = kXN / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+        if (es == 0 && eq < nr) E.partials[(size_t)(rbase + eq) * gridDim.x + blockIdx.x] = part;
+    }
+}
+
 }  // namespace
 
 int explicit_blocks(const Grid& G, int level) {
@@ -464,6 +702,36 @@ void form_half_matrices(const Grid& G, int level, const float* src32, hipStream_
     LP_HIP(hipGetLastError());
 }
 
+// the brick form's geometry when it is selected (tune_half_brick = ty * 10000
+// + tz * 100 + x-chunk, 0 = off) and the level has tune_half_brick_min bricks
+// or more (a brick marches its x-chunk plane by plane: on a small level too
+// few blocks run for that to pay)
+static bool half_brick(const Grid& G, int level, HBrick& B) {
+    const int v = G.tune_half_brick;
+    if (v <= 0) return false;
+    const AGeom A = make_apply_geom(G, level);
+    B.ty = v / 10000;
+    B.tz = (v / 100) % 100;
+    B.xchunk = v % 100;
+    B.nby = (A.Ny + B.ty - 1) / B.ty;
+    B.nbz = (A.Nz + B.tz - 1) / B.tz;
+    B.nxc = (A.Nx + B.xchunk - 1) / B.xchunk;
+    B.total = B.nxc * B.nby * B.nbz;
+    B.per = (B.total + 7) / 8;
+    return B.total >= G.tune_half_brick_min;
+}
+
+bool half_brick_shape_ok(int v) {
+    if (v == 0) return true;
+    const int ty = v / 10000, tz = (v / 100) % 100, xc = v % 100;
+    return v > 0 && ty >= 1 && tz >= 1 && xc >= 1 && ty * tz <= kXN && (ty + 2) * (tz + 2) <= kHBS;
+}
+
+int half_blocks(const Grid& G, int level) {
+    HBrick B;
+    return half_brick(G, level, B) ? 8 * B.per : explicit_blocks(G, level);
+}
+
 void launch_apply_half(const Grid& G, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,
                        hipStream_t s) {
     const Level& lv = G.lv[level];
@@ -473,6 +741,37 @@ void launch_apply_half(const Grid& G, int level, const void* u, size_t vstride, 
     const uint8_t* m = lv.mask.as<uint8_t>();
     const uint4v* k = lv.k16.as<uint4v>();
     const float* sc = lv.sc16.as<float>();
+    HBrick B;
+    if (half_brick(G, level, B)) {
+        const dim3 bgrid(8 * B.per);
+        for (int r0 = 0; r0 < nrhs; r0 += kXR) {
+#define LP_BRICK_CASE(K)                                                                                     \
+    case K:                                                                                                  \
+        hipLaunchKernelGGL((k_apply_half_brick<K>), bgrid, block, 0, s, uf, vstride, m, A, B, k, sc, nrhs, r0, \
+                           E);                                                                               \
+        break;
+            switch (E.kind) {
+                LP_BRICK_CASE(EPI_STORE)
+                LP_BRICK_CASE(EPI_STORE_DOT)
+                LP_BRICK_CASE(EPI_CHEB)
+                LP_BRICK_CASE(EPI_RESID_SCALED)
+                LP_BRICK_CASE(EPI_RESID_MASK)
+                LP_BRICK_CASE(EPI_JACOBI)
+                LP_BRICK_CASE(EPI_POWER)
+                LP_BRICK_CASE(EPI_RESID)
+                LP_BRICK_CASE(EPI_MASKED)
+                LP_BRICK_CASE(EPI_CHEB_FIRST)
+                LP_BRICK_CASE(EPI_CHEB_ONLY)
+                LP_BRICK_CASE(EPI_CHEB_LAST)
+                default:
+                    set_error("level operator: unknown epilogue");
+                    throw ArgFailure();
+            }
+#undef LP_BRICK_CASE
+            LP_HIP(hipGetLastError());
+        }
+        return;
+    }
 #define LP_HALF_CASE(K) \
     case K: hipLaunchKernelGGL((k_apply_half<K>), grid, block, 0, s, uf, vstride, m, A, k, sc, nrhs, E); break;
     switch (E.kind) {

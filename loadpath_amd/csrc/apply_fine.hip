@@ -42,7 +42,7 @@ AGeom make_apply_geom(const Grid& Gd, int level) {
 int fine_vector_blocks(const Grid& Gd) { return 8 * make_apply_geom(Gd, 0).per; }
 
 int apply_blocks(const Grid& Gd, int level, int eb) {
-    if (half_explicit(Gd, level, eb)) return explicit_blocks(Gd, level);
+    if (half_explicit(Gd, level, eb)) return half_blocks(Gd, level);
     if (weights_mfma_path(Gd, level, eb)) return mfma_blocks(Gd, level);
     if (Gd.lv[level].kind == LK_FINE && eb == 4 && Gd.fp16()) return mfma_blocks(Gd, level);
     if (Gd.lv[level].kind == LK_EXPLICIT || explicit_weights(Gd, level, eb)) return explicit_blocks(Gd, level);

@@ -185,9 +185,11 @@ struct Grid {
     int tune_bf16 = 0;          // mixed path, fine level: BF16 operands (1) or scaled FP16 ones (0)
     int tune_l1_explicit = 1;   // FP64 path: first coarse level as formed cell matrices (1) or matrix-free weights (0)
     int tune_l1_explicit32 = 0; // FP32 / mixed V-cycles: the same in FP32 (1) or matrix-free weights (0)
-    int tune_half_explicit = 1; // mixed V-cycle: Galerkin levels (1) / every coarse level (2) as FP16 cell matrices, or none (0)
+    int tune_half_explicit = 2; // mixed V-cycle: every coarse level (2) / the Galerkin levels (1) as FP16 cell matrices, or none (0)
     int tune_cg32_restart = 0;  // FP32 CG: restart at each reliable update (iterative refinement)
     int tune_cg_fused = 1;      // FP64 CG: direction and x updates fused into the next operator launch
+    int tune_half_brick = 40804; // FP16 cell-matrix levels: LDS-staged bricks (ty * 10000 + tz * 100 + x-chunk), 0 = off
+    int tune_half_brick_min = 512; // ... on levels of at least this many bricks
     int tune_flex = -1;         // flexible (Polak-Ribiere) beta: 1 on, 0 off, -1 on for the mixed path only
     bool flex() const { return tune_flex < 0 ? opt.precision == LP_PREC_MIXED : tune_flex != 0; }
     // mixed path: a solve the FP16 levels fail is repeated with FP32
@@ -292,6 +294,11 @@ bool explicit_weights(const Grid& G, int level, int eb);
 bool half_explicit(const Grid& G, int level, int eb);
 void launch_apply_half(const Grid& G, int level, const void* u, size_t vstride, int nrhs, const EpiArgs& E,
                        hipStream_t s);
+// blocks (= partials per rhs) of an FP16 cell-matrix launch (the brick form
+// when tune_half_brick selects it)
+int half_blocks(const Grid& G, int level);
+// tune_half_brick's value is a brick the kernel takes
+bool half_brick_shape_ok(int v);
 // FP16 cell matrices with a per-cell scale from the FP64 weights-level sums
 // (src32 == nullptr) or from an FP32 (576, cells) array
 void form_half_matrices(const Grid& G, int level, const float* src32, hipStream_t s);

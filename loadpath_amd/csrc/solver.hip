@@ -1202,7 +1202,7 @@ int lp_tune(lp_grid* g, int param, int value) {
         return old;
     }
     if (param == 20) {
-        // mixed V-cycle: the Galerkin levels (1) or every coarse level (2) as
+        // mixed V-cycle: every coarse level (2) or the Galerkin levels (1) as
         // FP16 cell matrices (takes effect at the next hierarchy build)
         if (value < 0 || value > 2) return LP_ERR_ARG;
         const int old = g->tune_half_explicit;
@@ -1215,6 +1215,25 @@ int lp_tune(lp_grid* g, int param, int value) {
         if (value != 0 && value != 1) return LP_ERR_ARG;
         const int old = g->tune_cg_fused;
         g->tune_cg_fused = value;
+        return old;
+    }
+    if (param == 22) {
+        // FP16 cell-matrix levels: LDS-staged node bricks of ty x tz nodes
+        // marching x-chunks of planes (value ty * 10000 + tz * 100 +
+        // x-chunk; ty * tz <= 32, (ty + 2)(tz + 2) <= 64), or 0 for the
+        // node-per-thread gathers
+        if (!half_brick_shape_ok(value)) return LP_ERR_ARG;
+        const int old = g->tune_half_brick;
+        g->tune_half_brick = value;
+        ++g->epoch;
+        return old;
+    }
+    if (param == 23) {
+        // the brick form only on levels of at least this many bricks
+        if (value < 0) return LP_ERR_ARG;
+        const int old = g->tune_half_brick_min;
+        g->tune_half_brick_min = value;
+        ++g->epoch;
         return old;
     }
     if (param == 18) {

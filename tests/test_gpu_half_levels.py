@@ -1,9 +1,8 @@
 """The mixed V-cycle's coarse levels as FP16 cell matrices (lp_tune param 20:
-1, the default, the Galerkin levels; 2, the first coarse level's
-sum_q w_q M_q as well, with patch matrices on cells touching held dofs,
-which these tests switch on): each cell's 576
-entries rounded to FP16 after a power-of-two scale of its own, applied with
-FP32 arithmetic. Checked against the FP64 oracle's levels (mg.py:609-677)
+2, the default, every coarse level, the first one's sum_q w_q M_q with
+patch matrices on cells touching held dofs; 1, the Galerkin levels below
+it): each cell's 576 entries rounded to FP16 after a power-of-two scale of
+its own, applied with FP32 arithmetic. Checked against the FP64 oracle's levels (mg.py:609-677)
 to FP16 tolerance, with a large common offset (each row subtracts its
 rounded entries' sum times the cell's corner-0 value, so translations still
 cancel), and through the V-cycle and the solve."""
@@ -28,13 +27,13 @@ def _holed(seed=13):
     return mesh, fixed, f, e
 
 
-def _grids(mesh, fixed, e, half=1, **kw):
+def _grids(mesh, fixed, e, half=2, **kw):
     from loadpath_amd import _lib
     from loadpath_amd.mg import GridElasticity
     from oracle.mg import GridOracle
 
     g = GridElasticity(mesh, 0.3, fixed, precision="mixed", **kw)
-    _lib.lib().lp_tune(g._h, 20, 2 if half else 0)
+    assert _lib.lib().lp_tune(g._h, 20, half) == 2
     g.set_moduli(e)
     o = GridOracle(mesh, 0.3, fixed, vdtype=np.float64, **kw)
     o.set_moduli(e)
@@ -96,7 +95,7 @@ def test_half_levels_vcycle_fused_matches_separate():
 def test_half_levels_solve_matches_direct():
     """A mixed solve through the FP16 matrix levels lands on the direct solve,
     with the CG count of the FP16-MFMA weights / FP32 levels within 2; so
-    does the default (Galerkin levels only)."""
+    do the FP16 Galerkin levels under the FP16-MFMA weights level."""
     from oracle.fem import assemble, hex8_ke
 
     mesh, fixed, f, e = cantilever_bar(20, 10, 10, seed=33)
@@ -106,16 +105,80 @@ def test_half_levels_solve_matches_direct():
     ref = np.zeros(mesh.n_dof)
     ref[free] = spla.spsolve(k[free][:, free].tocsc(), f[free])
     its = []
-    for half in (1, 0, None):
-        if half is None:
-            from loadpath_amd.mg import GridElasticity
-
-            g = GridElasticity(mesh, 0.3, fixed, precision="mixed", **kw)
-            g.set_moduli(e)
-        else:
-            g, _ = _grids(mesh, fixed, e, half=half, **kw)
+    for half in (2, 0, 1):
+        g, _ = _grids(mesh, fixed, e, half=half, **kw)
         x = g.solve_many([f, 2.0 * f])
         assert np.linalg.norm(x[0] - ref) <= 1e-7 * np.linalg.norm(ref), half
         assert np.linalg.norm(x[1] - 2.0 * ref) <= 1e-7 * np.linalg.norm(2.0 * ref), half
         its.append(g.last_iters)
     assert abs(its[0] - its[1]) <= 2 and abs(its[2] - its[1]) <= 2, its
+
+
+@pytest.mark.parametrize("brick", [40808, 50603, 21302])
+def test_half_brick_matches_gathers(brick):
+    """The LDS-staged brick form (lp_tune param 22: ty * 10000 + tz * 100 +
+    x-chunk) makes the same FP16 FMAs in the same order as the
+    node-per-thread gathers, so every level's product, the fused Chebyshev
+    launches of a V-cycle and a batch of 10 right-hand sides (two groups of
+    8) equal the gathers' bit for bit; bricks that do not tile the level
+    (65 = 8 x 8 + 1 nodes along z) and x-chunks of 2 planes included."""
+    from loadpath_amd import _lib
+
+    lib = _lib.lib()
+    mesh, fixed, f, e = cantilever_bar(34, 26, 20, seed=41)
+    e = e.copy()
+    e[np.random.default_rng(42).random(mesh.n_elem) < 0.1] = 0.0
+    e[mesh.elem_ijk[:, 0] == 0] = 1.0
+    from loadpath_amd.mg import GridElasticity
+
+    g = GridElasticity(mesh, 0.3, fixed, precision="mixed", coarse_cells=1, max_rhs=10)
+    assert lib.lp_tune(g._h, 23, 0) == 512     # every level, however small
+    assert lib.lp_tune(g._h, 22, 0) == 40804   # the default brick; gathers first
+    g.set_moduli(e)
+    rng = np.random.default_rng(43)
+    for level in range(1, len(g.levels)):
+        shape = g.levels[level].node_shape
+        u = rng.standard_normal((10,) + tuple(shape)).astype(np.float32)
+        a = g.apply(u, level=level, dtype=np.float32)
+        assert lib.lp_tune(g._h, 22, brick) == 0
+        b = g.apply(u, level=level, dtype=np.float32)
+        assert lib.lp_tune(g._h, 22, 0) == brick
+        assert np.isfinite(a).all()
+        assert np.array_equal(a, b), (level, np.abs(a - b).max())
+    r = rng.standard_normal((10,) + g.levels[0].node_shape) * g.level_mask(0)
+    a = g.precondition(r)
+    lib.lp_tune(g._h, 22, brick)
+    b = g.precondition(r)
+    lib.lp_tune(g._h, 22, 0)
+    assert np.array_equal(a, b), np.abs(a - b).max()
+
+
+def test_half_brick_solve_and_shapes():
+    """A mixed solve with the brick form on from the hierarchy build (its
+    power-iteration partials per brick) lands on the direct solve with the
+    gathers' CG count within 1; shapes the kernel cannot take are refused."""
+    from loadpath_amd import _lib
+    from loadpath_amd.mg import GridElasticity
+    from oracle.fem import assemble, hex8_ke
+
+    lib = _lib.lib()
+    mesh, fixed, f, e = cantilever_bar(20, 10, 10, seed=44)
+    kw = dict(coarse_cells=30, rtol=1e-10, maxiter=2000)
+    k = assemble(mesh, hex8_ke(0.3), e)
+    free = ~fixed
+    ref = np.zeros(mesh.n_dof)
+    ref[free] = spla.spsolve(k[free][:, free].tocsc(), f[free])
+    its = []
+    for brick in (0, 40808):
+        g = GridElasticity(mesh, 0.3, fixed, precision="mixed", **kw)
+        assert lib.lp_tune(g._h, 23, 0) == 512
+        assert lib.lp_tune(g._h, 22, brick) == 40804
+        g.set_moduli(e)
+        x = g.solve_many([f, 2.0 * f])
+        assert np.linalg.norm(x[0] - ref) <= 1e-7 * np.linalg.norm(ref), brick
+        assert np.linalg.norm(x[1] - 2.0 * ref) <= 1e-7 * np.linalg.norm(2.0 * ref), brick
+        its.append(g.last_iters)
+    assert abs(its[0] - its[1]) <= 1, its
+    for bad in (-1, 40800, 40008, 50708, 80808, 13201):
+        assert lib.lp_tune(g._h, 22, bad) < 0, bad
+    assert lib.lp_tune(g._h, 22, 0) == 40808
