@@ -69,6 +69,8 @@ def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None, keep_
         lib.lp_tune(s.grid._h, 20, int(os.environ["LP_HALF_EXPLICIT"]))
     if precision == "mixed" and os.environ.get("LP_HALF_BRICK"):
         lib.lp_tune(s.grid._h, 22, int(os.environ["LP_HALF_BRICK"]))
+    if precision == "mixed" and os.environ.get("LP_CHEB_R"):
+        lib.lp_tune(s.grid._h, 24, int(os.environ["LP_CHEB_R"]))
     hist = []
     for k in range(warmup):
         hist += s.iterate(1)

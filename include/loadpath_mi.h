@@ -535,7 +535,12 @@ int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best);
  * 100 + x-chunk, with ty * tz <= 32 and (ty + 2)(tz + 2) <= 64; 40804,
  * 4 x 8 nodes and 4 planes, by default; the same FMAs in the same order:
  * bitwise the same products) or gathered per node (0). Param 23: the brick form only on levels of at least this many
- * bricks (512 by default; smaller levels keep the gathers). */
+ * bricks (512 by default; smaller levels keep the gathers). Param 24: the
+ * mixed V-cycle's fused Chebyshev smoothers of degree 2 from a nonzero x:
+ * the residual launch stores r only and the last step stages its operand
+ * d = (dinv r) / theta from r (1, the default: one vector write and one
+ * read fewer per right-hand side and level), or the residual launch stores
+ * r and d (0). */
 int lp_tune(lp_grid* g, int param, int value);
 
 #ifdef __cplusplus

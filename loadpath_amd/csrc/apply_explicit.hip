@@ -242,8 +242,10 @@ k_apply_half(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
              const uint4v* __restrict__ k16, const float* __restrict__ sc16, int nrhs, EpiArgs E) {
     using namespace apply_detail;
     constexpr bool need_raw_all = EK == EPI_STORE_DOT || EK == EPI_JACOBI || EK == EPI_POWER;
-    // first Chebyshev step from zero: the operand is d = (dinv b) s of b
-    constexpr bool STAGE_B = EK == EPI_CHEB_FIRST || EK == EPI_CHEB_ONLY;
+    // Chebyshev steps that stage their operand d = (dinv u) s from the vector
+    // u they are given (b for the first step from zero, r for the last step
+    // after a residual-only step)
+    constexpr bool STAGE_B = EK == EPI_CHEB_FIRST || EK == EPI_CHEB_ONLY || EK == EPI_CHEB_LAST_R;
     (void)need_raw_all;
     __shared__ float red[kXR][8][kXN][3];
     const int t = (int)threadIdx.x;
@@ -390,7 +392,7 @@ k_apply_half(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
                 const float raw = u[(size_t)cc * nn + nid];
                 if (STAGE_B) {
                     ur[cc] = (in.v[3][cc] * raw) * s_b;
-                    in.v[0][cc] = raw;
+                    in.v[EK == EPI_CHEB_LAST_R ? 2 : 0][cc] = raw;
                 } else {
                     ur[cc] = raw;
                 }
@@ -438,7 +440,7 @@ k_apply_half_brick(const float* __restrict__ u0, size_t vstride, const uint8_t* 
                    HBrick B, const uint4v* __restrict__ k16, const float* __restrict__ sc16, int nrhs, int rbase,
                    EpiArgs E) {
     using namespace apply_detail;
-    constexpr bool STAGE_B = EK == EPI_CHEB_FIRST || EK == EPI_CHEB_ONLY;
+    constexpr bool STAGE_B = EK == EPI_CHEB_FIRST || EK == EPI_CHEB_ONLY || EK == EPI_CHEB_LAST_R;
     __shared__ float ring[3][kXR * 3 * kHBS];   // [(rhs, component), staged node] per plane slot
     __shared__ float red[kXR][8][kXN][3];
     const int b = (int)blockIdx.x;
@@ -629,7 +631,7 @@ k_apply_half_brick(const float* __restrict__ u0, size_t vstride, const uint8_t* 
                 const float raw = u[(size_t)cc * nn + nid];
                 if (STAGE_B) {
                     ur[cc] = (in.v[3][cc] * raw) * s_b;
-                    in.v[0][cc] = raw;
+                    in.v[EK == EPI_CHEB_LAST_R ? 2 : 0][cc] = raw;
                 } else {
                     ur[cc] = raw;
                 }
@@ -763,6 +765,7 @@ void launch_apply_half(const Grid& G, int level, const void* u, size_t vstride, 
                 LP_BRICK_CASE(EPI_CHEB_FIRST)
                 LP_BRICK_CASE(EPI_CHEB_ONLY)
                 LP_BRICK_CASE(EPI_CHEB_LAST)
+                LP_BRICK_CASE(EPI_CHEB_LAST_R)
                 default:
                     set_error("level operator: unknown epilogue");
                     throw ArgFailure();
@@ -787,6 +790,7 @@ void launch_apply_half(const Grid& G, int level, const void* u, size_t vstride, 
         LP_HALF_CASE(EPI_CHEB_FIRST)
         LP_HALF_CASE(EPI_CHEB_ONLY)
         LP_HALF_CASE(EPI_CHEB_LAST)
+        LP_HALF_CASE(EPI_CHEB_LAST_R)
         default:
             set_error("level operator: unknown epilogue");
             throw ArgFailure();

@@ -63,6 +63,7 @@ static double apply_bytes(const Grid& G, int level, int eb, int nrhs, int epi) {
         case EPI_CHEB_FIRST: vecs = 4; dinv = true; break;    // b, dinv -> x, r, d2
         case EPI_CHEB_ONLY: vecs = 2; dinv = true; break;     // b, dinv -> x
         case EPI_CHEB_LAST: vecs = 4; dinv = true; break;     // d, x, r, dinv -> x
+        case EPI_CHEB_LAST_R: vecs = 3; dinv = true; break;   // r, x, dinv -> x
         case EPI_RESID_SCALED: vecs = 4; dinv = true; break;  // x, b, dinv -> r, d2
         case EPI_RESID_MASK: vecs = 3; break;                 // x, b -> out
         case EPI_JACOBI: vecs = 3; dinv = true; break;        // x, b, dinv -> out

@@ -130,6 +130,10 @@ __device__ __forceinline__ void epi_load(const EpiArgs& E, size_t o, size_t nn, 
             case EPI_CHEB_ONLY:
                 in.v[3][c] = static_cast<const T*>(E.dinv)[c * nn + n];
                 break;
+            case EPI_CHEB_LAST_R:   // r is the raw operand (in.v[2], set by the kernel)
+                in.v[1][c] = static_cast<const T*>(E.x)[i];
+                in.v[3][c] = static_cast<const T*>(E.dinv)[c * nn + n];
+                break;
             case EPI_RESID_SCALED:
             case EPI_JACOBI:
                 in.v[0][c] = static_cast<const T*>(E.b)[i];
@@ -189,7 +193,8 @@ __device__ __forceinline__ void epi_finish(const EpiArgs& E, int m, size_t o, si
                 static_cast<T*>(E.x)[i] = di + dn;
                 break;
             }
-            case EPI_CHEB_LAST: {
+            case EPI_CHEB_LAST:
+            case EPI_CHEB_LAST_R: {
                 const T di = ur[c];
                 const T dn = (T)(c1 * (AR)di + c2 * (AR)in.v[3][c] * (AR)(in.v[2][c] - y_));
                 static_cast<T*>(E.x)[i] = in.v[1][c] + di + dn;

@@ -90,8 +90,10 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
     double part = 0.0;
     const bool skip = (E.scal != nullptr && !E.scal[rhs].active) || logical >= G.nblk;
     constexpr bool need_raw_all = EK == EPI_STORE_DOT || EK == EPI_JACOBI || EK == EPI_POWER;
-    // first Chebyshev step from zero: u is b, the operand is d = (dinv b) s
-    constexpr bool STAGE_B = EK == EPI_CHEB_FIRST || EK == EPI_CHEB_ONLY;
+    // Chebyshev steps that stage their operand d = (dinv u) s from the vector
+    // u they are given (b for the first step from zero, r for the last step
+    // after a residual-only step)
+    constexpr bool STAGE_B = EK == EPI_CHEB_FIRST || EK == EPI_CHEB_ONLY || EK == EPI_CHEB_LAST_R;
     if (!skip) {
         const int ntz = G.ntz;
         const int xc = logical / G.nyzb;
@@ -352,7 +354,7 @@ k_apply_mfma(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
                         // the residual
                         const float b = eraw[c];
                         ur[c] = mc ? q[c * PSTR] : (ein.v[3][c] * b) * s_b;
-                        ein.v[0][c] = b;
+                        ein.v[EK == EPI_CHEB_LAST_R ? 2 : 0][c] = b;
                     } else {
                         ur[c] = (!mc || need_raw_all) ? eraw[c] : q[c * PSTR];
                     }
@@ -488,7 +490,7 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
     double part = 0.0;
     const bool bskip = (E.scal != nullptr && !E.scal[rhs].active) || logical >= G.nblk;
     constexpr bool need_raw_all = EK == EPI_STORE_DOT || EK == EPI_JACOBI || EK == EPI_POWER;
-    constexpr bool STAGE_B = EK == EPI_CHEB_FIRST || EK == EPI_CHEB_ONLY;
+    constexpr bool STAGE_B = EK == EPI_CHEB_FIRST || EK == EPI_CHEB_ONLY || EK == EPI_CHEB_LAST_R;
     const int jc = lane & 31, hh = lane >> 5, r = (lane >> 4) & 1, c = lane & 15;
     half4 A[3];
     short4v Ab[3];
@@ -803,7 +805,7 @@ k_apply_wave(const float* __restrict__ u0, size_t vstride, const uint8_t* __rest
                         if (STAGE_B) {
                             const float b = cur.raw[cc];
                             ur[cc] = mc ? st : (cur.in.v[3][cc] * b) * s_b;
-                            cur.in.v[0][cc] = b;
+                            cur.in.v[EK == EPI_CHEB_LAST_R ? 2 : 0][cc] = b;
                         } else {
                             ur[cc] = (!mc || need_raw_all) ? cur.raw[cc] : st;
                         }
@@ -861,6 +863,7 @@ void launch_mfma(const Grid& Gd, int level, const void* u, size_t vstride, int n
         LP_MFMA_CASE(EPI_CHEB_FIRST)
         LP_MFMA_CASE(EPI_CHEB_ONLY)
         LP_MFMA_CASE(EPI_CHEB_LAST)
+        LP_MFMA_CASE(EPI_CHEB_LAST_R)
         default:
             set_error("level operator: unknown epilogue");
             throw ArgFailure();
@@ -912,6 +915,7 @@ void launch_wave(const Grid& Gd, int level, const void* u, size_t vstride, int n
         LP_WAVE_CASE(EPI_CHEB_FIRST)
         LP_WAVE_CASE(EPI_CHEB_ONLY)
         LP_WAVE_CASE(EPI_CHEB_LAST)
+        LP_WAVE_CASE(EPI_CHEB_LAST_R)
         default:
             set_error("level operator: unknown epilogue");
             throw ArgFailure();
@@ -972,7 +976,7 @@ void launch_apply_weights_mfma(const Grid& Gd, int level, const void* u, size_t 
         }
         S.pprod = lv.patch_prod.as<float>();
         const unsigned nb = (unsigned)((lv.npatch * 24 + kBlock - 1) / kBlock);
-        const bool staged = E.kind == EPI_CHEB_FIRST || E.kind == EPI_CHEB_ONLY;
+        const bool staged = E.kind == EPI_CHEB_FIRST || E.kind == EPI_CHEB_ONLY || E.kind == EPI_CHEB_LAST_R;
         hipLaunchKernelGGL(k_patch_prod, dim3(nb), dim3(kBlock), 0, s, static_cast<const float*>(u), vstride, nrhs,
                            lv.mask.as<uint8_t>(), lv.g, lv.patch_cell.as<int>(), lv.patch32.as<float>(), lv.npatch,
                            staged ? static_cast<const float*>(E.dinv) : nullptr, (float)E.c3,

@@ -190,6 +190,7 @@ struct Grid {
     int tune_cg_fused = 1;      // FP64 CG: direction and x updates fused into the next operator launch
     int tune_half_brick = 40804; // FP16 cell-matrix levels: LDS-staged bricks (ty * 10000 + tz * 100 + x-chunk), 0 = off
     int tune_half_brick_min = 512; // ... on levels of at least this many bricks
+    int tune_cheb_r = 1;        // fused degree-2 smoothers from nonzero x: residual step stores r only
     int tune_flex = -1;         // flexible (Polak-Ribiere) beta: 1 on, 0 off, -1 on for the mixed path only
     bool flex() const { return tune_flex < 0 ? opt.precision == LP_PREC_MIXED : tune_flex != 0; }
     // mixed path: a solve the FP16 levels fail is repeated with FP32
@@ -258,6 +259,10 @@ enum EpiKind {
     // (z = d, of element size zbytes); d2 = p; out = A p; partial += p . Ap;
     // x += alpha p_old where the CG step is pending (alpha, beta from scal)
     EPI_CG_APPLY = 12,
+    // the last Chebyshev step of a smoother whose residual step stored r
+    // only: the operand is r, staged as d = (dinv r) s (s = c3), and the
+    // operand itself is the epilogue's r (FP16-MFMA level operators only)
+    EPI_CHEB_LAST_R = 13, // d = (dinv r) s: x += d + c1 d + c2 dinv (r - y)
 };
 
 struct EpiArgs {
@@ -270,7 +275,7 @@ struct EpiArgs {
     void* d2 = nullptr;
     const void* dinv = nullptr;
     double c1 = 0, c2 = 0;
-    double c3 = 0;                // operand scale s of EPI_CHEB_FIRST / EPI_CHEB_ONLY
+    double c3 = 0;                // operand scale s of EPI_CHEB_FIRST / EPI_CHEB_ONLY / EPI_CHEB_LAST_R
     double* partials = nullptr;   // per (rhs, block)
     const CgScal* scal = nullptr; // skip inactive rhs when set
     const double* scale_dev = nullptr;  // optional per-rhs scale read on device

@@ -210,22 +210,32 @@ static void smooth(Grid& G, int i, int nrhs, bool from_zero) {
         // the same recurrence with its vector passes fused into the operator
         // launches: the first step stages d = (dinv b) / theta itself, the
         // last adds the next direction to x (no start or update passes)
+        const int napply = G.opt.degree - 1;
+        // from a nonzero x with one step to take (degree 2): the residual
+        // step stores r only and the step stages d = (dinv r) / theta from
+        // it, so d is neither written nor read back
+        const bool resid_only = !from_zero && napply == 1 && G.tune_cheb_r != 0;
         if (!from_zero) {
             EpiArgs E;
-            E.kind = EPI_RESID_SCALED;
             E.b = L.vb.p;
-            E.r = L.vr.p;
-            E.d2 = L.vd.p;
             E.dinv = dinv;
-            E.c1 = 1.0 / theta;
+            if (resid_only) {
+                E.kind = EPI_RESID;
+                E.out = L.vr.p;
+            } else {
+                E.kind = EPI_RESID_SCALED;
+                E.r = L.vr.p;
+                E.d2 = L.vd.p;
+                E.c1 = 1.0 / theta;
+            }
             launch_apply(G, i, eb, L.vx.p, n, nrhs, E, s);
         }
-        const int napply = G.opt.degree - 1;
         for (int it = 0; it < napply; ++it) {
             const bool first = from_zero && it == 0, last = it + 1 == napply;
             const double rho_new = 1.0 / (2.0 * sigma - rho);
             EpiArgs E;
             E.kind = first ? (last ? EPI_CHEB_ONLY : EPI_CHEB_FIRST) : (last ? EPI_CHEB_LAST : EPI_CHEB);
+            if (resid_only) E.kind = EPI_CHEB_LAST_R;
             E.x = L.vx.p;
             E.r = L.vr.p;
             E.d2 = L.vd2.p;
@@ -233,7 +243,7 @@ static void smooth(Grid& G, int i, int nrhs, bool from_zero) {
             E.c1 = rho_new * rho;
             E.c2 = 2.0 * rho_new / delta;
             E.c3 = 1.0 / theta;
-            launch_apply(G, i, eb, first ? L.vb.p : L.vd.p, n, nrhs, E, s);
+            launch_apply(G, i, eb, first ? L.vb.p : (resid_only ? L.vr.p : L.vd.p), n, nrhs, E, s);
             if (!last) swap_buf(L.vd, L.vd2);
             rho = rho_new;
         }
@@ -1233,6 +1243,16 @@ int lp_tune(lp_grid* g, int param, int value) {
         if (value < 0) return LP_ERR_ARG;
         const int old = g->tune_half_brick_min;
         g->tune_half_brick_min = value;
+        ++g->epoch;
+        return old;
+    }
+    if (param == 24) {
+        // fused Chebyshev smoothers of degree 2 from a nonzero x: the
+        // residual step stores r only and the last step stages d from it
+        // (1, the default), or the residual step stores r and d (0)
+        if (value != 0 && value != 1) return LP_ERR_ARG;
+        const int old = g->tune_cheb_r;
+        g->tune_cheb_r = value;
         ++g->epoch;
         return old;
     }

@@ -92,6 +92,38 @@ def test_half_levels_vcycle_fused_matches_separate():
     assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max()
 
 
+@pytest.mark.parametrize("half", [2, 1])
+def test_cheb_residual_only_step(half):
+    """Degree-2 smoothers from a nonzero x (lp_tune param 24): the residual
+    launch storing r only and the last step staging d = (dinv r) / theta
+    from it equal the residual launch storing r and d, to FP32 rounding of
+    d (then FP16 operands), on every level kind: the fine FP16-MFMA level,
+    the first coarse level as FP16 matrices (half 2) or the FP16-MFMA
+    weights kernel with its patch products (half 1), the Galerkin levels;
+    and the solve still lands on the direct solve."""
+    from loadpath_amd import _lib
+    from oracle.fem import assemble, hex8_ke
+
+    lib = _lib.lib()
+    mesh, fixed, f, e = _holed(seed=51)
+    g, o = _grids(mesh, fixed, e, half=half, coarse_cells=1)
+    r = np.random.default_rng(52).standard_normal((3,) + g.levels[0].node_shape) * g.level_mask(0)
+    a = g.precondition(r)
+    assert lib.lp_tune(g._h, 24, 0) == 1
+    b = g.precondition(r)
+    assert lib.lp_tune(g._h, 24, 1) == 0
+    assert np.isfinite(a).all()
+    assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max()
+    mesh, fixed, f, e = cantilever_bar(20, 10, 10, seed=53)
+    k = assemble(mesh, hex8_ke(0.3), e)
+    free = ~fixed
+    ref = np.zeros(mesh.n_dof)
+    ref[free] = spla.spsolve(k[free][:, free].tocsc(), f[free])
+    g, _ = _grids(mesh, fixed, e, half=half, coarse_cells=30, rtol=1e-10, maxiter=2000)
+    x = g.solve_many([f])
+    assert np.linalg.norm(x[0] - ref) <= 1e-7 * np.linalg.norm(ref)
+
+
 def test_half_levels_solve_matches_direct():
     """A mixed solve through the FP16 matrix levels lands on the direct solve,
     with the CG count of the FP16-MFMA weights / FP32 levels within 2; so
