@@ -65,12 +65,13 @@ def run_gpu(precision, steps, warmup, nx, ny, nz, profile=False, log=None, keep_
         _lib.check(lib.lp_tune(s.grid._h, 16, int(os.environ["LP_BF16"])), "tune")
     if precision != "fp64" and os.environ.get("LP_L1_EXPLICIT32"):
         _lib.check(lib.lp_tune(s.grid._h, 18, int(os.environ["LP_L1_EXPLICIT32"])), "tune")
-    if precision == "mixed" and os.environ.get("LP_HALF_EXPLICIT"):
-        lib.lp_tune(s.grid._h, 20, int(os.environ["LP_HALF_EXPLICIT"]))
-    if precision == "mixed" and os.environ.get("LP_HALF_BRICK"):
-        lib.lp_tune(s.grid._h, 22, int(os.environ["LP_HALF_BRICK"]))
-    if precision == "mixed" and os.environ.get("LP_CHEB_R"):
-        lib.lp_tune(s.grid._h, 24, int(os.environ["LP_CHEB_R"]))
+    # LP_TUNE="param:value,...": lp_tune settings of the mixed path's grid
+    # (kernel-variant comparisons; the defaults are the measured best)
+    if precision == "mixed" and os.environ.get("LP_TUNE"):
+        for item in os.environ["LP_TUNE"].split(","):
+            pv = item.split(":")
+            if lib.lp_tune(s.grid._h, int(pv[0]), int(pv[1])) < 0:
+                raise SystemExit(f"LP_TUNE: lp_tune({pv[0]}, {pv[1]}) refused")
     hist = []
     for k in range(warmup):
         hist += s.iterate(1)

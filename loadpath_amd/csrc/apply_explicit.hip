@@ -528,6 +528,21 @@ k_apply_half_brick(const float* __restrict__ u0, size_t vstride, const uint8_t* 
     const bool e_ok = eq < nr && es < B.ty * B.tz && ey < Ny && ez < Nz &&
                       !(E.scal != nullptr && !E.scal[rbase + min(eq, nr - 1)].active);
     double part = 0.0;
+    // the epilogue's inputs at node plane Xs, loaded a step ahead of their
+    // use (after the step before's epilogue) so their latency hides behind a
+    // whole step
+    int em = 0;
+    float eraw[3] = {0.0f, 0.0f, 0.0f};
+    EpiIn<float> ein;
+    auto epi_pre = [&](int Xs) {
+        if (!e_ok || Xs >= x1) return;
+        const unsigned nid = (unsigned)Xs * nyz + (unsigned)ey * (unsigned)Nz + (unsigned)ez;
+        const size_t o = (size_t)(rbase + eq) * vstride + nid;
+        em = mask[nid];
+        epi_load<float, EK>(E, o, nn, nid, ein);
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) eraw[cc] = u0[o + (size_t)cc * nn];
+    };
 
     // ring slots of planes X - 1, X, X + 1
     int sl0 = 0, sl1 = 1, sl2 = 2;
@@ -536,8 +551,7 @@ k_apply_half_brick(const float* __restrict__ u0, size_t vstride, const uint8_t* 
     fetch(x0);
     store(sl1);
     fetch(x0 + 1);
-    // the cell's 3 rows for step Xs (cell layer Xs - 1 + cx), loaded a step
-    // ahead of their products
+    // the cell's 3 rows for step Xs (cell layer Xs - 1 + cx)
     auto load_k = [&](int Xs, unsigned (&kwd)[4 * kHalfGroups], float& sfl) {
         const int CX = Xs - 1 + cx;
         const bool has = yz_cell && CX >= 0 && CX < G.nx;
@@ -550,16 +564,14 @@ k_apply_half_brick(const float* __restrict__ u0, size_t vstride, const uint8_t* 
         }
         sfl = has ? sc16[ci] : 0.0f;
     };
-    unsigned kw[4 * kHalfGroups], kwn[4 * kHalfGroups];
-    float sflag, sfn = 0.0f;
-    load_k(x0, kw, sflag);
+    unsigned kw[4 * kHalfGroups];
+    float sflag;
+    epi_pre(x0);
     for (int X = x0; X < x1; ++X) {
         store(sl2);
         __syncthreads();                          // planes X - 1 .. X + 1 staged; red free
-        if (X + 1 < x1) {
-            fetch(X + 2);                         // the next step's new plane and
-            load_k(X + 1, kwn, sfn);              // matrix rows, in flight during this step
-        }
+        load_k(X, kw, sflag);
+        if (X + 1 < x1) fetch(X + 2);             // the next step's new plane, in flight during this one
         const int CX = X - 1 + cx;
         const bool has = yz_cell && CX >= 0 && CX < G.nx;
         const float scl = fabsf(sflag);
@@ -620,32 +632,27 @@ k_apply_half_brick(const float* __restrict__ u0, size_t vstride, const uint8_t* 
                 const float s45 = red[q][4][es][cc] + red[q][5][es][cc], s67 = red[q][6][es][cc] + red[q][7][es][cc];
                 yv[cc] = (s01 + s23) + (s45 + s67);
             }
-            const float* __restrict__ u = u0 + (size_t)rhs * vstride;
-            const int m = mask[nid];
+            const int m = em;
             const size_t o = (size_t)rhs * vstride + nid;
-            EpiIn<float> in;
-            epi_load<float, EK>(E, o, nn, nid, in);
             float ur[3];
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) {
-                const float raw = u[(size_t)cc * nn + nid];
+                const float raw = eraw[cc];
                 if (STAGE_B) {
-                    ur[cc] = (in.v[3][cc] * raw) * s_b;
-                    in.v[EK == EPI_CHEB_LAST_R ? 2 : 0][cc] = raw;
+                    ur[cc] = (ein.v[3][cc] * raw) * s_b;
+                    ein.v[EK == EPI_CHEB_LAST_R ? 2 : 0][cc] = raw;
                 } else {
                     ur[cc] = raw;
                 }
                 if (!((m >> cc) & 1)) yv[cc] = ur[cc];
             }
-            epi_finish<float, EK, float>(E, m, o, nn, yv, ur, in, part);
+            epi_finish<float, EK, float>(E, m, o, nn, yv, ur, ein, part);
         }
+        epi_pre(X + 1);
         const int s = sl0;
         sl0 = sl1;
         sl1 = sl2;
         sl2 = s;
-#pragma unroll
-        for (int j = 0; j < 4 * kHalfGroups; ++j) kw[j] = kwn[j];
-        sflag = sfn;
     }
     if (E.partials != nullptr) {
         // the kXN threads of one rhs are one half of a wave; the block's
