@@ -15,6 +15,16 @@ void launch_apply_fine(const Grid& G, int level, int eb, const void* u, size_t v
 
 namespace lp {
 
+// x-chunk of a fine-level kernel whose launch takes `units` blocks (or
+// waves) per chunk: 16 node planes unless that leaves fewer than `target`
+// of them, then 8 or 4 (a small grid would otherwise run a few dozen blocks,
+// each marching its planes one after another)
+int size_xchunk(int nx, int units, int target) {
+    int xc = 16;
+    while (xc > 4 && (long long)units * ((nx + xc - 1) / xc) < target) xc /= 2;
+    return xc;
+}
+
 AGeom make_apply_geom(const Grid& Gd, int level) {
     const Level& lv = Gd.lv[level];
     AGeom G{};
@@ -25,8 +35,8 @@ AGeom make_apply_geom(const Grid& Gd, int level) {
     G.ncell = lv.g.ncell;
     if (lv.kind == LK_FINE) {
         G.ny_per = 1;
-        G.xchunk = Gd.tune_fine_xchunk > 0 ? Gd.tune_fine_xchunk : 16;
         G.nyzb = (G.Ny * G.Nz + kBlock - 1) / kBlock;
+        G.xchunk = Gd.tune_fine_xchunk > 0 ? Gd.tune_fine_xchunk : size_xchunk(G.Nx, G.nyzb, kSizeXChunkBlocks);
         G.nxc = (G.Nx + G.xchunk - 1) / G.xchunk;
         G.nblk = G.nyzb * G.nxc;
         G.per = (G.nblk + 7) / 8;

@@ -82,6 +82,8 @@ template <typename T> struct CSrc {
 };
 
 AGeom make_apply_geom(const Grid& Gd, int level);
+int size_xchunk(int nx, int units, int target);
+constexpr int kSizeXChunkBlocks = 1024;   // blocks (FP64 / FP32 fine kernel) or waves (wave-tile kernel)
 
 namespace apply_detail {
 

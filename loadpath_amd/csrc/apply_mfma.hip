@@ -829,6 +829,8 @@ AGeom mfma_geom(const Grid& Gd, int level) {
     const int nty = (G.Ny + TN - 1) / TN, ntz = (G.Nz + TN - 1) / TN;
     G.ntz = ntz;
     G.nyzb = nty * ntz;
+    if (Gd.lv[level].kind == LK_FINE && Gd.tune_fine_xchunk <= 0)
+        G.xchunk = size_xchunk(G.Nx, G.nyzb * (kThreads / 64), kSizeXChunkBlocks);
     G.nxc = (G.Nx + G.xchunk - 1) / G.xchunk;
     G.nblk = G.nyzb * G.nxc;
     G.per = (G.nblk + 7) / 8;
@@ -880,6 +882,8 @@ AGeom wave_geom(const Grid& Gd, int level) {
     const int nty = (G.Ny + WN - 1) / WN, ntz = (G.Nz + WN - 1) / WN;
     G.ntz = ntz;
     G.nyzb = nty * ntz;
+    if (Gd.lv[level].kind == LK_FINE && Gd.tune_fine_xchunk <= 0)
+        G.xchunk = size_xchunk(G.Nx, G.nyzb, kSizeXChunkBlocks);
     G.nxc = (G.Nx + G.xchunk - 1) / G.xchunk;
     G.nwt = G.nyzb * G.nxc;
     G.nblk = (G.nwt + kWaveTiles - 1) / kWaveTiles;

@@ -26,66 +26,71 @@ constexpr int NB = 64;
 
 // Factor the kb x kb diagonal block at (k, k) in place (lower), write
 // inv(L_kk) to dinv (NB x NB, row-major). fail set on a non-positive pivot.
-__global__ void __launch_bounds__(256) k_potrf_diag(double* a, int n, int k, int kb, double* dinv, int* fail) {
-    __shared__ double s[NB][NB + 1];
-    __shared__ double inv[NB][NB + 1];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < kb * kb; e += blockDim.x) {
-        const int i = e / kb, j = e % kb;
-        s[i][j] = a[(size_t)(k + i) * n + k + j];
+//
+// One wave. Lane i holds row i of the block in registers (rows and columns
+// past kb are the identity's, so the padded block factors to the identity).
+// Column j's step (right-looking): lane j posts the pivot to LDS, every lane
+// below it divides its entry by the pivot's root and posts the multiplier
+// L_ij (0 above the diagonal), and each lane updates its row's entries
+// m > j by L_ij L_mj with L_mj read back as an LDS broadcast (j and m
+// compile-time: both loops unrolled; the entries past the diagonal are
+// updated too, never read and zeroed at the end, so no lane masks). L then goes
+// to LDS and lane c forms column c of inv(L) row by row, reading L's rows as
+// broadcasts: inv[i][c] = (delta_ic - sum_{m < i} L[i][m] inv[m][c]) / L[i][i].
+__global__ void __launch_bounds__(64) k_potrf_diag(double* a, int n, int k, int kb, double* dinv, int* fail) {
+    __shared__ double L[NB][NB + 1];
+    const int i = (int)threadIdx.x;
+    // lane i's row: every load issued before the first is waited for
+    // (indices clamped into the block, the padding selected afterwards)
+    double r[NB];
+    {
+        const double* arow = a + (size_t)(k + min(i, kb - 1)) * n + k;
+#pragma unroll
+        for (int m = 0; m < NB; ++m) {
+            const double v = arow[min(m, kb - 1)];
+            r[m] = i < kb && m < kb ? v : (i == m ? 1.0 : 0.0);
+        }
     }
-    __shared__ double red[4][NB];
+    __shared__ double col[NB + 1];
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (i == j) col[NB] = r[j];
+        __syncthreads();
+        const double djj = col[NB];
+        const bool ok = djj > 0.0;
+        bad |= !ok;
+        const double piv = ok ? sqrt(djj) : 1.0;
+        const double lij = i > j ? r[j] / piv : 0.0;
+        col[i] = lij;
+        r[j] = i > j ? lij : (i == j ? piv : r[j]);
+        __syncthreads();
+#pragma unroll
+        for (int m = j + 1; m < NB; ++m) r[m] = fma(-lij, col[m], r[m]);
+    }
+    if (bad && i == 0) *fail = 1;
+#pragma unroll
+    for (int m = 0; m < NB; ++m) L[i][m] = m <= i ? r[m] : 0.0;
     __syncthreads();
-    // left-looking, one column a step: thread (part, i) adds every fourth
-    // term of L[i, :j] . L[j, :j]; the four partial sums meet in LDS (a fixed
-    // order), every thread forms the pivot of column j from row j's sum, and
-    // row i > j divides by it
-    {
-        const int i = tid & (NB - 1), part = tid >> 6;
-        for (int j = 0; j < kb; ++j) {
-            double acc = 0.0;
-            if (i >= j && i < kb)
-                for (int m = part; m < j; m += 4) acc += s[i][m] * s[j][m];
-            red[part][i] = acc;
-            const double ajj = s[j][j];   // read before thread j overwrites it below
-            __syncthreads();
-            if (tid < kb && tid >= j) {
-                const double dj = ajj - ((red[0][j] + red[1][j]) + (red[2][j] + red[3][j]));
-                const bool ok = dj > 0.0;
-                const double piv = ok ? sqrt(dj) : 1.0;
-                if (tid == j) {
-                    if (!ok) *fail = 1;
-                    s[j][j] = piv;
-                } else {
-                    s[tid][j] = (s[tid][j] - ((red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]))) / piv;
-                }
-            }
-            __syncthreads();
-        }
+#pragma unroll 8
+    for (int e = i; e < kb * kb; e += 64) {
+        const int row = e / kb, col = e - (e / kb) * kb;
+        a[(size_t)(k + row) * n + k + col] = L[row][col];
     }
-    // inv(L) by rows: inv[i][c] = (delta_ic - sum_{c <= m < i} L[i][m] inv[m][c]) / L[i][i];
-    // thread (part, c) adds every fourth term of column c, the four partial
-    // sums meet in LDS (a fixed order)
-    {
-        const int c = tid & (NB - 1), part = tid >> 6;
-        for (int i = 0; i < kb; ++i) {
-            double acc = 0.0;
-            if (c < i)
-                for (int m = c + part; m < i; m += 4) acc += s[i][m] * inv[m][c];
-            red[part][c] = acc;
-            __syncthreads();
-            if (tid < kb) {
-                const double v = (i == tid ? 1.0 : 0.0) - ((red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]));
-                inv[i][tid] = tid <= i ? v / s[i][i] : 0.0;
-            }
-            __syncthreads();
-        }
+    // (four partial sums over m mod 4, so the row's terms are not one
+    // dependent chain)
+    const int c = i;
+    double v[NB];
+#pragma unroll
+    for (int row = 0; row < NB; ++row) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int m = 0; m < row; ++m) acc[m & 3] = fma(L[row][m], v[m], acc[m & 3]);
+        const double sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        v[row] = c <= row ? ((row == c ? 1.0 : 0.0) - sum) / L[row][row] : 0.0;
     }
-    for (int e = tid; e < kb * kb; e += blockDim.x) {
-        const int i = e / kb, j = e % kb;
-        a[(size_t)(k + i) * n + k + j] = j <= i ? s[i][j] : 0.0;
-        dinv[i * NB + j] = inv[i][j];
-    }
+#pragma unroll
+    for (int row = 0; row < NB; ++row) dinv[row * NB + c] = v[row];
 }
 
 __global__ void k_copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols) {
@@ -174,7 +179,7 @@ bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s) {
     for (int b = 0; b < nblk; ++b) {
         const int k = b * NB, kb = std::min(NB, n - k);
         double* Db = D + (size_t)b * NB * NB;
-        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, s, a, n, k, kb, Db, fail.as<int>());
+        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, s, a, n, k, kb, Db, fail.as<int>());
         LP_HIP(hipGetLastError());
         const int rest = n - k - kb;
         if (rest > 0) {
@@ -215,7 +220,7 @@ bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s) {
 }
 
 void launch_potrf_diag(double* a, int lda, int k, int kb, double* dinv, int* fail, hipStream_t s) {
-    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, s, a, lda, k, kb, dinv, fail);
+    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, s, a, lda, k, kb, dinv, fail);
     LP_HIP(hipGetLastError());
 }
 

@@ -176,7 +176,7 @@ struct Grid {
     bool ke_sym = false;        // Ke has the cube's symmetry (ke_pattern.h): orbit values used
     double ke_orbit[32] = {};
     int tune_fine_ny = 1;       // fine-level nodes per thread along y (1 or 2)
-    int tune_fine_xchunk = 16;  // fine-level nodes per thread along x
+    int tune_fine_xchunk = 0;   // fine-level nodes per thread along x (0: from the grid's size)
     int tune_weights_xchunk = 8; // weights-level node planes per block (FP16-MFMA kernels)
     int tune_fused_cheb = 1;    // MFMA levels: Chebyshev vector passes fused into the operator
     int tune_graph = 1;         // PCG replays the V-cycle from a captured hipGraph
