@@ -228,6 +228,76 @@ def config1_bench(iterations, cpu_iterations, log=None):
     return out
 
 
+# The reference's own published timings on this path (its author's 4-core
+# CPU): the MG-PCG solve of a solid 128 x 64 x 64 cantilever (524,288 cells,
+# rtol 1e-6, 11 CG iterations; docs/BENCHMARKS.md:788) and the two golden
+# top3d runs (sparse direct solves; the 'seconds' and 'iterations' of
+# benchmarks/reference/{cantilever-3d,bridge-3d}.top3d.npz).
+PUBLISHED = {
+    "mg_solve_solid_cantilever_128x64x64": {"seconds": 1.46, "cg_iters": 11, "source": "docs/BENCHMARKS.md:788"},
+    "top3d_cantilever-3d": {"seconds": 142.0, "iterations": 133,
+                            "source": "benchmarks/reference/cantilever-3d.top3d.npz"},
+    "top3d_bridge-3d": {"seconds": 761.0, "iterations": 172, "source": "benchmarks/reference/bridge-3d.top3d.npz"},
+}
+
+
+def published_bench(log=None):
+    """The runs behind the reference's published CPU timings, here:
+    - the MG-PCG solve of a solid cantilever 128 x 64 x 64 (x = 0 clamped,
+      the x = 128 bottom edge loaded down; unit moduli) at rtol 1e-6 through
+      GridElasticity's host API (set_moduli + solve_many, host arrays in and
+      out), the V-cycle in FP32 (the reference's default) and mixed, after
+      one untimed solve;
+    - the golden top3d runs (plain element, vf and rmin of each, FP64 device
+      loop with multigrid PCG to 1e-12, warm-started) for the golden's
+      iteration count, with the final compliance beside the golden's."""
+    import numpy as np
+    import torch
+
+    from loadpath_amd.mg import GridElasticity
+    from loadpath_amd.problems import bridge_3d, cantilever_3d
+    from loadpath_amd.simp import GridSimp
+
+    out = {}
+    p = cantilever_3d(128, 64, 64)
+    mesh = p.mesh()
+    fixed, loads = p.dof_vectors(mesh)
+    e = np.ones(mesh.n_elem)
+    for prec in ("fp32", "mixed"):
+        g = GridElasticity(mesh, 0.3, fixed, rtol=1e-6, maxiter=500, precision=prec)
+        g.set_moduli(e)
+        g.solve_many([loads[0]])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.set_moduli(e)
+        u = g.solve_many([loads[0]])[0]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[f"mg_solve_solid_cantilever_128x64x64_{prec}"] = dict(
+            seconds=round(dt, 4), cg_iters=int(g.last_iters), compliance=float(loads[0] @ u),
+            published_cpu_seconds=PUBLISHED["mg_solve_solid_cantilever_128x64x64"]["seconds"],
+            published_cg_iters=PUBLISHED["mg_solve_solid_cantilever_128x64x64"]["cg_iters"],
+            speedup=round(PUBLISHED["mg_solve_solid_cantilever_128x64x64"]["seconds"] / dt, 1))
+        del g
+    for name, make in (("cantilever-3d", cantilever_3d), ("bridge-3d", bridge_3d)):
+        pub = PUBLISHED[f"top3d_{name}"]
+        s = GridSimp(make(), element="plain", rtol=1e-12, maxiter=4000, coarse_cells=500, precision="fp64")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h = s.iterate(pub["iterations"])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[f"top3d_{name}"] = dict(iterations=pub["iterations"], seconds=round(dt, 3),
+                                    iter_s=round(pub["iterations"] / dt, 2), compliance_last=h[-1]["compliance"],
+                                    published_cpu_seconds=pub["seconds"],
+                                    published_iter_s=round(pub["iterations"] / pub["seconds"], 3),
+                                    speedup=round(pub["seconds"] / dt, 1))
+        del s
+    if log:
+        log(f"  published runs: {out}")
+    return out
+
+
 def run_studies_bench(log=None, max_iterations=40, loop=False):
     """The 32 uneven studies (problems.batched_studies, default_rng(5)):
     all at once on their own streams (mixed path, graph-replayed V-cycles,
@@ -416,6 +486,9 @@ def main():
     ap.add_argument("--config1-cpu-iters", type=int, default=50,
                     help="oracle iterations of the 48x24x24 cantilever timed on the host cores (0: none)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-published", action="store_true",
+                    help="skip the runs behind the reference's published CPU timings (the 128x64x64 MG solve, the "
+                         "golden top3d runs)")
     ap.add_argument("--size", type=int, nargs=3, default=None, help="override the grid (testing)")
     ap.add_argument("-v", "--verbose", action="store_true")
     ap.add_argument("--studies", action="store_true",
@@ -502,6 +575,7 @@ def main():
                     compute=dict(achieved=round(tfs, 2), peak=FP16_MFMA_PEAK_TFLOPS, unit="TFLOP/s",
                                  frac=round(tfs / FP16_MFMA_PEAK_TFLOPS, 4), flop_per_cell=1176))
     cfg1 = None if a.no_config1 else config1_bench(50, 0 if a.no_cpu else a.config1_cpu_iters, log)
+    pubr = None if a.no_published else published_bench(log)
     cpu = None if a.no_cpu else cpu_baseline(mixed["timed"], nx, ny, nz, log)
     ms = 1e3 * mixed["seconds"] / a.steps
     out = {
@@ -532,6 +606,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "config1": cfg1,
+        "published_runs": pubr,
     }
     print(json.dumps(out))
 
