@@ -7,10 +7,10 @@
 // coarsest solve becomes one dense FP64 matvec with the inverse (no host
 // round trip; a few thousand dofs).
 //
-// A = L L^T (right-looking, 64-wide panels): a one-workgroup kernel factors
-// the diagonal block in LDS and inverts it; the panel is a GEMM with that
-// inverse; the trailing update a GEMM. Then Linv by block rows and
-// A^-1 = Linv^T Linv.
+// A = L L^T (right-looking, 64-wide panels): a one-wave kernel factors the
+// diagonal block, a forward substitution per row solves the panel below it in
+// place, the trailing update is a GEMM. Then every diagonal block's inverse
+// at once, Linv by block rows (two GEMMs each) and A^-1 = Linv^T Linv.
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
@@ -24,8 +24,24 @@ namespace {
 
 constexpr int NB = 64;
 
-// Factor the kb x kb diagonal block at (k, k) in place (lower), write
-// inv(L_kk) to dinv (NB x NB, row-major). fail set on a non-positive pivot.
+// inv(L) of the lower-triangular NB x NB block in LDS (identity past its
+// size), column c per lane: inv[i][c] = (delta_ic - sum_{m < i} L[i][m]
+// inv[m][c]) / L[i][i], L's rows read as LDS broadcasts (four partial sums
+// over m mod 4, so a row's terms are not one dependent chain)
+__device__ __forceinline__ void invert_lower(const double (*L)[NB + 1], int c, double (&v)[NB]) {
+#pragma unroll
+    for (int row = 0; row < NB; ++row) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int m = 0; m < row; ++m) acc[m & 3] = fma(L[row][m], v[m], acc[m & 3]);
+        const double sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        v[row] = c <= row ? ((row == c ? 1.0 : 0.0) - sum) / L[row][row] : 0.0;
+    }
+}
+
+// Factor the kb x kb diagonal block at (k, k) in place (lower); with INV,
+// write inv(L_kk) to dinv (NB x NB, row-major). fail set on a non-positive
+// pivot.
 //
 // One wave. Lane i holds row i of the block in registers (rows and columns
 // past kb are the identity's, so the padded block factors to the identity).
@@ -34,9 +50,9 @@ constexpr int NB = 64;
 // L_ij (0 above the diagonal), and each lane updates its row's entries
 // m > j by L_ij L_mj with L_mj read back as an LDS broadcast (j and m
 // compile-time: both loops unrolled; the entries past the diagonal are
-// updated too, never read and zeroed at the end, so no lane masks). L then goes
-// to LDS and lane c forms column c of inv(L) row by row, reading L's rows as
-// broadcasts: inv[i][c] = (delta_ic - sum_{m < i} L[i][m] inv[m][c]) / L[i][i].
+// updated too, never read and zeroed at the end, so no lane masks). Then
+// L goes through LDS to memory (and to invert_lower).
+template <bool INV>
 __global__ void __launch_bounds__(64) k_potrf_diag(double* a, int n, int k, int kb, double* dinv, int* fail) {
     __shared__ double L[NB][NB + 1];
     const int i = (int)threadIdx.x;
@@ -77,27 +93,65 @@ __global__ void __launch_bounds__(64) k_potrf_diag(double* a, int n, int k, int 
         const int row = e / kb, col = e - (e / kb) * kb;
         a[(size_t)(k + row) * n + k + col] = L[row][col];
     }
-    // (four partial sums over m mod 4, so the row's terms are not one
-    // dependent chain)
-    const int c = i;
-    double v[NB];
+    if (INV) {
+        double v[NB];
+        invert_lower(L, i, v);
 #pragma unroll
-    for (int row = 0; row < NB; ++row) {
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int m = 0; m < row; ++m) acc[m & 3] = fma(L[row][m], v[m], acc[m & 3]);
-        const double sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-        v[row] = c <= row ? ((row == c ? 1.0 : 0.0) - sum) / L[row][row] : 0.0;
+        for (int row = 0; row < NB; ++row) dinv[row * NB + i] = v[row];
     }
-#pragma unroll
-    for (int row = 0; row < NB; ++row) dinv[row * NB + c] = v[row];
 }
 
-__global__ void k_copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= rows * cols) return;
-    const int i = e / cols, j = e % cols;
-    dst[(size_t)i * ldd + j] = src[(size_t)i * lds + j];
+// L21 = A21 inv(L11)^T in place for the rest rows below the factored
+// diagonal block at (k, k) (kb = NB): thread = one row x of the panel,
+// forward substitution x_j = (a_j - sum_{m < j} x_m L_jm) / L_jj with L11 read
+// as LDS broadcasts (four partial sums over m mod 4)
+__global__ void __launch_bounds__(64) k_trsm_panel(double* a, int n, int k, int rest) {
+    __shared__ double L[NB][NB + 1];
+#pragma unroll 16
+    for (int e = (int)threadIdx.x; e < NB * NB; e += 64) {
+        const int row = e / NB, col = e - (e / NB) * NB;
+        L[row][col] = a[(size_t)(k + row) * n + k + col];
+    }
+    __syncthreads();
+    const int r = (int)(blockIdx.x * 64 + threadIdx.x);
+    double* arow = a + (size_t)(k + NB + min(r, rest - 1)) * n + k;
+    double x[NB];
+#pragma unroll
+    for (int m = 0; m < NB; ++m) x[m] = arow[m];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int m = 0; m < j; ++m) acc[m & 3] = fma(x[m], L[j][m], acc[m & 3]);
+        x[j] = (x[j] - ((acc[0] + acc[1]) + (acc[2] + acc[3]))) / L[j][j];
+    }
+    if (r < rest)
+#pragma unroll
+        for (int m = 0; m < NB; ++m) arow[m] = x[m];
+}
+
+// inv(L_bb) of every factored diagonal block b (grid = the blocks, one wave
+// each) into dinv (NB x NB per block) and into the diagonal blocks of linv
+// (n x n)
+__global__ void __launch_bounds__(64) k_trtri_blocks(const double* a, int n, double* dinv, double* linv) {
+    __shared__ double L[NB][NB + 1];
+    const int b = (int)blockIdx.x, k = b * NB, kb = min(NB, n - k);
+#pragma unroll 16
+    for (int e = (int)threadIdx.x; e < NB * NB; e += 64) {
+        const int row = e / NB, col = e - (e / NB) * NB;
+        L[row][col] = row < kb && col < kb ? (col <= row ? a[(size_t)(k + row) * n + k + col] : 0.0)
+                                           : (row == col ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    const int c = (int)threadIdx.x;
+    double v[NB];
+    invert_lower(L, c, v);
+    double* Db = dinv + (size_t)b * NB * NB;
+#pragma unroll
+    for (int row = 0; row < NB; ++row) {
+        Db[row * NB + c] = v[row];
+        if (row < kb && c < kb) linv[(size_t)(k + row) * n + k + c] = v[row];
+    }
 }
 
 __global__ void k_zero_upper(double* a, int n) {
@@ -165,12 +219,10 @@ bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s) {
     const int nblk = (n + NB - 1) / NB;
     DBuf& dinv = w.dn_dinv;
     DBuf& fail = w.dn_fail;
-    DBuf& panel = w.dn_panel;
     DBuf& linv = w.dn_linv;
     DBuf& t = w.dn_t;
     dinv.alloc((size_t)nblk * NB * NB * 8);
     fail.alloc(sizeof(int));
-    panel.alloc((size_t)n * NB * 8);
     linv.alloc((size_t)n * n * 8);
     t.alloc((size_t)n * n * 8);
     LP_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), s));
@@ -178,38 +230,34 @@ bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s) {
     double* D = dinv.as<double>();
     for (int b = 0; b < nblk; ++b) {
         const int k = b * NB, kb = std::min(NB, n - k);
-        double* Db = D + (size_t)b * NB * NB;
-        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, s, a, n, k, kb, Db, fail.as<int>());
+        hipLaunchKernelGGL(k_potrf_diag<false>, dim3(1), dim3(64), 0, s, a, n, k, kb, (double*)nullptr,
+                           fail.as<int>());
         LP_HIP(hipGetLastError());
         const int rest = n - k - kb;
         if (rest > 0) {
-            // panel L21 = A21 inv(L11)^T  (rest x kb)
-            gemm(bh, rest, kb, kb, 1.0, a + (size_t)(k + kb) * n + k, n, 0, Db, NB, 1, 0.0, panel.as<double>(), NB);
-            const int cnt = rest * kb;
-            hipLaunchKernelGGL(k_copy2d, dim3((cnt + 255) / 256), dim3(256), 0, s, panel.as<double>(), NB,
-                               a + (size_t)(k + kb) * n + k, n, rest, kb);
+            // panel L21 = A21 inv(L11)^T in place (rest x NB: a block with
+            // rows below it is a full one)
+            hipLaunchKernelGGL(k_trsm_panel, dim3((unsigned)((rest + 63) / 64)), dim3(64), 0, s, a, n, k, rest);
             LP_HIP(hipGetLastError());
             // trailing A22 -= L21 L21^T
-            gemm(bh, rest, rest, kb, -1.0, panel.as<double>(), NB, 0, panel.as<double>(), NB, 1, 1.0,
-                 a + (size_t)(k + kb) * n + (k + kb), n);
+            const double* l21 = a + (size_t)(k + kb) * n + k;
+            gemm(bh, rest, rest, kb, -1.0, l21, n, 0, l21, n, 1, 1.0, a + (size_t)(k + kb) * n + (k + kb), n);
         }
     }
     hipLaunchKernelGGL(k_zero_upper, dim3((unsigned)(((size_t)n * n + 255) / 256)), dim3(256), 0, s, a, n);
     LP_HIP(hipGetLastError());
-    // Linv by block rows: Linv[i, :i] = -inv(L_ii) (L[i, :i] Linv[:i, :i]); Linv[i,i] = inv(L_ii)
+    // inv(L_ii) of every diagonal block at once (into D and Linv's diagonal),
+    // then Linv by block rows: Linv[i, :i] = -inv(L_ii) (L[i, :i] Linv[:i, :i])
     double* Li = linv.as<double>();
-    for (int b = 0; b < nblk; ++b) {
+    hipLaunchKernelGGL(k_trtri_blocks, dim3((unsigned)nblk), dim3(64), 0, s, a, n, D, Li);
+    LP_HIP(hipGetLastError());
+    for (int b = 1; b < nblk; ++b) {
         const int k = b * NB, kb = std::min(NB, n - k);
         double* Db = D + (size_t)b * NB * NB;
-        hipLaunchKernelGGL(k_copy2d, dim3((kb * kb + 255) / 256), dim3(256), 0, s, Db, NB, Li + (size_t)k * n + k, n,
-                           kb, kb);
-        LP_HIP(hipGetLastError());
-        if (k > 0) {
-            // T (kb x k) = L[k:k+kb, :k] Linv[:k, :k]
-            gemm(bh, kb, k, k, 1.0, a + (size_t)k * n, n, 0, Li, n, 0, 0.0, t.as<double>(), n);
-            // Linv[k:k+kb, :k] = -inv(L_kk) T
-            gemm(bh, kb, k, kb, -1.0, Db, NB, 0, t.as<double>(), n, 0, 0.0, Li + (size_t)k * n, n);
-        }
+        // T (kb x k) = L[k:k+kb, :k] Linv[:k, :k]
+        gemm(bh, kb, k, k, 1.0, a + (size_t)k * n, n, 0, Li, n, 0, 0.0, t.as<double>(), n);
+        // Linv[k:k+kb, :k] = -inv(L_kk) T
+        gemm(bh, kb, k, kb, -1.0, Db, NB, 0, t.as<double>(), n, 0, 0.0, Li + (size_t)k * n, n);
     }
     // A^-1 = Linv^T Linv
     gemm(bh, n, n, n, 1.0, Li, n, 1, Li, n, 0, 0.0, a, n);
@@ -220,7 +268,7 @@ bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s) {
 }
 
 void launch_potrf_diag(double* a, int lda, int k, int kb, double* dinv, int* fail, hipStream_t s) {
-    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, s, a, lda, k, kb, dinv, fail);
+    hipLaunchKernelGGL(k_potrf_diag<true>, dim3(1), dim3(64), 0, s, a, lda, k, kb, dinv, fail);
     LP_HIP(hipGetLastError());
 }
 
