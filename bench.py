@@ -36,7 +36,7 @@ FP16_MFMA_PEAK_TFLOPS = 2500.0   # dense FP16 MFMA, MI355X
 # each of its four epilogue kinds): FETCH_SIZE x 2 + WRITE_SIZE x 1, the
 # factors from a 1 GiB copy in the same run; from
 # profiles/counters_multiload_256x128x128.txt
-FINE_MFMA_TRAFFIC_BYTES = 2.2218e9
+FINE_MFMA_TRAFFIC_BYTES = 2.0775e9
 WORKLOAD = dict(nx=256, ny=128, nz=128, cases=8)
 FINE_MFMA_KERNEL = ("k_apply_wave<K, false> (V-cycle fine-level operator: FP16 MFMA contraction, node sums "
                     "across lanes in registers, fused Chebyshev / residual epilogues in FP32)")
