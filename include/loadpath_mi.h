@@ -475,7 +475,9 @@ int lp_autotune(lp_grid* g, int nrhs, int reps, int* xchunk, double* ms_best);
 
 /* Launch-shape tuning (no reference counterpart: the autotuner's knob).
  * param 0: fine-level nodes per thread along y (1 or 2); 1: fine-level
- * nodes per thread along x (x-chunk, 1..64); 2: on the FP16-MFMA levels of
+ * nodes per thread along x (x-chunk, 1..64; until set, 16, 8 or 4 from the
+ * grid's size, so that a launch has about 1,024 blocks or waves; returns
+ * the previous setting, 0 for that default); 2: on the FP16-MFMA levels of
  * the mixed path, fuse the Chebyshev vector passes into the operator
  * launches (1, the default) or run them as separate passes (0); 3: replay
  * the PCG's V-cycle from a captured hipGraph (1, the default with the
