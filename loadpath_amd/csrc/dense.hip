@@ -237,6 +237,7 @@ bool dense_spd_inverse(double* a, int n, Scratch& w, hipStream_t s) {
         if (rest > 0) {
             // panel L21 = A21 inv(L11)^T in place (rest x NB: a block with
             // rows below it is a full one)
+            if (kb != NB) throw std::logic_error("dense factor: a partial diagonal block with rows below it");
             hipLaunchKernelGGL(k_trsm_panel, dim3((unsigned)((rest + 63) / 64)), dim3(64), 0, s, a, n, k, rest);
             LP_HIP(hipGetLastError());
             // trailing A22 -= L21 L21^T
