@@ -334,6 +334,10 @@ void launch_cast_dot(const void* v, int vb, const double* r, double* z, size_t n
                      double* partials, const CgScal* scal, hipStream_t s, const double* ap = nullptr);
 // flexible beta from the two sums of launch_cast_dot(..., ap)
 void launch_cg_beta_flex(Grid& G, int nrhs, int nb, hipStream_t s);
+// v /= sqrt(sum of dotp[0, nbd)), that sum into *sv, and (prev given) the
+// sum of prev[0, nbp) into *sp, in one launch (one right-hand side)
+void launch_norm_scale(void* v, int elem_bytes, size_t n, const double* dotp, int nbd, double* sv,
+                       const double* prev, int nbp, double* sp, hipStream_t s);
 void launch_scale_by_dev(void* v, int elem_bytes, size_t n, size_t stride, int nrhs,
                          const double* num, size_t num_stride, int mode, hipStream_t s);
 void launch_cheb_start(const void* b, const void* dinv, double s_, void* x, void* r, void* d,

@@ -149,21 +149,26 @@ static double estimate_lmax_level(Grid& G, int i, const double* start_host) {
     const void* dinv = eb == 8 ? L.dinv64.p : L.dinv32.p;
     const int nbd = dot_blocks(n);
     const int nba = apply_blocks(G, i, eb);
-    ensure_partials(G, std::max(nbd, nba));
+    ensure_partials(G, (size_t)nbd + nba);
     double* S = sc.as<double>();
+    // per step: v . v (partials Pd), then one launch that stores its sum,
+    // normalizes v and sums the previous step's v . (dinv A v) partials (Pa),
+    // then the operator; the last step's partials are summed after the loop
+    double* Pd = G.partials.as<double>();
+    double* Pa = Pd + nbd;
     for (int it = 0; it < its; ++it) {
-        launch_dot(L.lmax_vec.p, L.lmax_vec.p, eb, n, n, 1, G.partials.as<double>(), nullptr, s);
-        launch_reduce(G.partials.as<double>(), nbd, 1, S + 2 * it, 0, s);
-        launch_scale_by_dev(L.lmax_vec.p, eb, n, n, 1, S + 2 * it, 0, 1, s);
+        launch_dot(L.lmax_vec.p, L.lmax_vec.p, eb, n, n, 1, Pd, nullptr, s);
+        launch_norm_scale(L.lmax_vec.p, eb, n, Pd, nbd, S + 2 * it, it > 0 ? Pa : nullptr, nba,
+                          it > 0 ? S + 2 * it - 1 : nullptr, s);
         EpiArgs E;
         E.kind = EPI_POWER;
         E.out = av.p;
         E.dinv = dinv;
-        E.partials = G.partials.as<double>();
+        E.partials = Pa;
         launch_apply(G, i, eb, L.lmax_vec.p, n, 1, E, s);
-        launch_reduce(G.partials.as<double>(), nba, 1, S + 2 * it + 1, 0, s);
         swap_buf(L.lmax_vec, av);
     }
+    launch_reduce(Pa, nba, 1, S + 2 * its - 1, 0, s);
     std::vector<double> h(2 * its);
     LP_HIP(hipMemcpyAsync(h.data(), S, h.size() * 8, hipMemcpyDeviceToHost, s));
     LP_HIP(hipStreamSynchronize(s));

@@ -80,6 +80,38 @@ __global__ void k_scale_dev(T* v, size_t n, size_t stride, const double* num, si
     }
 }
 
+// one power step's normalization in one launch: every block sums the dot
+// partials of v . v in k_reduce's order (the same bits in every block and as
+// k_reduce's), block 0 stores that sum in *sv and, when `prev` is given, the
+// sum of the previous step's partials in *sp (k_reduce's order too), and
+// each block divides its share of v by the root of the sum (k_scale_dev's
+// mode 1: the reduce and scale launches of a power step, and the reduce of
+// the step before, as one)
+template <typename T>
+__global__ void k_norm_scale(T* v, size_t n, const double* __restrict__ dotp, int nbd, double* sv,
+                             const double* __restrict__ prev, int nbp, double* sp) {
+    __shared__ double sred[4];
+    double acc = 0;
+    for (int i = threadIdx.x; i < nbd; i += blockDim.x) acc += dotp[i];
+    const double s = block_sum(acc, sred);
+    __shared__ double ss;
+    if (threadIdx.x == 0) ss = s;
+    __syncthreads();
+    const double sum = ss;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) *sv = sum;
+        if (prev != nullptr) {
+            double a = 0;
+            for (int i = threadIdx.x; i < nbp; i += blockDim.x) a += prev[i];
+            const double t = block_sum(a, sred);
+            if (threadIdx.x == 0) *sp = t;
+        }
+    }
+    const T f = (T)sqrt(sum);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        v[i] = f != T(0) ? v[i] / f : v[i];
+}
+
 // x = 0; r = b; d = (dinv b) s   (mgops._cheb_start, mgops.py:320-327)
 template <typename T>
 __global__ void k_cheb_start(const T* __restrict__ b, const T* __restrict__ dinv, T s, T* __restrict__ x,
@@ -475,6 +507,16 @@ void launch_fill(void* dst, int eb, size_t n, double v, hipStream_t s) {
     if (eb == 8) hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(kBlock), 0, s, (double*)dst, n, v);
     else if (eb == 4) hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(kBlock), 0, s, (float*)dst, n, (float)v);
     else hipLaunchKernelGGL(k_fill<uint8_t>, dim3(grid_for(n)), dim3(kBlock), 0, s, (uint8_t*)dst, n, (uint8_t)v);
+    LP_HIP(hipGetLastError());
+}
+
+void launch_norm_scale(void* v, int eb, size_t n, const double* dotp, int nbd, double* sv, const double* prev,
+                       int nbp, double* sp, hipStream_t s) {
+    dim3 grid(grid_for(n));
+    if (eb == 8)
+        hipLaunchKernelGGL(k_norm_scale<double>, grid, dim3(kBlock), 0, s, (double*)v, n, dotp, nbd, sv, prev, nbp, sp);
+    else
+        hipLaunchKernelGGL(k_norm_scale<float>, grid, dim3(kBlock), 0, s, (float*)v, n, dotp, nbd, sv, prev, nbp, sp);
     LP_HIP(hipGetLastError());
 }
 
