@@ -486,6 +486,8 @@ def main():
     ap.add_argument("--config1-cpu-iters", type=int, default=50,
                     help="oracle iterations of the 48x24x24 cantilever timed on the host cores (0: none)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--config1-only", action="store_true",
+                    help="instead: only the 48x24x24 cantilever leg (config1 in the default line), as its own JSON")
     ap.add_argument("--no-published", action="store_true",
                     help="skip the runs behind the reference's published CPU timings (the 128x64x64 MG solve, the "
                          "golden top3d runs)")
@@ -515,6 +517,9 @@ def main():
         return
     if a.constrained:
         print(json.dumps(run_constrained_bench(a.steps, a.warmup, a.no_cpu, log)))
+        return
+    if a.config1_only:
+        print(json.dumps(config1_bench(50, 0 if a.no_cpu else a.config1_cpu_iters, log)))
         return
     nx, ny, nz = a.size or (WORKLOAD["nx"], WORKLOAD["ny"], WORKLOAD["nz"])
     if a.loop:
